@@ -1,0 +1,138 @@
+/*
+ * nrfft.h -- C ABI of the MI355X (gfx950) FFT library for the numrs FFT path.
+ *
+ * Conventions of the numrs routines this library replaces:
+ *   - complex data are interleaved re,im; lengths are powers of two;
+ *   - isign = +1 is the kernel exp(+2 pi i jk/N), isign = -1 exp(-2 pi i jk/N);
+ *   - inverses are unnormalised: four1 forward then inverse gives N*x,
+ *     realft forward then inverse gives (n/2)*x.
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * nrfft_status (0 = success, negative = error) and does not throw.
+ *
+ * Two groups of entry points:
+ *   1. Host-pointer calls with the reference signatures (drop-in): data live
+ *      in host memory; the call stages them through HBM, runs the HIP kernels
+ *      and returns when the result is back in host memory.
+ *   2. Device-pointer plans: data already resident in HBM, execution enqueued
+ *      on the caller's HIP stream (passed as void*, a hipStream_t; NULL = the
+ *      default stream).  A plan is not to be executed concurrently on two
+ *      streams (it owns one workspace).
+ * All calls are thread-safe.
+ */
+#ifndef NRFFT_H
+#define NRFFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum nrfft_status {
+    NRFFT_OK = 0,
+    NRFFT_E_ISIGN = -1,          /* isign not +1/-1 (ConvlvError::InvalidIsign, Fourn.rs:371-373) */
+    NRFFT_E_DIMS = -2,           /* ndim == 0, ndim > len(nn), or a dimension <= 1 (Fourn.rs:368-376) */
+    NRFFT_E_NOT_POW2 = -3,       /* a length that is not a power of two */
+    NRFFT_E_LEN = -4,            /* length mismatch (CorrelError::LengthMismatch, Correlation.rs:14-16) */
+    NRFFT_E_EMPTY = -5,          /* empty input (ConvlvError/CorrelError::EmptyInput) */
+    NRFFT_E_RESP_TOO_LONG = -6,  /* m > n (ConvlvError::ResponseTooLong, Convolve.rs:16-18) */
+    NRFFT_E_HIP = -7,            /* a HIP runtime call failed (see nrfft_last_error) */
+    NRFFT_E_NOMEM = -8,          /* device or host allocation failed */
+    NRFFT_E_UNSUPPORTED = -9,    /* size beyond what this build plans for */
+    NRFFT_E_INVALID = -10        /* null pointer / bad plan / bad argument */
+} nrfft_status;
+
+typedef enum nrfft_dtype { NRFFT_F32 = 0, NRFFT_F64 = 1 } nrfft_dtype;
+
+/* ---- library ------------------------------------------------------------ */
+const char* nrfft_version(void);
+const char* nrfft_status_string(int status);
+/* Message of the last failing call on this thread ("" if none). */
+const char* nrfft_last_error(void);
+/* Select the HIP device used by this thread and build its twiddle tables
+ * (FP64, generated on the GPU).  Optional: every call initialises lazily. */
+int nrfft_init(int device);
+
+/* ---- 1. host-pointer drop-in calls ------------------------------------- */
+
+/* four1: FFT_1.rs:5 `pub fn four1(data: &mut [f64], nn: usize, isign: i32)`.
+ * In place on 2*nn values.  nn must be a power of two (1 allowed). */
+int nrfft_four1_f64(double* data, size_t nn, int isign);
+int nrfft_four1_f32(float* data, size_t nn, int isign);
+
+/* FFTProcessor::fft_batch (FFT_1.rs:185-189) for `batch` transforms of equal
+ * length nn stored back to back (2*nn*batch values). */
+int nrfft_four1_batch_f64(double* data, size_t nn, size_t batch, int isign);
+int nrfft_four1_batch_f32(float* data, size_t nn, size_t batch, int isign);
+
+/* realft: Real_FT.rs:4 `pub fn realft(data: &mut [f64], n: usize, isign: i32)`.
+ * In place on n values.  isign=+1: n reals -> packed spectrum
+ * [F_0, F_{n/2}, Re F_1, Im F_1, ...], F_k = sum_j x_j e^{+2 pi i jk/n};
+ * isign=-1: packed spectrum -> (n/2) x.  n must be a power of two >= 2. */
+int nrfft_realft_f64(double* data, size_t n, int isign);
+int nrfft_realft_f32(float* data, size_t n, int isign);
+
+/* fourn: the in-memory NR fourn that Real_FT3.rs:35 calls as
+ * Fourn(&mut data, &nn, ndim, isign) (no such function exists in the
+ * reference; Fourn.rs:56 is a data-less stub).  C order: nn[0] slowest.
+ * Validation follows Fourn.rs:367-378; dimensions must be powers of two. */
+int nrfft_fourn_f64(double* data, const size_t* nn, size_t ndim, int isign);
+int nrfft_fourn_f32(float* data, const size_t* nn, size_t ndim, int isign);
+
+/* convlv: Convolve.rs:8 `pub fn convlv(data: &[f64], respns: &[f64], isign: i32)
+ * -> Result<Array1<f64>, ConvlvError>`.  Writes n values to ans.
+ * isign=+1 convolution, -1 deconvolution (bins with |R|^2 < 1e-12 -> 0).
+ * The response is placed as Convolve.rs:41-63 places it. */
+int nrfft_convlv_f64(const double* data, size_t n, const double* respns, size_t m, int isign, double* ans);
+int nrfft_convlv_f32(const float* data, size_t n, const float* respns, size_t m, int isign, float* ans);
+
+/* convlv_batch (Convolve.rs:241-250), extended to one response per signal:
+ * signal b is data[b*n .. b*n+n), its response respns[b*m .. b*m+m). */
+int nrfft_convlv_batch_f64(const double* data, size_t n, const double* respns, size_t m, size_t batch,
+                           int isign, double* ans);
+int nrfft_convlv_batch_f32(const float* data, size_t n, const float* respns, size_t m, size_t batch,
+                           int isign, float* ans);
+
+/* correl: Correlation.rs:8 `pub fn correl(data1: &[f64], data2: &[f64])
+ * -> Result<Array1<f64>, CorrelError>`.  n <= 32: linear correlation at lags
+ * 0..n-1 (Correlation.rs:37-50); n > 32 (power of two): circular correlation,
+ * negative lags wrapped to the top (NR correl). */
+int nrfft_correl_f64(const double* data1, const double* data2, size_t n, double* ans);
+int nrfft_correl_f32(const float* data1, const float* data2, size_t n, float* ans);
+
+/* ---- 2. device-pointer plans ------------------------------------------ */
+typedef struct nrfft_plan_s* nrfft_plan;
+
+/* C2C: `batch` transforms of shape nn[0..ndim) (C order) stored back to back.
+ * ndim == 1 is a batched four1, ndim >= 2 a batched fourn. */
+int nrfft_plan_c2c(nrfft_plan* plan, int dtype, const size_t* nn, size_t ndim, size_t batch);
+/* Out of place or in place (d_out == d_in). */
+int nrfft_execute_c2c(nrfft_plan plan, const void* d_in, void* d_out, int isign, void* stream);
+
+/* realft on `batch` signals of n reals, in place. */
+int nrfft_plan_realft(nrfft_plan* plan, int dtype, size_t n, size_t batch);
+int nrfft_execute_realft(nrfft_plan plan, void* d_data, int isign, void* stream);
+
+/* convlv over `batch` signals of length n, one response of length m per
+ * signal (d_respns holds batch*m values).  Fused R2C -> spectrum
+ * multiply/divide -> C2R in one launch when n/2 fits one work-group. */
+int nrfft_plan_convlv(nrfft_plan* plan, int dtype, size_t n, size_t m, size_t batch);
+int nrfft_execute_convlv(nrfft_plan plan, const void* d_data, const void* d_respns, void* d_ans, int isign,
+                         void* stream);
+
+/* correl over `batch` pairs of length n. */
+int nrfft_plan_correl(nrfft_plan* plan, int dtype, size_t n, size_t batch);
+int nrfft_execute_correl(nrfft_plan plan, const void* d_data1, const void* d_data2, void* d_ans, void* stream);
+
+int nrfft_plan_destroy(nrfft_plan plan);
+/* Bytes of HBM workspace the plan owns. */
+size_t nrfft_plan_workspace_bytes(nrfft_plan plan);
+/* Number of kernel launches one execution enqueues. */
+int nrfft_plan_num_launches(nrfft_plan plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NRFFT_H */

@@ -1,0 +1,38 @@
+"""numrs_amd -- the numrs FFT path (four1, fourn, realft, convlv, correl) as
+hand-written HIP kernels for MI355X (gfx950), behind the C ABI of
+include/nrfft.h.
+
+``api`` mirrors the reference's Rust functions and processors on host numpy
+arrays; ``device`` exposes the device-pointer plans used with data resident
+in HBM.
+"""
+from .api import (  # noqa: F401
+    ConvlvError,
+    ConvlvProcessor,
+    CorrelError,
+    FFTProcessor,
+    FournError,
+    NrfftError,
+    RealFTProcessor,
+    autocorrel,
+    autocorrel_fast,
+    complex_to_real,
+    convlv,
+    convlv_batch,
+    convlv_batch_responses,
+    correl,
+    correl_batch,
+    correl_normalized,
+    correl_normalized_fast,
+    four1,
+    four1_optimized,
+    fourn,
+    magnitude_spectrum,
+    power_spectrum,
+    real_to_complex,
+    realft,
+    realft_optimized,
+)
+from .device import C2CPlan, ConvlvPlan, CorrelPlan, RealFTPlan  # noqa: F401
+
+__version__ = "0.1.0"

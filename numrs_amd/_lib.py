@@ -1,0 +1,105 @@
+"""ctypes binding of numrs_amd/libnrfft.so (the C ABI declared in include/nrfft.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (make -C
+numrs_amd/csrc).  There is no fallback: if the shared library is missing the
+import of any compute entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnrfft.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "nrfft.h")
+
+# Status codes (include/nrfft.h).
+OK = 0
+E_ISIGN = -1
+E_DIMS = -2
+E_NOT_POW2 = -3
+E_LEN = -4
+E_EMPTY = -5
+E_RESP_TOO_LONG = -6
+E_HIP = -7
+E_NOMEM = -8
+E_UNSUPPORTED = -9
+E_INVALID = -10
+
+F32 = 0
+F64 = 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+_szp = ctypes.POINTER(ctypes.c_size_t)
+_i = ctypes.c_int
+
+_SIGS = {
+    "nrfft_version": (ctypes.c_char_p, []),
+    "nrfft_status_string": (ctypes.c_char_p, [_i]),
+    "nrfft_last_error": (ctypes.c_char_p, []),
+    "nrfft_init": (_i, [_i]),
+    "nrfft_four1_f64": (_i, [_dp, _sz, _i]),
+    "nrfft_four1_f32": (_i, [_fp, _sz, _i]),
+    "nrfft_four1_batch_f64": (_i, [_dp, _sz, _sz, _i]),
+    "nrfft_four1_batch_f32": (_i, [_fp, _sz, _sz, _i]),
+    "nrfft_realft_f64": (_i, [_dp, _sz, _i]),
+    "nrfft_realft_f32": (_i, [_fp, _sz, _i]),
+    "nrfft_fourn_f64": (_i, [_dp, _szp, _sz, _i]),
+    "nrfft_fourn_f32": (_i, [_fp, _szp, _sz, _i]),
+    "nrfft_convlv_f64": (_i, [_dp, _sz, _dp, _sz, _i, _dp]),
+    "nrfft_convlv_f32": (_i, [_fp, _sz, _fp, _sz, _i, _fp]),
+    "nrfft_convlv_batch_f64": (_i, [_dp, _sz, _dp, _sz, _sz, _i, _dp]),
+    "nrfft_convlv_batch_f32": (_i, [_fp, _sz, _fp, _sz, _sz, _i, _fp]),
+    "nrfft_correl_f64": (_i, [_dp, _dp, _sz, _dp]),
+    "nrfft_correl_f32": (_i, [_fp, _fp, _sz, _fp]),
+    "nrfft_plan_c2c": (_i, [ctypes.POINTER(_vp), _i, _szp, _sz, _sz]),
+    "nrfft_execute_c2c": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "nrfft_plan_realft": (_i, [ctypes.POINTER(_vp), _i, _sz, _sz]),
+    "nrfft_execute_realft": (_i, [_vp, _vp, _i, _vp]),
+    "nrfft_plan_convlv": (_i, [ctypes.POINTER(_vp), _i, _sz, _sz, _sz]),
+    "nrfft_execute_convlv": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
+    "nrfft_plan_correl": (_i, [ctypes.POINTER(_vp), _i, _sz, _sz]),
+    "nrfft_execute_correl": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "nrfft_plan_destroy": (_i, [_vp]),
+    "nrfft_plan_workspace_bytes": (_sz, [_vp]),
+    "nrfft_plan_num_launches": (_i, [_vp]),
+}
+
+_lib = None
+
+
+def header_symbols() -> list[str]:
+    """Names of every function include/nrfft.h declares."""
+    with open(HEADER_PATH) as f:
+        text = f.read()
+    decl = re.compile(r"^\s*(?:const\s+)?(?:int|char\s*\*|size_t|void)\s*\*?\s*(nrfft_[a-z0-9_]+)\s*\(", re.M)
+    return sorted(set(decl.findall(text)))
+
+
+def lib() -> ctypes.CDLL:
+    """Load libnrfft.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build the HIP extension first "
+                "(python -c 'import __graft_entry__ as g; g.build()')")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def status_string(code: int) -> str:
+    return lib().nrfft_status_string(code).decode()
+
+
+def last_error() -> str:
+    return lib().nrfft_last_error().decode()

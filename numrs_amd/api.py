@@ -1,0 +1,427 @@
+"""Host-side mirror of the numrs FFT-path API on top of the C ABI.
+
+Names, argument meaning and error behaviour follow the Rust routines each
+function replaces (cited per function).  Every transform runs in the HIP
+kernels of libnrfft.so; nothing here computes an FFT on the CPU.  Rust panics
+(slice out of bounds, ``assert!``) surface as ``ValueError`` /
+``AssertionError``; Rust ``Result`` errors as the exception classes below.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Iterable, Sequence
+
+import numpy as np
+
+from . import _lib
+
+# ---------------------------------------------------------------------------
+# Errors
+# ---------------------------------------------------------------------------
+
+
+class NrfftError(RuntimeError):
+    """A non-zero nrfft_status from the C ABI."""
+
+    def __init__(self, code: int, message: str = ""):
+        self.code = code
+        text = message or _lib.status_string(code)
+        detail = _lib.last_error() if code in (_lib.E_HIP, _lib.E_NOMEM, _lib.E_UNSUPPORTED) else ""
+        super().__init__(f"{text}" + (f" ({detail})" if detail and detail != text else ""))
+
+
+class ConvlvError(NrfftError):
+    """Convolve.rs:226-238 ``ConvlvError``; ``kind`` names the variant."""
+
+    _KIND = {_lib.E_EMPTY: "EmptyInput", _lib.E_RESP_TOO_LONG: "ResponseTooLong",
+             _lib.E_ISIGN: "InvalidIsign"}
+    _MSG = {"EmptyInput": "Input arrays cannot be empty",
+            "ResponseTooLong": "Response function longer than data",
+            "InvalidIsign": "isign must be 1 (convolution) or -1 (deconvolution)",
+            "DivisionByZero": "Division by zero in deconvolution"}
+
+    def __init__(self, code: int):
+        self.kind = self._KIND.get(code, "FftError")
+        msg = self._MSG.get(self.kind) or f"FFT computation error: {_lib.status_string(code)}"
+        super().__init__(code, msg)
+
+
+class CorrelError(NrfftError):
+    """Correlation.rs:389-399 ``CorrelError``; ``kind`` names the variant."""
+
+    _MSG = {"EmptyInput": "Input arrays cannot be empty",
+            "LengthMismatch": "Input arrays must have the same length",
+            "ZeroStdDev": "Normalization error: standard deviation is zero"}
+
+    def __init__(self, kind: str, code: int = 0):
+        self.kind = kind
+        msg = self._MSG.get(kind) or f"FFT computation error: {_lib.status_string(code)}"
+        super().__init__(code, msg)
+
+
+class FournError(ValueError):
+    """``io::Error`` of kind InvalidInput with the messages of Fourn.rs:367-378."""
+
+    def __init__(self, code: int):
+        self.code = code
+        super().__init__(_lib.status_string(code) if code != _lib.E_DIMS else (_lib.last_error() or "Invalid dimensions"))
+
+
+def _check(code: int) -> None:
+    if code != _lib.OK:
+        raise NrfftError(code)
+
+
+# ---------------------------------------------------------------------------
+# Array plumbing
+# ---------------------------------------------------------------------------
+
+def _buf(a: np.ndarray):
+    """(ctypes pointer, suffix) for a contiguous float32/float64 array."""
+    if not isinstance(a, np.ndarray):
+        raise TypeError("expected a numpy array")
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(_lib._dp), "f64"
+    if a.dtype == np.float32:
+        return a.ctypes.data_as(_lib._fp), "f32"
+    raise TypeError(f"unsupported dtype {a.dtype} (float32 or float64)")
+
+
+def _inplace_view(data: np.ndarray, count: int) -> np.ndarray:
+    if data.ndim != 1 or not data.flags.c_contiguous or not data.flags.writeable:
+        raise ValueError("data must be a writable contiguous 1-D array")
+    if data.size < count:
+        raise ValueError(f"index out of bounds: the len is {data.size} but the transform needs {count}")
+    return data[:count]
+
+
+def _as_input(x, dtype=None) -> np.ndarray:
+    a = np.asarray(x)
+    if dtype is None:
+        dtype = a.dtype if a.dtype in (np.float32, np.float64) else np.float64
+    return np.ascontiguousarray(a, dtype=dtype).reshape(-1)
+
+
+# ---------------------------------------------------------------------------
+# four1 (FFT_1.rs)
+# ---------------------------------------------------------------------------
+
+def four1(data: np.ndarray, nn: int, isign: int) -> None:
+    """FFT_1.rs:5 ``four1(data, nn, isign)``: in-place complex FFT of 2*nn
+    interleaved values, kernel exp(+isign 2 pi i jk/nn), unnormalised."""
+    v = _inplace_view(data, 2 * nn)
+    p, sfx = _buf(v)
+    _check(getattr(_lib.lib(), f"nrfft_four1_{sfx}")(p, nn, isign))
+
+
+# FFT_1.rs:110 -- same transform, all stages with the recurrence on the CPU.
+four1_optimized = four1
+
+
+class FFTProcessor:
+    """FFT_1.rs:142-190.  ``fft`` transforms one buffer (nn = len/2);
+    ``fft_batch`` runs a list of buffers, equal-length groups batched into one
+    GPU execution each."""
+
+    def __init__(self) -> None:
+        self.max_threads = 1
+        self.use_optimized = True
+
+    @classmethod
+    def new(cls) -> "FFTProcessor":
+        return cls()
+
+    def with_threads(self, threads: int) -> "FFTProcessor":
+        self.max_threads = threads
+        return self
+
+    def with_optimized(self, use_optimized: bool) -> "FFTProcessor":
+        self.use_optimized = use_optimized
+        return self
+
+    def fft(self, data: np.ndarray, isign: int) -> None:
+        four1(data, data.size // 2, isign)
+
+    def fft_batch(self, batches: Sequence[np.ndarray], isign: int) -> None:
+        groups: dict = {}
+        for i, b in enumerate(batches):
+            groups.setdefault((b.dtype.str, b.size), []).append(i)
+        for (_, size), idx in groups.items():
+            nn = size // 2
+            stack = np.ascontiguousarray(np.stack([batches[i][:2 * nn] for i in idx]))
+            p, sfx = _buf(stack)
+            _check(getattr(_lib.lib(), f"nrfft_four1_batch_{sfx}")(p, nn, len(idx), isign))
+            for k, i in enumerate(idx):
+                batches[i][:2 * nn] = stack[k]
+
+
+def real_to_complex(real_data) -> np.ndarray:
+    """FFT_1.rs:193-200."""
+    r = np.asarray(real_data, dtype=np.float64)
+    out = np.zeros(2 * r.size, np.float64)
+    out[0::2] = r
+    return out
+
+
+def complex_to_real(complex_data) -> np.ndarray:
+    """FFT_1.rs:202-204."""
+    return np.asarray(complex_data)[0::2].copy()
+
+
+def magnitude_spectrum(complex_data) -> np.ndarray:
+    """FFT_1.rs:206-216."""
+    c = np.asarray(complex_data, dtype=np.float64)
+    m = c.size // 2
+    return np.sqrt(c[0:2 * m:2] ** 2 + c[1:2 * m:2] ** 2)
+
+
+def power_spectrum(complex_data) -> np.ndarray:
+    """FFT_1.rs:218-228."""
+    c = np.asarray(complex_data, dtype=np.float64)
+    m = c.size // 2
+    return c[0:2 * m:2] ** 2 + c[1:2 * m:2] ** 2
+
+
+# ---------------------------------------------------------------------------
+# realft (Real_FT.rs)
+# ---------------------------------------------------------------------------
+
+def realft(data: np.ndarray, n: int, isign: int) -> None:
+    """Real_FT.rs:4 ``realft(data, n, isign)`` with NR semantics: forward
+    (isign=1) packs [F_0, F_{n/2}, Re F_1, Im F_1, ...]; inverse returns (n/2) x."""
+    assert n % 2 == 0, "n must be even"                      # Real_FT.rs:5
+    assert data.size >= n, "data length must be at least n"  # Real_FT.rs:6
+    v = _inplace_view(data, n)
+    p, sfx = _buf(v)
+    _check(getattr(_lib.lib(), f"nrfft_realft_{sfx}")(p, n, 1 if isign == 1 else -1))
+
+
+realft_optimized = realft  # Real_FT.rs:216
+
+
+class RealFTProcessor:
+    """Real_FT.rs:331-370."""
+
+    def __init__(self) -> None:
+        self.use_optimized = True
+        self.parallel_threshold = 1024
+
+    @classmethod
+    def new(cls) -> "RealFTProcessor":
+        return cls()
+
+    def with_optimized(self, use_optimized: bool) -> "RealFTProcessor":
+        self.use_optimized = use_optimized
+        return self
+
+    def with_threshold(self, threshold: int) -> "RealFTProcessor":
+        self.parallel_threshold = threshold
+        return self
+
+    def process(self, data: np.ndarray, n: int, isign: int) -> None:
+        realft(data, n, isign)
+
+    def process_batch(self, batches: Iterable[tuple]) -> None:
+        for data, n, isign in batches:
+            realft(data, n, isign)
+
+
+# ---------------------------------------------------------------------------
+# fourn (Fourn.rs / Real_FT3.rs:35)
+# ---------------------------------------------------------------------------
+
+def fourn(data: np.ndarray, nn: Sequence[int], ndim: int, isign: int) -> None:
+    """In-memory NR fourn as Real_FT3.rs:35 calls it: in place, C order (nn[0]
+    slowest).  Errors: Fourn.rs:367-378 (``FournError``)."""
+    nn = [int(v) for v in nn]
+    arr = (ctypes.c_size_t * max(len(nn), 1))(*nn)
+    if ndim == 0 or ndim > len(nn):
+        code = _lib.E_DIMS
+        _lib.lib()  # message from the library for consistency
+        raise FournError(code)
+    total = int(np.prod(nn[:ndim]))
+    v = _inplace_view(data, 2 * total) if all(x > 1 for x in nn[:ndim]) else data
+    p, sfx = _buf(v)
+    code = getattr(_lib.lib(), f"nrfft_fourn_{sfx}")(p, arr, ndim, isign)
+    if code != _lib.OK:
+        if code in (_lib.E_DIMS, _lib.E_ISIGN, _lib.E_NOT_POW2):
+            raise FournError(code)
+        raise NrfftError(code)
+
+
+# ---------------------------------------------------------------------------
+# convlv (Convolve.rs)
+# ---------------------------------------------------------------------------
+
+def _convlv_call(data: np.ndarray, respns: np.ndarray, isign: int, batch: int) -> np.ndarray:
+    n = data.size // batch if batch else 0
+    m = respns.size // batch if batch else 0
+    ans = np.empty(max(n * batch, 1), data.dtype)
+    pd, sfx = _buf(data if data.size else np.zeros(1, data.dtype))
+    pr, _ = _buf(respns if respns.size else np.zeros(1, data.dtype))
+    pa, _ = _buf(ans)
+    L = _lib.lib()
+    if batch == 1:
+        code = getattr(L, f"nrfft_convlv_{sfx}")(pd, n, pr, m, isign, pa)
+    else:
+        code = getattr(L, f"nrfft_convlv_batch_{sfx}")(pd, n, pr, m, batch, isign, pa)
+    if code != _lib.OK:
+        raise ConvlvError(code)
+    return ans[:n * batch]
+
+
+def convlv(data, respns, isign: int) -> np.ndarray:
+    """Convolve.rs:8 ``convlv(data, respns, isign) -> Result<Array1, ConvlvError>``:
+    circular convolution (isign=1) or deconvolution (isign=-1) of data with the
+    response placed as Convolve.rs:41-63 places it."""
+    d = _as_input(data)
+    r = _as_input(respns, d.dtype)
+    if d.size == 0 or r.size == 0:
+        raise ConvlvError(_lib.E_EMPTY)
+    return _convlv_call(d, r, isign, 1)
+
+
+def convlv_batch(data_batch: Sequence, respns, isign: int) -> list:
+    """Convolve.rs:241-250: every signal convolved with the same response."""
+    if len(data_batch) == 0:
+        return []
+    out = []
+    groups: dict = {}
+    for i, d in enumerate(data_batch):
+        a = _as_input(d)
+        groups.setdefault((a.dtype.str, a.size), []).append(i)
+    res: list = [None] * len(data_batch)
+    for (_, n), idx in groups.items():
+        stack = np.ascontiguousarray(np.stack([_as_input(data_batch[i]) for i in idx]))
+        r = _as_input(respns, stack.dtype)
+        if n == 0 or r.size == 0:
+            raise ConvlvError(_lib.E_EMPTY)
+        rr = np.ascontiguousarray(np.tile(r, len(idx)))
+        ans = _convlv_call(stack.reshape(-1), rr, isign, len(idx)).reshape(len(idx), n)
+        for k, i in enumerate(idx):
+            res[i] = ans[k].copy()
+    out.extend(res)
+    return out
+
+
+def convlv_batch_responses(data: np.ndarray, respns: np.ndarray, isign: int) -> np.ndarray:
+    """convlv over a (batch, n) array with one response per signal, (batch, m):
+    the batched entry point nrfft_convlv_batch_*."""
+    d = np.ascontiguousarray(data)
+    r = np.ascontiguousarray(respns, dtype=d.dtype)
+    if d.ndim != 2 or r.ndim != 2 or d.shape[0] != r.shape[0]:
+        raise ValueError("data (batch, n) and respns (batch, m) expected")
+    if d.size == 0 or r.size == 0:
+        raise ConvlvError(_lib.E_EMPTY)
+    return _convlv_call(d.reshape(-1), r.reshape(-1), isign, d.shape[0]).reshape(d.shape)
+
+
+class ConvlvProcessor:
+    """Convolve.rs:253-339."""
+
+    def __init__(self) -> None:
+        self.use_optimized = True
+        self.parallel_threshold = 1024
+
+    @classmethod
+    def new(cls) -> "ConvlvProcessor":
+        return cls()
+
+    def with_optimized(self, use_optimized: bool) -> "ConvlvProcessor":
+        self.use_optimized = use_optimized
+        return self
+
+    def with_threshold(self, threshold: int) -> "ConvlvProcessor":
+        self.parallel_threshold = threshold
+        return self
+
+    def process(self, data, respns, isign: int) -> np.ndarray:
+        return convlv(data, respns, isign)
+
+    def process_batch(self, data_batch, respns, isign: int) -> list:
+        return convlv_batch(data_batch, respns, isign)
+
+
+# ---------------------------------------------------------------------------
+# correl (Correlation.rs)
+# ---------------------------------------------------------------------------
+
+def correl(data1, data2) -> np.ndarray:
+    """Correlation.rs:8 ``correl(data1, data2) -> Result<Array1, CorrelError>``.
+    n <= 32: linear correlation at lags 0..n-1; n > 32: circular correlation
+    (negative lags wrapped to the top)."""
+    a = _as_input(data1)
+    b = _as_input(data2, a.dtype)
+    if a.size == 0:
+        raise CorrelError("EmptyInput", _lib.E_EMPTY)
+    if b.size != a.size:
+        raise CorrelError("LengthMismatch", _lib.E_LEN)
+    ans = np.empty(a.size, a.dtype)
+    pa, sfx = _buf(a)
+    pb, _ = _buf(b)
+    po, _ = _buf(ans)
+    code = getattr(_lib.lib(), f"nrfft_correl_{sfx}")(pa, pb, a.size, po)
+    if code != _lib.OK:
+        kind = {_lib.E_EMPTY: "EmptyInput", _lib.E_LEN: "LengthMismatch"}.get(code, "FftError")
+        raise CorrelError(kind, code)
+    return ans
+
+
+def _moments(x: np.ndarray):
+    mean = float(np.sum(x) / x.size)
+    std = float(np.sqrt(np.sum((x - mean) ** 2) / x.size))
+    return mean, std
+
+
+def correl_normalized(data1, data2) -> np.ndarray:
+    """Correlation.rs:189-223: z-score both inputs, then correl."""
+    a = _as_input(data1, np.float64)
+    b = _as_input(data2, np.float64)
+    if a.size == 0:
+        raise CorrelError("EmptyInput")
+    if b.size != a.size:
+        raise CorrelError("LengthMismatch")
+    m1, s1 = _moments(a)
+    m2, s2 = _moments(b)
+    if s1 == 0.0 or s2 == 0.0:
+        raise CorrelError("ZeroStdDev")
+    return correl((a - m1) / s1, (b - m2) / s2)
+
+
+def correl_normalized_fast(data1, data2) -> np.ndarray:
+    """Correlation.rs:226-270 (single-pass moments).  For n <= 32 the
+    reference divides the direct sum by (std1 std2 n); for n > 32 it returns
+    correl of the z-scored inputs."""
+    a = _as_input(data1, np.float64)
+    b = _as_input(data2, np.float64)
+    n = a.size
+    if n == 0:
+        raise CorrelError("EmptyInput")
+    if b.size != n:
+        raise CorrelError("LengthMismatch")
+    m1, m2 = a.sum() / n, b.sum() / n
+    s1 = float(np.sqrt((a * a).sum() / n - m1 * m1))
+    s2 = float(np.sqrt((b * b).sum() / n - m2 * m2))
+    if s1 == 0.0 or s2 == 0.0:
+        raise CorrelError("ZeroStdDev")
+    if n <= 32:
+        return correl(a - m1, b - m2) * (1.0 / (s1 * s2 * n))
+    return correl((a - m1) / s1, (b - m2) / s2)
+
+
+def correl_batch(data_pairs: Sequence[tuple]) -> list:
+    """Correlation.rs:273-278."""
+    return [correl(a, b) for a, b in data_pairs]
+
+
+def autocorrel(data) -> np.ndarray:
+    """Correlation.rs:281-283."""
+    return correl(data, data)
+
+
+def autocorrel_fast(data) -> np.ndarray:
+    """Correlation.rs:286-323 (|F|^2 then inverse; equal to autocorrel)."""
+    a = _as_input(data)
+    if a.size == 0:
+        raise CorrelError("EmptyInput")
+    return correl(a, a)

@@ -1,0 +1,544 @@
+// api.hip -- the extern "C" boundary (include/nrfft.h): device contexts,
+// planning, execution and the host-pointer drop-in calls.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/nrfft.h"
+#include "internal.hpp"
+
+namespace nrfft {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                      \
+    do {                                                                                               \
+        hipError_t e_ = (x);                                                                           \
+        if (e_ != hipSuccess) return fail(NRFFT_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+bool is_pow2(size_t n) { return n != 0 && (n & (n - 1)) == 0; }
+int ilog2(size_t n) {
+    int l = 0;
+    while ((size_t(1) << l) < n) ++l;
+    return l;
+}
+
+// ---------------------------------------------------------------------------
+// Device context: twiddle tables generated in FP64 on the GPU.
+// ---------------------------------------------------------------------------
+__global__ void k_twiddles(c64* c64c, c64* c64f, c32* c32c, c32* c32f) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= TW_COARSE) return;
+    double s, c;
+    sincospi((double)i / (double)(TW_COARSE / 2), &s, &c);  // exp(2 pi i m / 2^13)
+    c64c[i] = c64{c, s};
+    c32c[i] = c32{(float)c, (float)s};
+    sincospi((double)i / (double)(1 << (TW_LOG_VIRT - 1)), &s, &c);  // exp(2 pi i m / 2^26)
+    c64f[i] = c64{c, s};
+    c32f[i] = c32{(float)c, (float)s};
+}
+
+static std::mutex g_ctx_mu;
+static std::map<int, DeviceCtx*> g_ctx;
+
+int get_ctx(DeviceCtx** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    auto it = g_ctx.find(dev);
+    if (it != g_ctx.end()) {
+        *out = it->second;
+        return NRFFT_OK;
+    }
+    auto* c = new DeviceCtx();
+    c->device = dev;
+    HIPCHK(hipMalloc(&c->c64c, TW_COARSE * sizeof(c64)));
+    HIPCHK(hipMalloc(&c->c64f, TW_COARSE * sizeof(c64)));
+    HIPCHK(hipMalloc(&c->c32c, TW_COARSE * sizeof(c32)));
+    HIPCHK(hipMalloc(&c->c32f, TW_COARSE * sizeof(c32)));
+    hipLaunchKernelGGL(k_twiddles, dim3(TW_COARSE / 256), dim3(256), 0, 0, c->c64c, c->c64f, c->c32c, c->c32f);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    g_ctx[dev] = c;
+    *out = c;
+    return NRFFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel lookup.
+// ---------------------------------------------------------------------------
+const KernelRef& c2c_lookup(int dtype, int sign, int mode, int logl, int tilog) {
+    static const KernelRef none{nullptr, 0, 0, 0};
+    if (logl < 1 || logl > C2C_MAX_LOGL_ || tilog < 0 || tilog >= C2C_NTI_ || mode < 0 || mode > 3) return none;
+    if (dtype == NRFFT_F32) return sign > 0 ? c2c_lookup_f32p(mode, logl, tilog) : c2c_lookup_f32m(mode, logl, tilog);
+    return sign > 0 ? c2c_lookup_f64p(mode, logl, tilog) : c2c_lookup_f64m(mode, logl, tilog);
+}
+
+// Largest valid tile width <= 2^want_log (and <= nb0 rounded up to pow2).
+int pick_tilog(int dtype, int mode, int logl, int64_t nb0, int want_log) {
+    int cap = 0;
+    while ((int64_t(1) << cap) < nb0 && cap < want_log) ++cap;
+    for (int t = std::min(cap, want_log); t >= 0; --t)
+        if (c2c_lookup(dtype, +1, mode, logl, t).fn) return t;
+    return -1;
+}
+
+// ---------------------------------------------------------------------------
+// Plans.
+// ---------------------------------------------------------------------------
+int plan_add_c2c_pass(Plan& p, int mode, int logl, int tilog, const PassDesc& d, int src, int dst) {
+    const KernelRef& kp = c2c_lookup(p.dtype, +1, mode, logl, tilog);
+    const KernelRef& km = c2c_lookup(p.dtype, -1, mode, logl, tilog);
+    if (!kp.fn || !km.fn) return fail(NRFFT_E_UNSUPPORTED, "no kernel for pass (mode " + std::to_string(mode) +
+                                                               ", log2 L " + std::to_string(logl) + ")");
+    Launch l;
+    l.kind = LK_C2C;
+    l.fn[0] = kp.fn;
+    l.fn[1] = km.fn;
+    l.nt = kp.nt;
+    const int64_t tiles0 = (d.nb0 + kp.ti - 1) / kp.ti;
+    const int64_t grid = tiles0 * d.nb1 * d.nb2;
+    if (grid <= 0 || grid > 0x7fffffffLL) return fail(NRFFT_E_UNSUPPORTED, "grid too large");
+    // In-tile offsets must fit 32 bits.
+    const int64_t span_in = int64_t(kp.ti - 1) * d.si0 + int64_t((1 << logl) - 1) * d.esi;
+    const int64_t span_out = int64_t(kp.ti - 1) * d.so0 + int64_t((1 << logl) - 1) * d.eso;
+    if (span_in > 0x7fffffffLL || span_out > 0x7fffffffLL) return fail(NRFFT_E_UNSUPPORTED, "tile span too large");
+    l.grid = (unsigned)grid;
+    l.desc = d;
+    l.src = src;
+    l.dst = dst;
+    l.lds = kp.lds_bytes;
+    l.logl = logl;
+    l.ti = kp.ti;
+    l.mode = mode;
+    p.launches.push_back(l);
+    return NRFFT_OK;
+}
+
+// Element size of one complex value.
+size_t csize(int dtype) { return dtype == NRFFT_F32 ? 8 : 16; }
+
+// Preferred tile width (log2) for column-mapped passes.
+int col_tilog(int dtype) { return dtype == NRFFT_F32 ? 4 : 3; }
+
+// 1-D batched C2C of length N = 2^p over `batch` contiguous transforms, from
+// buffer src to buffer dst (B_TMP used as the intermediate of two-pass sizes).
+// Output element k of transform b lands at dst[b*N*ostride_b... ] -- contiguous.
+int plan_c2c_1d(Plan& p, int logn, int64_t batch, int src, int dst, size_t* tmp_elems) {
+    const int64_t N = int64_t(1) << logn;
+    if (logn == 0) {
+        // nn == 1: the identity; a copy when out of place.
+        Launch l;
+        l.kind = LK_COPY;
+        l.src = src;
+        l.dst = dst;
+        l.bytes = size_t(batch) * csize(p.dtype);
+        p.launches.push_back(l);
+        return NRFFT_OK;
+    }
+    if (logn <= C2C_MAX_LOGL_) {
+        PassDesc d{};
+        d.nb0 = batch;
+        d.nb1 = 1;
+        d.nb2 = 1;
+        d.si0 = (int32_t)N;
+        d.so0 = (int32_t)N;
+        d.esi = 1;
+        d.eso = 1;
+        if (N * std::min<int64_t>(batch, 256) > 0x7fffffffLL) return fail(NRFFT_E_UNSUPPORTED, "too large");
+        return plan_add_c2c_pass(p, MODE_JJ, logn, 0, d, src, dst);
+    }
+    if (logn > 2 * C2C_MAX_LOGL_) return fail(NRFFT_E_UNSUPPORTED, "transform longer than 2^26");
+    // Four-step: N = N1 * N2, x viewed as [n1][n2].
+    int l1 = logn / 2, l2 = logn - l1;
+    if (p.tune.split_log1 > 0 && p.tune.split_log1 < logn && logn - p.tune.split_log1 <= C2C_MAX_LOGL_ &&
+        p.tune.split_log1 <= C2C_MAX_LOGL_) {
+        l1 = p.tune.split_log1;
+        l2 = logn - l1;
+    }
+    const int64_t N1 = int64_t(1) << l1, N2 = int64_t(1) << l2;
+    *tmp_elems = std::max<size_t>(*tmp_elems, size_t(N) * size_t(batch));
+    // Pass A: FFT over n1 (stride N2), tile over n2, twiddle W_N^{k1 n2}; -> tmp[k1][n2].
+    {
+        PassDesc d{};
+        d.nb0 = N2;
+        d.si0 = 1;
+        d.so0 = 1;
+        d.esi = (int32_t)N2;
+        d.eso = (int32_t)N2;
+        d.nb1 = batch;
+        d.si1 = N;
+        d.so1 = N;
+        d.nb2 = 1;
+        d.tw_logn = logn;
+        d.tw_b0_shift = 0;
+        int want = p.tune.tilog_a >= 0 ? p.tune.tilog_a : col_tilog(p.dtype);
+        int t = pick_tilog(p.dtype, MODE_TT, l1, N2, want);
+        if (t < 0) return fail(NRFFT_E_UNSUPPORTED, "no column kernel");
+        int rc = plan_add_c2c_pass(p, MODE_TT, l1, t, d, src, B_TMP);
+        if (rc) return rc;
+    }
+    // Pass B: FFT over n2 (contiguous rows of tmp), rows k1 tiled; X[k2 N1 + k1].
+    {
+        PassDesc d{};
+        d.nb0 = N1;
+        d.si0 = (int32_t)N2;
+        d.esi = 1;
+        d.so0 = 1;
+        d.eso = (int32_t)N1;
+        d.nb1 = batch;
+        d.si1 = N;
+        d.so1 = N;
+        d.nb2 = 1;
+        int want = p.tune.tilog_b >= 0 ? p.tune.tilog_b : col_tilog(p.dtype);
+        int t = pick_tilog(p.dtype, MODE_JT, l2, N1, want);
+        if (t < 0) return fail(NRFFT_E_UNSUPPORTED, "no transpose kernel");
+        int rc = plan_add_c2c_pass(p, MODE_JT, l2, t, d, B_TMP, dst);
+        if (rc) return rc;
+    }
+    return NRFFT_OK;
+}
+
+// n-D (C order) batched C2C; axes processed fastest first.
+int plan_c2c_nd(Plan& p, const std::vector<size_t>& nn, int64_t batch) {
+    size_t tmp = 0;
+    const size_t ndim = nn.size();
+    if (ndim == 1) {
+        int rc = plan_c2c_1d(p, ilog2(nn[0]), batch, B_IN, B_OUT, &tmp);
+        p.tmp_elems = tmp;
+        return rc;
+    }
+    int64_t total = 1;
+    for (size_t v : nn) total *= (int64_t)v;
+    bool first = true;
+    for (size_t dd = ndim; dd-- > 0;) {
+        const int64_t L = (int64_t)nn[dd];
+        const int logl = ilog2(nn[dd]);
+        int64_t inner = 1;
+        for (size_t e = dd + 1; e < ndim; ++e) inner *= (int64_t)nn[e];
+        const int64_t outer = (total / (L * inner)) * batch;
+        const int src = first ? B_IN : B_OUT;
+        first = false;
+        if (logl > C2C_MAX_LOGL_) {
+            if (inner != 1) return fail(NRFFT_E_UNSUPPORTED, "strided axis longer than 8192");
+            int rc = plan_c2c_1d(p, logl, outer, src, B_OUT, &tmp);
+            if (rc) return rc;
+            continue;
+        }
+        PassDesc d{};
+        if (inner == 1) {
+            d.nb0 = outer;
+            d.si0 = (int32_t)L;
+            d.so0 = (int32_t)L;
+            d.esi = 1;
+            d.eso = 1;
+            d.nb1 = 1;
+            d.nb2 = 1;
+            int rc = plan_add_c2c_pass(p, MODE_JJ, logl, 0, d, src, B_OUT);
+            if (rc) return rc;
+        } else {
+            if (L * inner > 0x7fffffffLL) return fail(NRFFT_E_UNSUPPORTED, "axis span too large");
+            d.nb0 = inner;
+            d.si0 = 1;
+            d.so0 = 1;
+            d.esi = (int32_t)inner;
+            d.eso = (int32_t)inner;
+            d.nb1 = outer;
+            d.si1 = L * inner;
+            d.so1 = L * inner;
+            d.nb2 = 1;
+            int t = pick_tilog(p.dtype, MODE_TT, logl, inner, p.tune.tilog_a >= 0 ? p.tune.tilog_a : col_tilog(p.dtype));
+            if (t < 0) return fail(NRFFT_E_UNSUPPORTED, "no column kernel");
+            int rc = plan_add_c2c_pass(p, MODE_TT, logl, t, d, src, B_OUT);
+            if (rc) return rc;
+        }
+    }
+    p.tmp_elems = tmp;
+    return NRFFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Execution.
+// ---------------------------------------------------------------------------
+int exec_plan(const Plan& p, const void* in, void* out, void* tmp, const void* aux, int isign, hipStream_t stream) {
+    DeviceCtx* ctx = nullptr;
+    int rc = get_ctx(&ctx);
+    if (rc) return rc;
+    void* tmp2 = tmp ? static_cast<char*>(tmp) + p.tmp2_offset * csize(p.dtype) : nullptr;
+    void* bufs[B_COUNT] = {const_cast<void*>(in), out, tmp, const_cast<void*>(aux), tmp2};
+    for (const Launch& l : p.launches) {
+        if (l.args[7] != 0 && ((l.args[7] > 0) != (isign > 0))) continue;
+        const int eff = l.args[6] != 0 ? (int)l.args[6] : isign;
+        const int si = eff > 0 ? 0 : 1;
+        void* a = bufs[l.src];
+        void* b = bufs[l.dst];
+        if (l.kind == LK_COPY) {
+            if (a != b) HIPCHK(hipMemcpyAsync(b, a, l.bytes, hipMemcpyDeviceToDevice, stream));
+            continue;
+        }
+        if (l.kind == LK_C2C) {
+            PassDesc d = l.desc;
+            if (p.dtype == NRFFT_F32) {
+                Twiddles<float> tw{ctx->c32c, ctx->c32f};
+                void* args[] = {&a, &b, &d, &tw};
+                HIPCHK(hipLaunchKernel(l.fn[si], dim3(l.grid), dim3(l.nt), args, 0, stream));
+            } else {
+                Twiddles<double> tw{ctx->c64c, ctx->c64f};
+                void* args[] = {&a, &b, &d, &tw};
+                HIPCHK(hipLaunchKernel(l.fn[si], dim3(l.grid), dim3(l.nt), args, 0, stream));
+            }
+            continue;
+        }
+        rc = exec_special(p, l, bufs, eff, stream, ctx);
+        if (rc) return rc;
+    }
+    return NRFFT_OK;
+}
+
+// Plan cache for the host-pointer calls (plans without workspace).
+static std::mutex g_plan_mu;
+static std::map<std::tuple<int, int, std::vector<size_t>, size_t, size_t>, std::shared_ptr<Plan>> g_plans;
+
+int cached_plan(int kind, int dtype, const std::vector<size_t>& nn, size_t batch, size_t extra,
+                std::shared_ptr<Plan>* out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    auto key = std::make_tuple(kind * 64 + dev, dtype, nn, batch, extra);
+    {
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        auto it = g_plans.find(key);
+        if (it != g_plans.end()) {
+            *out = it->second;
+            return NRFFT_OK;
+        }
+    }
+    auto p = std::make_shared<Plan>();
+    p->kind = kind;
+    p->dtype = dtype;
+    p->nn = nn;
+    p->batch = batch;
+    p->extra = extra;
+    p->device = dev;
+    tune_lookup(*p);
+    int rc = build_plan(*p);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    g_plans[key] = p;
+    *out = p;
+    return NRFFT_OK;
+}
+
+// Tunables: defaults here; the autotuner (tune.hip) overrides them.
+__attribute__((weak)) void tune_lookup(Plan& p) { p.tune = Tune{}; }
+
+int build_plan(Plan& p) {
+    switch (p.kind) {
+        case PK_C2C:
+            return plan_c2c_nd(p, p.nn, (int64_t)p.batch);
+        default:
+            return build_plan_real(p);
+    }
+}
+
+int dev_alloc(DevBuf& b, size_t bytes, hipStream_t s) {
+    b.s = s;
+    if (bytes == 0) return NRFFT_OK;
+    hipError_t e = hipMallocAsync(&b.p, bytes, s);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(NRFFT_E_NOMEM, std::string("hipMallocAsync: ") + hipGetErrorString(e));
+    }
+    return NRFFT_OK;
+}
+
+// Host-pointer C2C (four1 / fourn / batch): in place on host data.
+int host_c2c(int dtype, void* data, const std::vector<size_t>& nn, size_t batch, int isign) {
+    std::shared_ptr<Plan> p;
+    int rc = cached_plan(PK_C2C, dtype, nn, batch, 0, &p);
+    if (rc) return rc;
+    size_t elems = batch;
+    for (size_t v : nn) elems *= v;
+    const size_t bytes = elems * csize(dtype);
+    hipStream_t s = hipStreamPerThread;
+    DevBuf buf, tmp;
+    if ((rc = dev_alloc(buf, bytes, s))) return rc;
+    if ((rc = dev_alloc(tmp, p->tmp_elems * csize(dtype), s))) return rc;
+    HIPCHK(hipMemcpyAsync(buf.p, data, bytes, hipMemcpyHostToDevice, s));
+    if ((rc = exec_plan(*p, buf.p, buf.p, tmp.p, nullptr, isign, s))) return rc;
+    HIPCHK(hipMemcpyAsync(data, buf.p, bytes, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return NRFFT_OK;
+}
+
+int make_plan(nrfft_plan* plan, int kind, int dtype, std::vector<size_t> nn, size_t batch, size_t extra) {
+    if (!plan) return fail(NRFFT_E_INVALID, "null plan pointer");
+    *plan = nullptr;
+    if (dtype != NRFFT_F32 && dtype != NRFFT_F64) return fail(NRFFT_E_INVALID, "dtype");
+    if (batch == 0) return fail(NRFFT_E_EMPTY, "batch == 0");
+    auto* p = new (std::nothrow) nrfft_plan_s();
+    if (!p) return fail(NRFFT_E_NOMEM, "plan");
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        delete p;
+        return fail(NRFFT_E_HIP, hipGetErrorString(e));
+    }
+    p->plan.kind = kind;
+    p->plan.dtype = dtype;
+    p->plan.nn = std::move(nn);
+    p->plan.batch = batch;
+    p->plan.extra = extra;
+    p->plan.device = dev;
+    tune_lookup(p->plan);
+    int rc = build_plan(p->plan);
+    if (!rc && p->plan.tmp_elems) {
+        const size_t bytes = p->plan.tmp_elems * csize(dtype);
+        e = hipMalloc(&p->tmp, bytes);
+        if (e != hipSuccess) rc = fail(NRFFT_E_NOMEM, std::string("workspace: ") + hipGetErrorString(e));
+        else p->tmp_bytes = bytes;
+    }
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    *plan = p;
+    return NRFFT_OK;
+}
+}  // namespace nrfft
+
+using namespace nrfft;
+
+// ---------------------------------------------------------------------------
+// extern "C"
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* nrfft_version(void) { return "nrfft-mi355x 0.1 (gfx950)"; }
+
+const char* nrfft_status_string(int s) {
+    switch (s) {
+        case NRFFT_OK: return "ok";
+        case NRFFT_E_ISIGN: return "isign must be 1 or -1";
+        case NRFFT_E_DIMS: return "Invalid dimensions";
+        case NRFFT_E_NOT_POW2: return "length must be a power of two";
+        case NRFFT_E_LEN: return "Input arrays must have the same length";
+        case NRFFT_E_EMPTY: return "Input arrays cannot be empty";
+        case NRFFT_E_RESP_TOO_LONG: return "Response function longer than data";
+        case NRFFT_E_HIP: return "HIP runtime error";
+        case NRFFT_E_NOMEM: return "out of memory";
+        case NRFFT_E_UNSUPPORTED: return "size not supported";
+        case NRFFT_E_INVALID: return "invalid argument";
+        default: return "unknown status";
+    }
+}
+
+const char* nrfft_last_error(void) { return g_err.c_str(); }
+
+int nrfft_init(int device) {
+    g_err.clear();
+    HIPCHK(hipSetDevice(device));
+    DeviceCtx* c = nullptr;
+    return get_ctx(&c);
+}
+
+static int check_four1(const void* data, size_t nn, int isign) {
+    if (isign != 1 && isign != -1) return fail(NRFFT_E_ISIGN, "isign must be 1 or -1");
+    if (nn == 0) return fail(NRFFT_E_EMPTY, "nn == 0");
+    if (!data) return fail(NRFFT_E_INVALID, "null data");
+    if (!is_pow2(nn)) return fail(NRFFT_E_NOT_POW2, "nn must be a power of two");
+    return NRFFT_OK;
+}
+
+int nrfft_four1_f64(double* data, size_t nn, int isign) {
+    g_err.clear();
+    int rc = check_four1(data, nn, isign);
+    return rc ? rc : host_c2c(NRFFT_F64, data, {nn}, 1, isign);
+}
+int nrfft_four1_f32(float* data, size_t nn, int isign) {
+    g_err.clear();
+    int rc = check_four1(data, nn, isign);
+    return rc ? rc : host_c2c(NRFFT_F32, data, {nn}, 1, isign);
+}
+int nrfft_four1_batch_f64(double* data, size_t nn, size_t batch, int isign) {
+    g_err.clear();
+    int rc = check_four1(data, nn, isign);
+    if (rc) return rc;
+    if (batch == 0) return NRFFT_OK;
+    return host_c2c(NRFFT_F64, data, {nn}, batch, isign);
+}
+int nrfft_four1_batch_f32(float* data, size_t nn, size_t batch, int isign) {
+    g_err.clear();
+    int rc = check_four1(data, nn, isign);
+    if (rc) return rc;
+    if (batch == 0) return NRFFT_OK;
+    return host_c2c(NRFFT_F32, data, {nn}, batch, isign);
+}
+
+// Fourn.rs:367-378 order of checks: dims, isign, sizes.
+static int check_fourn(const void* data, const size_t* nn, size_t ndim, int isign, std::vector<size_t>* dims) {
+    if (ndim == 0 || !nn) return fail(NRFFT_E_DIMS, "Invalid dimensions");
+    if (isign != 1 && isign != -1) return fail(NRFFT_E_ISIGN, "isign must be 1 or -1");
+    for (size_t i = 0; i < ndim; ++i)
+        if (nn[i] <= 1) return fail(NRFFT_E_DIMS, "Invalid dimension size");
+    for (size_t i = 0; i < ndim; ++i)
+        if (!is_pow2(nn[i])) return fail(NRFFT_E_NOT_POW2, "dimension must be a power of two");
+    if (!data) return fail(NRFFT_E_INVALID, "null data");
+    dims->assign(nn, nn + ndim);
+    return NRFFT_OK;
+}
+
+int nrfft_fourn_f64(double* data, const size_t* nn, size_t ndim, int isign) {
+    g_err.clear();
+    std::vector<size_t> d;
+    int rc = check_fourn(data, nn, ndim, isign, &d);
+    return rc ? rc : host_c2c(NRFFT_F64, data, d, 1, isign);
+}
+int nrfft_fourn_f32(float* data, const size_t* nn, size_t ndim, int isign) {
+    g_err.clear();
+    std::vector<size_t> d;
+    int rc = check_fourn(data, nn, ndim, isign, &d);
+    return rc ? rc : host_c2c(NRFFT_F32, data, d, 1, isign);
+}
+
+// ---- device plans ---------------------------------------------------------
+
+int nrfft_plan_c2c(nrfft_plan* plan, int dtype, const size_t* nn, size_t ndim, size_t batch) {
+    g_err.clear();
+    std::vector<size_t> d;
+    int rc = check_fourn(nn, nn, ndim, 1, &d);
+    if (rc) return rc;
+    return make_plan(plan, PK_C2C, dtype, d, batch, 0);
+}
+
+int nrfft_execute_c2c(nrfft_plan plan, const void* d_in, void* d_out, int isign, void* stream) {
+    g_err.clear();
+    if (!plan || plan->plan.kind != PK_C2C) return fail(NRFFT_E_INVALID, "not a c2c plan");
+    if (isign != 1 && isign != -1) return fail(NRFFT_E_ISIGN, "isign must be 1 or -1");
+    if (!d_in || !d_out) return fail(NRFFT_E_INVALID, "null buffer");
+    return exec_plan(plan->plan, d_in, d_out, plan->tmp, nullptr, isign, (hipStream_t)stream);
+}
+
+int nrfft_plan_destroy(nrfft_plan plan) {
+    g_err.clear();
+    if (!plan) return NRFFT_OK;
+    if (plan->tmp) (void)hipFree(plan->tmp);
+    delete plan;
+    return NRFFT_OK;
+}
+
+size_t nrfft_plan_workspace_bytes(nrfft_plan plan) { return plan ? plan->tmp_bytes : 0; }
+
+int nrfft_plan_num_launches(nrfft_plan plan) { return plan ? (int)plan->plan.launches.size() : 0; }
+
+}  // extern "C"

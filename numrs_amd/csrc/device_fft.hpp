@@ -1,0 +1,375 @@
+// device_fft.hpp -- register/LDS building blocks of the Stockham autosort FFT
+// for gfx950 (CDNA4, 64-wide wavefronts).
+//
+// A work-group owns a tile of TI independent transforms of length L (a power
+// of two).  Each of its NT threads holds EPT = TI*L/NT complex values in
+// registers.  Stage s applies radix R_s butterflies with the Stockham
+// indexing (Ns = product of the earlier radices, butterfly j in [0, L/R)):
+//   read   X[j + r*L/R]            r = 0..R-1
+//   scale  by W_{Ns*R}^{S*r*(j mod Ns)}
+//   DFT_R
+//   write  Y[(j/Ns)*Ns*R + (j mod Ns) + r*Ns]
+// so the first stage reads, and the last stage writes, unit-stride runs: the
+// global load and store of a pass are coalesced without a separate
+// transpose.  Stages exchange data through LDS; the thread->butterfly mapping
+// may change between stages (MAP_J: consecutive threads take consecutive j of
+// one transform, MAP_T: consecutive threads take the same j of consecutive
+// transforms), which is how a pass reads rows and writes columns, or the
+// reverse, with both sides coalesced.
+//
+// Sign convention (numrs four1, FFT_1.rs:5-44): S = isign, kernel
+// exp(+S * 2 pi i jk / L), no normalisation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nrfft {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cpx {
+    T x, y;
+};
+using c32 = cpx<float>;
+using c64 = cpx<double>;
+
+template <typename T>
+__device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+template <typename T>
+__device__ __forceinline__ cpx<T> cmulc(cpx<T> a, cpx<T> b) {
+    return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> cscale(cpx<T> a, T s) { return {a.x * s, a.y * s}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> conj(cpx<T> a) { return {a.x, -a.y}; }
+
+// a * (S i)
+template <int S, typename T>
+__device__ __forceinline__ cpx<T> mul_si(cpx<T> a) {
+    if constexpr (S > 0) return {-a.y, a.x};
+    else return {a.y, -a.x};
+}
+// Twiddle tables hold exp(+2 pi i m / M); direction S uses the conjugate for S < 0.
+template <int S, typename T>
+__device__ __forceinline__ cpx<T> dirw(cpx<T> w) {
+    if constexpr (S > 0) return w;
+    else return {w.x, -w.y};
+}
+
+// ---------------------------------------------------------------------------
+// Small DFTs in registers, natural order in and out:
+//   v[k] <- sum_j v[j] exp(S 2 pi i jk / R)
+// ---------------------------------------------------------------------------
+template <int S, typename T>
+__device__ __forceinline__ void dft2(cpx<T>* v) {
+    cpx<T> a = v[0], b = v[1];
+    v[0] = a + b;
+    v[1] = a - b;
+}
+
+template <int S, typename T>
+__device__ __forceinline__ void dft4(cpx<T>& x0, cpx<T>& x1, cpx<T>& x2, cpx<T>& x3) {
+    cpx<T> a0 = x0 + x2, a1 = x0 - x2, a2 = x1 + x3, a3 = mul_si<S>(x1 - x3);
+    x0 = a0 + a2;
+    x2 = a0 - a2;
+    x1 = a1 + a3;
+    x3 = a1 - a3;
+}
+
+template <int S, typename T>
+__device__ __forceinline__ void dft4(cpx<T>* v) { dft4<S>(v[0], v[1], v[2], v[3]); }
+
+// exp(S i pi m / 8) * a for the m used by the 8- and 16-point kernels.
+template <int S, int M, typename T>
+__device__ __forceinline__ cpx<T> rot16(cpx<T> a) {
+    constexpr double C1 = 0.92387953251128675612818318939678829;  // cos(pi/8)
+    constexpr double S1 = 0.38268343236508977172845998403039887;  // sin(pi/8)
+    constexpr double H = 0.70710678118654752440084436210484904;   // cos(pi/4)
+    constexpr double sg = (double)S;
+    if constexpr (M == 0) return a;
+    else if constexpr (M == 4) return mul_si<S>(a);
+    else if constexpr (M == 8) return {-a.x, -a.y};
+    else if constexpr (M == 2) {
+        const T h = (T)H;
+        return {h * (a.x - (T)sg * a.y), h * ((T)sg * a.x + a.y)};
+    } else if constexpr (M == 6) {
+        const T h = (T)H;
+        return {h * (-a.x - (T)sg * a.y), h * ((T)sg * a.x - a.y)};
+    } else {
+        constexpr double c = (M == 1) ? C1 : (M == 3) ? S1 : (M == 5) ? -S1 : (M == 7) ? -C1 : (M == 9) ? -C1 : 0.0;
+        constexpr double s = (M == 1) ? S1 : (M == 3) ? C1 : (M == 5) ? C1 : (M == 7) ? S1 : (M == 9) ? -S1 : 0.0;
+        const T cc = (T)c, ss = (T)(sg * s);
+        return {a.x * cc - a.y * ss, a.x * ss + a.y * cc};
+    }
+}
+
+template <int S, typename T>
+__device__ __forceinline__ void dft8(cpx<T>* v) {
+    cpx<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    cpx<T> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<S>(e0, e1, e2, e3);
+    dft4<S>(o0, o1, o2, o3);
+    o1 = rot16<S, 2>(o1);
+    o2 = rot16<S, 4>(o2);
+    o3 = rot16<S, 6>(o3);
+    v[0] = e0 + o0;
+    v[4] = e0 - o0;
+    v[1] = e1 + o1;
+    v[5] = e1 - o1;
+    v[2] = e2 + o2;
+    v[6] = e2 - o2;
+    v[3] = e3 + o3;
+    v[7] = e3 - o3;
+}
+
+// 16 = 4 x 4: DFT4 down the columns j1 of x[4 j1 + j2], twiddle by
+// w16^(j2 k1), DFT4 along j2, output X[k1 + 4 k2].
+template <int S, typename T>
+__device__ __forceinline__ void dft16(cpx<T>* v) {
+#pragma unroll
+    for (int j2 = 0; j2 < 4; ++j2) dft4<S>(v[j2], v[4 + j2], v[8 + j2], v[12 + j2]);
+    // v[j2 + 4 k1] = Y[j2][k1]
+    v[5] = rot16<S, 1>(v[5]);
+    v[6] = rot16<S, 2>(v[6]);
+    v[7] = rot16<S, 3>(v[7]);
+    v[9] = rot16<S, 2>(v[9]);
+    v[10] = rot16<S, 4>(v[10]);
+    v[11] = rot16<S, 6>(v[11]);
+    v[13] = rot16<S, 3>(v[13]);
+    v[14] = rot16<S, 6>(v[14]);
+    v[15] = rot16<S, 9>(v[15]);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) dft4<S>(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]);
+    // v[4 k1 + k2] = X[k1 + 4 k2]; transpose the 4x4 index to natural order.
+    cpx<T> t[16];
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) t[k1 + 4 * k2] = v[4 * k1 + k2];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = t[i];
+}
+
+template <int R, int S, typename T>
+__device__ __forceinline__ void dft(cpx<T>* v) {
+    if constexpr (R == 1) {
+    } else if constexpr (R == 2) dft2<S>(v);
+    else if constexpr (R == 4) dft4<S>(v);
+    else if constexpr (R == 8) dft8<S>(v);
+    else if constexpr (R == 16) dft16<S>(v);
+    else static_assert(R == 2, "radix must be 2, 4, 8 or 16");
+}
+
+// ---------------------------------------------------------------------------
+// Radix plan: log2(L) bits covered by radix-16 stages, the remainder (2, 4 or
+// 8) taken first.  L < 16 is a single stage.
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int ilog2c(long long v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
+
+template <int L>
+struct RadixPlan {
+    static constexpr int LOG = ilog2c(L);
+    static constexpr int NST = (L < 16) ? 1 : (LOG + 3) / 4;
+    static constexpr int radix(int s) {
+        return (L < 16) ? L : (s == 0 ? ((LOG % 4) == 0 ? 16 : (1 << (LOG % 4))) : 16);
+    }
+    static constexpr int ns(int s) {
+        int p = 1;
+        for (int i = 0; i < s; ++i) p *= radix(i);
+        return p;
+    }
+    static constexpr int EPT_MIN = (L < 16) ? L : 16;
+};
+
+enum { MAP_J = 0, MAP_T = 1 };
+
+// Twiddle tables (device memory, built in FP64 on the GPU):
+//   coarse[m] = exp(2 pi i m / 2^13),       m < 2^13
+//   fine[m]   = exp(2 pi i m / 2^26),       m < 2^13
+// exp(2 pi i M / 2^26) = coarse[M >> 13] * fine[M & 8191] for M < 2^26.
+constexpr int TW_LOG_COARSE = 13;
+constexpr int TW_COARSE = 1 << TW_LOG_COARSE;
+constexpr int TW_LOG_VIRT = 26;
+
+template <typename T>
+struct Twiddles {
+    const cpx<T>* coarse;
+    const cpx<T>* fine;
+};
+
+// exp(S 2 pi i m / 2^logn), 0 <= m < 2^logn <= 2^26.
+template <int S, typename T>
+__device__ __forceinline__ cpx<T> twiddle_big(const Twiddles<T>& tw, uint32_t m, int logn) {
+    uint32_t M = m << (TW_LOG_VIRT - logn);
+    cpx<T> w = cmul(tw.coarse[M >> TW_LOG_COARSE], tw.fine[M & (TW_COARSE - 1)]);
+    return dirw<S>(w);
+}
+
+// ---------------------------------------------------------------------------
+// The stage engine.
+// ---------------------------------------------------------------------------
+template <typename T, int S, int L, int TI, int NT, int MAPL, int MAPS>
+struct Engine {
+    using P = RadixPlan<L>;
+    static constexpr int EPT = TI * L / NT;
+    static_assert(TI * L % NT == 0, "tile must split evenly over threads");
+    static constexpr int NST = P::NST;
+    static_assert(EPT % P::EPT_MIN == 0 || L < 16, "EPT must cover the largest radix");
+    // LDS layout follows the mapping used after the first stage.
+    static constexpr int LAYOUT = (NST == 1) ? MAPL : MAPS;
+    static constexpr int map_of(int s) { return s == 0 ? MAPL : MAPS; }
+
+    // Padded LDS index: one pad slot per 16 complex values breaks the
+    // power-of-two strides of the early Stockham stages across banks.
+    static __device__ __forceinline__ int pad(int a) { return a + (a >> 4); }
+    static constexpr int LDS_ELEMS = TI * L + (TI * L) / 16 + 1;
+
+    static __device__ __forceinline__ int lidx(int t, int i) {
+        if constexpr (LAYOUT == MAP_T) return pad(i * TI + t);
+        else return pad(t * L + i);
+    }
+
+    template <int MAP, int NB>
+    static __device__ __forceinline__ void bmap(int q, int& t, int& j) {
+        if constexpr (MAP == MAP_J) {
+            t = q / NB;
+            j = q % NB;
+        } else {
+            t = q % TI;
+            j = q / TI;
+        }
+    }
+
+    // Stage-0 inputs straight from global memory through a load functor
+    // ld(t, i) -> cpx<T> (element i of tile transform t).
+    template <class LD>
+    static __device__ __forceinline__ void load0(cpx<T>* v, int tid, LD&& ld) {
+        constexpr int R = P::radix(0), NB = L / R, BPT = EPT / R;
+#pragma unroll
+        for (int b = 0; b < BPT; ++b) {
+            int t, j;
+            bmap<MAPL, NB>(tid + b * NT, t, j);
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b * R + r] = ld(t, j + r * NB);
+        }
+    }
+
+    // Stage-s inputs from an LDS buffer holding the previous stage's output
+    // (or any natural-order data when s == 0).
+    template <int s>
+    static __device__ __forceinline__ void load_lds(cpx<T>* v, int tid, const cpx<T>* lds) {
+        constexpr int R = P::radix(s), NB = L / R, BPT = EPT / R;
+#pragma unroll
+        for (int b = 0; b < BPT; ++b) {
+            int t, j;
+            bmap<map_of(s), NB>(tid + b * NT, t, j);
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b * R + r] = lds[lidx(t, j + r * NB)];
+        }
+    }
+
+    template <int s>
+    static __device__ __forceinline__ void compute(cpx<T>* v, int tid, const cpx<T>* __restrict__ coarse) {
+        constexpr int R = P::radix(s), NB = L / R, BPT = EPT / R, NS = P::ns(s);
+#pragma unroll
+        for (int b = 0; b < BPT; ++b) {
+            if constexpr (NS > 1) {
+                int t, j;
+                bmap<map_of(s), NB>(tid + b * NT, t, j);
+                const int k = j % NS;
+                // W_{NS*R}^{r k} = coarse[r k 2^13 / (NS R)]
+                constexpr int STEP = TW_COARSE / (NS * R);
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b * R + r] = cmul(v[b * R + r], dirw<S>(coarse[r * k * STEP]));
+            }
+            dft<R, S>(v + b * R);
+        }
+    }
+
+    // Stage-s outputs to LDS at their Stockham positions.
+    template <int s>
+    static __device__ __forceinline__ void store_lds(const cpx<T>* v, int tid, cpx<T>* lds) {
+        constexpr int R = P::radix(s), NB = L / R, BPT = EPT / R, NS = P::ns(s);
+#pragma unroll
+        for (int b = 0; b < BPT; ++b) {
+            int t, j;
+            bmap<map_of(s), NB>(tid + b * NT, t, j);
+            const int base = (j / NS) * NS * R + (j % NS);
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[lidx(t, base + r * NS)] = v[b * R + r];
+        }
+    }
+
+    // Last-stage outputs (natural order k = j + r L/R) through a store
+    // functor st(t, k, value).
+    template <class ST>
+    static __device__ __forceinline__ void store_last(const cpx<T>* v, int tid, ST&& st) {
+        constexpr int s = NST - 1;
+        constexpr int R = P::radix(s), NB = L / R, BPT = EPT / R;
+#pragma unroll
+        for (int b = 0; b < BPT; ++b) {
+            int t, j;
+            bmap<map_of(s), NB>(tid + b * NT, t, j);
+#pragma unroll
+            for (int r = 0; r < R; ++r) st(t, j + r * NB, v[b * R + r]);
+        }
+    }
+
+    // Stages 1..NST-1 (stage 0 has been computed on v).  The leading barrier
+    // of each exchange protects the LDS buffer against the previous stage's
+    // readers (WAR); the trailing one publishes the writes (RAW).
+    template <int s = 1>
+    static __device__ __forceinline__ void middle(cpx<T>* v, int tid, cpx<T>* lds, const cpx<T>* coarse,
+                                                  bool lead_barrier) {
+        if constexpr (s < NST) {
+            if (lead_barrier) __syncthreads();
+            store_lds<s - 1>(v, tid, lds);
+            __syncthreads();
+            load_lds<s>(v, tid, lds);
+            compute<s>(v, tid, coarse);
+            middle<s + 1>(v, tid, lds, coarse, true);
+        }
+    }
+
+    // Whole transform: load0 -> stages -> store_last.
+    template <class LD, class ST>
+    static __device__ __forceinline__ void run(int tid, cpx<T>* lds, const cpx<T>* coarse, LD&& ld, ST&& st) {
+        cpx<T> v[EPT];
+        load0(v, tid, ld);
+        compute<0>(v, tid, coarse);
+        middle<1>(v, tid, lds, coarse, false);
+        store_last(v, tid, st);
+    }
+
+    // Transform whose output is left in LDS in natural order (index lidx(t, k)).
+    template <class LD>
+    static __device__ __forceinline__ void run_to_lds(int tid, cpx<T>* lds, const cpx<T>* coarse, LD&& ld) {
+        cpx<T> v[EPT];
+        load0(v, tid, ld);
+        compute<0>(v, tid, coarse);
+        middle<1>(v, tid, lds, coarse, false);
+        if constexpr (NST > 1) __syncthreads();
+        store_lds<NST - 1>(v, tid, lds);
+        __syncthreads();
+    }
+
+    // Transform whose natural-order input is already in LDS (a barrier must
+    // separate the writes of that input from this call).
+    template <class ST>
+    static __device__ __forceinline__ void run_from_lds(int tid, cpx<T>* lds, const cpx<T>* coarse, ST&& st) {
+        cpx<T> v[EPT];
+        load_lds<0>(v, tid, lds);
+        compute<0>(v, tid, coarse);
+        middle<1>(v, tid, lds, coarse, true);
+        store_last(v, tid, st);
+    }
+};
+
+}  // namespace nrfft

@@ -1,0 +1,113 @@
+// internal.hpp -- declarations shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "pass_c2c.hpp"
+
+namespace nrfft {
+
+const KernelRef& c2c_lookup_f32p(int mode, int logl, int tilog);
+const KernelRef& c2c_lookup_f32m(int mode, int logl, int tilog);
+const KernelRef& c2c_lookup_f64p(int mode, int logl, int tilog);
+const KernelRef& c2c_lookup_f64m(int mode, int logl, int tilog);
+const KernelRef& c2c_lookup(int dtype, int sign, int mode, int logl, int tilog);
+int pick_tilog(int dtype, int mode, int logl, int64_t nb0, int want_log);
+
+// Per-device state: twiddle tables (FP64-generated; FP32 copies rounded).
+struct DeviceCtx {
+    int device = 0;
+    c64* c64c = nullptr;
+    c64* c64f = nullptr;
+    c32* c32c = nullptr;
+    c32* c32f = nullptr;
+};
+int get_ctx(DeviceCtx** out);
+
+// Buffers a launch reads or writes.
+enum BufId { B_IN = 0, B_OUT = 1, B_TMP = 2, B_AUX = 3, B_TMP2 = 4, B_COUNT = 5 };
+// Plan kinds.
+enum PlanKind { PK_C2C = 0, PK_REALFT = 1, PK_CONVLV = 2, PK_CORREL = 3 };
+// Launch kinds.
+enum LaunchKind { LK_C2C = 0, LK_COPY = 1, LK_REAL = 2 };
+
+// Tunables chosen per (kind, N, batch, dtype); -1 = default.
+struct Tune {
+    int split_log1 = -1;  // four-step split: log2 N1 (column pass length)
+    int tilog_a = -1;     // tile width (log2) of column passes
+    int tilog_b = -1;     // tile width (log2) of transpose passes
+    int chunk = -1;       // transforms per four-step chunk (0/-1 = whole batch)
+};
+
+struct Launch {
+    int kind = LK_C2C;
+    const void* fn[2] = {nullptr, nullptr};  // [0] isign = +1, [1] isign = -1
+    unsigned grid = 0;
+    int nt = 0;
+    int lds = 0;
+    int logl = 0;
+    int ti = 0;
+    int mode = 0;
+    PassDesc desc{};
+    int src = B_IN;
+    int dst = B_OUT;
+    size_t bytes = 0;   // LK_COPY
+    int special = 0;    // LK_REAL: which real-FFT kernel
+    // args[0..5]: kernel arguments of LK_REAL launches.
+    // args[6]: if non-zero, the isign this launch always runs with.
+    // args[7]: if non-zero, the launch runs only for calls with that isign sign.
+    int64_t args[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+struct Plan {
+    int kind = PK_C2C;
+    int dtype = 0;
+    std::vector<size_t> nn;
+    size_t batch = 1;
+    size_t extra = 0;  // convlv: response length m
+    int device = 0;
+    Tune tune;
+    std::vector<Launch> launches;
+    size_t tmp_elems = 0;    // complex elements of workspace
+    size_t tmp2_offset = 0;  // B_TMP2 = workspace + tmp2_offset complex elements
+};
+
+extern thread_local std::string g_err;
+int fail(int code, const std::string& msg);
+bool is_pow2(size_t n);
+int ilog2(size_t n);
+size_t csize(int dtype);
+int build_plan(Plan& p);
+int build_plan_real(Plan& p);
+int exec_special(const Plan& p, const Launch& l, void* const* bufs, int isign, hipStream_t stream, DeviceCtx* ctx);
+void tune_lookup(Plan& p);
+int plan_c2c_1d(Plan& p, int logn, int64_t batch, int src, int dst, size_t* tmp_elems);
+int plan_add_c2c_pass(Plan& p, int mode, int logl, int tilog, const PassDesc& d, int src, int dst);
+int exec_plan(const Plan& p, const void* in, void* out, void* tmp, const void* aux, int isign, hipStream_t stream);
+int cached_plan(int kind, int dtype, const std::vector<size_t>& nn, size_t batch, size_t extra,
+                std::shared_ptr<Plan>* out);
+
+// Stream-ordered device scratch of the host-pointer calls.
+struct DevBuf {
+    void* p = nullptr;
+    hipStream_t s = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFreeAsync(p, s);
+    }
+};
+int dev_alloc(DevBuf& b, size_t bytes, hipStream_t s);
+
+}  // namespace nrfft
+
+struct nrfft_plan_s {
+    nrfft::Plan plan;
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+};
+
+namespace nrfft {
+int make_plan(nrfft_plan_s** plan, int kind, int dtype, std::vector<size_t> nn, size_t batch, size_t extra);
+}

@@ -1,0 +1,499 @@
+// real_api.hip -- realft / convlv / correl: kernel instantiations, planning,
+// execution and the extern "C" entry points of include/nrfft.h.
+#include <cstring>
+#include <memory>
+#include <utility>
+#include <vector>
+
+#include "../../include/nrfft.h"
+#include "c2c_instances.hpp"
+#include "internal.hpp"
+#include "real_fft.hpp"
+
+namespace nrfft {
+
+#define HIPCHK(x)                                                                                      \
+    do {                                                                                               \
+        hipError_t e_ = (x);                                                                           \
+        if (e_ != hipSuccess) return fail(NRFFT_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int REAL_MAX_LOGL = 13;  // fused single-group path: n/2 <= 8192
+
+// ---------------------------------------------------------------------------
+// Simple kernels of the multi-pass path (n/2 > 8192) and of small sizes.
+// ---------------------------------------------------------------------------
+// In-place untangle of packed spectra, one thread per bin pair.
+template <typename T, int DIR>
+__global__ void k_untangle(cpx<T>* data, int64_t L, int64_t nsig, int logn, Twiddles<T> tw) {
+    const int64_t np = L / 2;
+    const int64_t total = np * nsig;
+    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < total; q += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t s = q / np, p = q % np;
+        cpx<T>* d = data + s * L;
+        if (p == 0) {
+            d[0] = DIR > 0 ? dc_forward(d[0]) : dc_inverse(d[0]);
+        } else {
+            cpx<T> a, b;
+            untangle_pair(d[p], d[L - p], twiddle_big<DIR>(tw, (uint32_t)p, logn), (T)(-0.5 * DIR), a, b);
+            d[p] = a;
+            d[L - p] = b;
+        }
+    }
+}
+
+// Padded responses (Convolve.rs:41-63) for nsig signals.
+template <typename T>
+__global__ void k_place(const T* r, T* out, int64_t n, int64_t m, int64_t nsig) {
+    const int64_t total = n * nsig;
+    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < total; q += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t s = q / n, e = q % n;
+        out[q] = placed_response(r + s * m, e, n, m);
+    }
+}
+
+// a <- inverse-untangle(op(a, b) / no2) on forward-untangled packed spectra.
+template <typename T>
+__global__ void k_spec(cpx<T>* a, const cpx<T>* b, int64_t L, int64_t nsig, int op, int logn, Twiddles<T> tw) {
+    const int64_t np = L / 2;
+    const int64_t total = np * nsig;
+    const T inv_no2 = (T)1 / (T)L;
+    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < total; q += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t s = q / np, p = q % np;
+        cpx<T>* x = a + s * L;
+        const cpx<T>* y = b + s * L;
+        if (p == 0) {
+            cpx<T> y0{spec_apply_real(op, x[0].x, y[0].x, inv_no2), spec_apply_real(op, x[0].y, y[0].y, inv_no2)};
+            x[0] = dc_inverse(y0);
+            x[np] = spec_apply(op, x[np], y[np], inv_no2);
+        } else {
+            const cpx<T> w = twiddle_big<1>(tw, (uint32_t)p, logn);
+            const cpx<T> ya = spec_apply(op, x[p], y[p], inv_no2);
+            const cpx<T> yb = spec_apply(op, x[L - p], y[L - p], inv_no2);
+            cpx<T> o1, o2;
+            untangle_pair(ya, yb, conj(w), (T)0.5, o1, o2);
+            x[p] = o1;
+            x[L - p] = o2;
+        }
+    }
+}
+
+// n = 2 realft: [x0, x1] <-> [x0 + x1, x0 - x1] (inverse returns (n/2) x = x).
+template <typename T, int DIR>
+__global__ void k_real_n2(const cpx<T>* in, cpx<T>* out, int64_t nsig) {
+    const int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (s < nsig) out[s] = DIR > 0 ? dc_forward(in[s]) : dc_inverse(in[s]);
+}
+
+// correl for n <= 32: linear correlation at lags 0..n-1 (Correlation.rs:37-50).
+template <typename T>
+__global__ void k_correl_direct(const T* d1, const T* d2, T* ans, int64_t n, int64_t npairs) {
+    const int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (q >= n * npairs) return;
+    const int64_t s = q / n, lag = q % n;
+    const T* a = d1 + s * n;
+    const T* b = d2 + s * n;
+    T sum = (T)0;
+    for (int64_t i = 0; i < n - lag; ++i) sum += a[i + lag] * b[i];
+    ans[q] = sum;
+}
+
+// Direct circular convolution / deconvolution for n = 2 (n/2 = 1 has no FFT):
+// spectrum [x0 + x1, x0 - x1], both bins real.
+template <typename T>
+__global__ void k_convlv_n2(const T* data, const T* second, T* ans, int64_t m, int64_t nsig, int op) {
+    const int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (s >= nsig) return;
+    const T* x = data + 2 * s;
+    T r0, r1;
+    if (op == OP_CORREL) {
+        r0 = second[2 * s];
+        r1 = second[2 * s + 1];
+    } else {
+        r0 = placed_response(second + s * m, 0, 2, m);
+        r1 = placed_response(second + s * m, 1, 2, m);
+    }
+    const T f0 = x[0] + x[1], f1 = x[0] - x[1];
+    const T g0 = r0 + r1, g1 = r0 - r1;
+    const T y0 = spec_apply_real(op, f0, g0, (T)1), y1 = spec_apply_real(op, f1, g1, (T)1);
+    ans[2 * s] = (T)0.5 * (y0 + y1);
+    ans[2 * s + 1] = (T)0.5 * (y0 - y1);
+}
+
+// ---------------------------------------------------------------------------
+// Registry of the fused kernels.
+// ---------------------------------------------------------------------------
+struct RealRef {
+    const void* fwd;
+    const void* inv;
+    const void* conv;
+    int nt, ti;
+};
+
+template <typename T, int LOGL>
+RealRef real_ref_one() {
+    constexpr int L = 1 << LOGL;
+    constexpr int TI = ti_row<T, L>();
+    constexpr int EPT = L < 16 ? L : 16;
+    constexpr int NT = TI * L / EPT;
+    return RealRef{reinterpret_cast<const void*>(&k_realft<T, L, TI, NT, 1>),
+                   reinterpret_cast<const void*>(&k_realft<T, L, TI, NT, -1>),
+                   reinterpret_cast<const void*>(&k_convlv<T, L, TI, NT>), NT, TI};
+}
+
+template <typename T, int... LS>
+void real_fill(RealRef* tab, std::integer_sequence<int, LS...>) {
+    ((tab[LS + 1] = real_ref_one<T, LS + 1>()), ...);
+}
+
+template <typename T>
+struct RealTable {
+    RealRef t[REAL_MAX_LOGL + 1];
+    RealTable() {
+        t[0] = RealRef{nullptr, nullptr, nullptr, 0, 0};
+        real_fill<T>(t, std::make_integer_sequence<int, REAL_MAX_LOGL>{});
+    }
+};
+
+static const RealRef& real_lookup(int dtype, int logl) {
+    static const RealTable<float> tf;
+    static const RealTable<double> td;
+    return dtype == NRFFT_F32 ? tf.t[logl] : td.t[logl];
+}
+
+// Kinds of LK_REAL launches.
+enum RealSpecial {
+    RS_FUSED_REALFT = 1,  // args: nsig, L
+    RS_FUSED_CONV = 2,    // args: nsig, L, m
+    RS_UNTANGLE = 3,      // args: nsig, L, dir
+    RS_PLACE = 4,         // args: nsig, n, m
+    RS_SPEC = 5,          // args: nsig, L
+    RS_N2 = 6,            // args: nsig, dir(0 = both directions by isign)
+    RS_CORREL_DIRECT = 7, // args: npairs, n
+    RS_CONV_N2 = 8,       // args: nsig, m
+};
+
+static Launch real_launch(int special, int src, int dst, std::initializer_list<int64_t> args) {
+    Launch l;
+    l.kind = LK_REAL;
+    l.special = special;
+    l.src = src;
+    l.dst = dst;
+    int i = 0;
+    for (int64_t a : args) l.args[i++] = a;
+    return l;
+}
+
+int build_plan_real(Plan& p) {
+    const size_t n = p.nn.at(0);
+    const int64_t batch = (int64_t)p.batch;
+    if (p.kind == PK_CORREL && n <= 32) {
+        p.launches.push_back(real_launch(RS_CORREL_DIRECT, B_IN, B_OUT, {batch, (int64_t)n}));
+        return NRFFT_OK;
+    }
+    if (!is_pow2(n) || n < 2) return fail(NRFFT_E_NOT_POW2, "n must be a power of two >= 2");
+    const int64_t L = (int64_t)n / 2;
+    const int logl = ilog2((size_t)L);
+    if (p.kind == PK_REALFT) {
+        if (L == 1) {
+            p.launches.push_back(real_launch(RS_N2, B_IN, B_OUT, {batch}));
+        } else if (logl <= REAL_MAX_LOGL) {
+            p.launches.push_back(real_launch(RS_FUSED_REALFT, B_IN, B_OUT, {batch, L}));
+        } else {
+            // Forward: C2C(+1) then untangle; inverse: untangle^-1 then C2C(-1).
+            // The direction-dependent order is resolved at execution time: the
+            // plan holds both sequences, tagged by args[3] (1 = fwd only, -1 = inv only).
+            size_t tmp = 0;
+            Plan fw = p, iv = p;
+            fw.launches.clear();
+            iv.launches.clear();
+            int rc = plan_c2c_1d(fw, logl, batch, B_IN, B_OUT, &tmp);
+            if (rc) return rc;
+            fw.launches.push_back(real_launch(RS_UNTANGLE, B_OUT, B_OUT, {batch, L, 1}));
+            iv.launches.push_back(real_launch(RS_UNTANGLE, B_IN, B_OUT, {batch, L, -1}));
+            rc = plan_c2c_1d(iv, logl, batch, B_OUT, B_OUT, &tmp);
+            if (rc) return rc;
+            for (auto& l : fw.launches) {
+                l.args[7] = 1;
+                p.launches.push_back(l);
+            }
+            for (auto& l : iv.launches) {
+                l.args[7] = -1;
+                p.launches.push_back(l);
+            }
+            p.tmp_elems = tmp;
+        }
+        return NRFFT_OK;
+    }
+    // convlv / correl
+    const int64_t m = (int64_t)p.extra;
+    if (L == 1) {
+        p.launches.push_back(real_launch(RS_CONV_N2, B_IN, B_OUT, {batch, m}));
+        return NRFFT_OK;
+    }
+    if (logl <= REAL_MAX_LOGL) {
+        p.launches.push_back(real_launch(RS_FUSED_CONV, B_IN, B_OUT, {batch, L, m}));
+        return NRFFT_OK;
+    }
+    // Multi-pass: spectra of the signal (into out) and of the response
+    // (into TMP2), spectrum op + inverse untangle, inverse C2C.
+    size_t tmp = 0;
+    int rc;
+    if (p.kind == PK_CONVLV) {
+        p.launches.push_back(real_launch(RS_PLACE, B_AUX, B_TMP2, {batch, (int64_t)n, m}));
+        rc = plan_c2c_1d(p, logl, batch, B_TMP2, B_TMP2, &tmp);
+    } else {
+        rc = plan_c2c_1d(p, logl, batch, B_AUX, B_TMP2, &tmp);
+    }
+    if (rc) return rc;
+    p.launches.push_back(real_launch(RS_UNTANGLE, B_TMP2, B_TMP2, {batch, L, 1}));
+    if ((rc = plan_c2c_1d(p, logl, batch, B_IN, B_OUT, &tmp))) return rc;
+    p.launches.push_back(real_launch(RS_UNTANGLE, B_OUT, B_OUT, {batch, L, 1}));
+    p.launches.push_back(real_launch(RS_SPEC, B_OUT, B_OUT, {batch, L}));
+    // The inverse C2C of the last step runs with isign = -1 whatever isign
+    // the call carries (isign selects conv/deconv); args[6] marks it.
+    const size_t before = p.launches.size();
+    if ((rc = plan_c2c_1d(p, logl, batch, B_OUT, B_OUT, &tmp))) return rc;
+    for (size_t i = before; i < p.launches.size(); ++i) p.launches[i].args[6] = -1;
+    // TMP2 (response spectrum) follows the C2C scratch in the workspace.
+    p.tmp2_offset = std::max<size_t>(tmp, size_t(batch) * size_t(L));
+    p.tmp_elems = p.tmp2_offset + size_t(batch) * size_t(L);
+    return NRFFT_OK;
+}
+
+static unsigned grid_for(int64_t work, int nt) {
+    int64_t g = (work + nt - 1) / nt;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
+template <typename T>
+static int exec_real_t(const Plan& p, const Launch& l, void* const* bufs, int isign, hipStream_t stream,
+                       DeviceCtx* ctx) {
+    Twiddles<T> tw;
+    if constexpr (sizeof(T) == 4) tw = Twiddles<T>{ctx->c32c, ctx->c32f};
+    else tw = Twiddles<T>{ctx->c64c, ctx->c64f};
+    void* src = bufs[l.src];
+    void* dst = bufs[l.dst];
+    const int64_t* a = l.args;
+    switch (l.special) {
+        case RS_FUSED_REALFT: {
+            const RealRef& r = real_lookup(p.dtype, ilog2((size_t)a[1]));
+            int64_t nsig = a[0];
+            void* args[] = {&src, &dst, &nsig, &tw};
+            unsigned grid = (unsigned)((nsig + r.ti - 1) / r.ti);
+            HIPCHK(hipLaunchKernel(isign > 0 ? r.fwd : r.inv, dim3(grid), dim3(r.nt), args, 0, stream));
+            return NRFFT_OK;
+        }
+        case RS_FUSED_CONV: {
+            const RealRef& r = real_lookup(p.dtype, ilog2((size_t)a[1]));
+            int64_t nsig = a[0], m = a[2];
+            int op = p.kind == PK_CORREL ? OP_CORREL : (isign > 0 ? OP_CONV : OP_DECONV);
+            void* aux = bufs[B_AUX];
+            void* args[] = {&src, &aux, &dst, &nsig, &m, &op, &tw};
+            unsigned grid = (unsigned)((nsig + r.ti - 1) / r.ti);
+            HIPCHK(hipLaunchKernel(r.conv, dim3(grid), dim3(r.nt), args, 0, stream));
+            return NRFFT_OK;
+        }
+        case RS_UNTANGLE: {
+            if (src != dst) HIPCHK(hipMemcpyAsync(dst, src, size_t(a[0] * a[1]) * sizeof(cpx<T>),
+                                                  hipMemcpyDeviceToDevice, stream));
+            const int logn = ilog2((size_t)a[1]) + 1;
+            if (a[2] > 0)
+                hipLaunchKernelGGL((k_untangle<T, 1>), dim3(grid_for(a[0] * a[1] / 2, 256)), dim3(256), 0, stream,
+                                   (cpx<T>*)dst, a[1], a[0], logn, tw);
+            else
+                hipLaunchKernelGGL((k_untangle<T, -1>), dim3(grid_for(a[0] * a[1] / 2, 256)), dim3(256), 0, stream,
+                                   (cpx<T>*)dst, a[1], a[0], logn, tw);
+            HIPCHK(hipGetLastError());
+            return NRFFT_OK;
+        }
+        case RS_PLACE:
+            hipLaunchKernelGGL((k_place<T>), dim3(grid_for(a[0] * a[1], 256)), dim3(256), 0, stream,
+                               (const T*)src, (T*)dst, a[1], a[2], a[0]);
+            HIPCHK(hipGetLastError());
+            return NRFFT_OK;
+        case RS_SPEC: {
+            int op = p.kind == PK_CORREL ? OP_CORREL : (isign > 0 ? OP_CONV : OP_DECONV);
+            const int logn = ilog2((size_t)a[1]) + 1;
+            hipLaunchKernelGGL((k_spec<T>), dim3(grid_for(a[0] * a[1] / 2, 256)), dim3(256), 0, stream, (cpx<T>*)dst,
+                               (const cpx<T>*)bufs[B_TMP2], a[1], a[0], op, logn, tw);
+            HIPCHK(hipGetLastError());
+            return NRFFT_OK;
+        }
+        case RS_N2:
+            if (isign > 0)
+                hipLaunchKernelGGL((k_real_n2<T, 1>), dim3(grid_for(a[0], 256)), dim3(256), 0, stream,
+                                   (const cpx<T>*)src, (cpx<T>*)dst, a[0]);
+            else
+                hipLaunchKernelGGL((k_real_n2<T, -1>), dim3(grid_for(a[0], 256)), dim3(256), 0, stream,
+                                   (const cpx<T>*)src, (cpx<T>*)dst, a[0]);
+            HIPCHK(hipGetLastError());
+            return NRFFT_OK;
+        case RS_CORREL_DIRECT:
+            hipLaunchKernelGGL((k_correl_direct<T>), dim3(grid_for(a[0] * a[1], 256)), dim3(256), 0, stream,
+                               (const T*)src, (const T*)bufs[B_AUX], (T*)dst, a[1], a[0]);
+            HIPCHK(hipGetLastError());
+            return NRFFT_OK;
+        case RS_CONV_N2: {
+            int op = p.kind == PK_CORREL ? OP_CORREL : (isign > 0 ? OP_CONV : OP_DECONV);
+            hipLaunchKernelGGL((k_convlv_n2<T>), dim3(grid_for(a[0], 256)), dim3(256), 0, stream, (const T*)src,
+                               (const T*)bufs[B_AUX], (T*)dst, a[1], a[0], op);
+            HIPCHK(hipGetLastError());
+            return NRFFT_OK;
+        }
+        default:
+            return fail(NRFFT_E_INVALID, "unknown launch");
+    }
+}
+
+int exec_special(const Plan& p, const Launch& l, void* const* bufs, int isign, hipStream_t stream, DeviceCtx* ctx) {
+    if (p.dtype == NRFFT_F32) return exec_real_t<float>(p, l, bufs, isign, stream, ctx);
+    return exec_real_t<double>(p, l, bufs, isign, stream, ctx);
+}
+
+// ---------------------------------------------------------------------------
+// Host-pointer calls.
+// ---------------------------------------------------------------------------
+int host_real(int kind, int dtype, const void* in, const void* aux, size_t aux_bytes, void* out, size_t n, size_t m,
+              size_t batch, int isign) {
+    std::shared_ptr<Plan> p;
+    int rc = cached_plan(kind, dtype, {n}, batch, m, &p);
+    if (rc) return rc;
+    const size_t es = dtype == NRFFT_F32 ? 4 : 8;
+    const size_t bytes = n * batch * es;
+    hipStream_t s = hipStreamPerThread;
+    DevBuf dbuf, dout, daux, dtmp;
+    if ((rc = dev_alloc(dbuf, bytes, s))) return rc;
+    if ((rc = dev_alloc(dout, bytes, s))) return rc;
+    if (aux_bytes && (rc = dev_alloc(daux, aux_bytes, s))) return rc;
+    if ((rc = dev_alloc(dtmp, p->tmp_elems * csize(dtype), s))) return rc;
+    HIPCHK(hipMemcpyAsync(dbuf.p, in, bytes, hipMemcpyHostToDevice, s));
+    if (aux_bytes) HIPCHK(hipMemcpyAsync(daux.p, aux, aux_bytes, hipMemcpyHostToDevice, s));
+    if ((rc = exec_plan(*p, dbuf.p, dout.p, dtmp.p, daux.p, isign, s))) return rc;
+    HIPCHK(hipMemcpyAsync(out, dout.p, bytes, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return NRFFT_OK;
+}
+
+}  // namespace nrfft
+
+using namespace nrfft;
+
+static int check_realft(const void* data, size_t n, int isign) {
+    if (isign != 1 && isign != -1) return fail(NRFFT_E_ISIGN, "isign must be 1 or -1");
+    if (n == 0) return fail(NRFFT_E_EMPTY, "n == 0");
+    if (!data) return fail(NRFFT_E_INVALID, "null data");
+    // Real_FT.rs:5 asserts n % 2 == 0; four1 underneath needs n/2 a power of two.
+    if (n % 2 != 0 || !is_pow2(n)) return fail(NRFFT_E_NOT_POW2, "n must be an even power of two");
+    return NRFFT_OK;
+}
+
+// Convolve.rs:13-21 order of checks, then the power-of-two requirement.
+static int check_convlv(const void* data, size_t n, const void* respns, size_t m, int isign, const void* ans) {
+    if (n == 0 || m == 0) return fail(NRFFT_E_EMPTY, "Input arrays cannot be empty");
+    if (m > n) return fail(NRFFT_E_RESP_TOO_LONG, "Response function longer than data");
+    if (isign != 1 && isign != -1) return fail(NRFFT_E_ISIGN, "isign must be 1 (convolution) or -1 (deconvolution)");
+    if (!data || !respns || !ans) return fail(NRFFT_E_INVALID, "null pointer");
+    if (!is_pow2(n) || n < 2) return fail(NRFFT_E_NOT_POW2, "n must be a power of two >= 2");
+    return NRFFT_OK;
+}
+
+// Correlation.rs:10-16; n > 32 must be a power of two.
+static int check_correl(const void* d1, const void* d2, size_t n, const void* ans) {
+    if (n == 0) return fail(NRFFT_E_EMPTY, "Input arrays cannot be empty");
+    if (!d1 || !d2 || !ans) return fail(NRFFT_E_INVALID, "null pointer");
+    if (n > 32 && !is_pow2(n)) return fail(NRFFT_E_NOT_POW2, "n > 32 must be a power of two");
+    return NRFFT_OK;
+}
+
+extern "C" {
+
+int nrfft_realft_f64(double* data, size_t n, int isign) {
+    g_err.clear();
+    int rc = check_realft(data, n, isign);
+    return rc ? rc : host_real(PK_REALFT, NRFFT_F64, data, nullptr, 0, data, n, 0, 1, isign);
+}
+int nrfft_realft_f32(float* data, size_t n, int isign) {
+    g_err.clear();
+    int rc = check_realft(data, n, isign);
+    return rc ? rc : host_real(PK_REALFT, NRFFT_F32, data, nullptr, 0, data, n, 0, 1, isign);
+}
+
+int nrfft_convlv_f64(const double* data, size_t n, const double* respns, size_t m, int isign, double* ans) {
+    g_err.clear();
+    int rc = check_convlv(data, n, respns, m, isign, ans);
+    return rc ? rc : host_real(PK_CONVLV, NRFFT_F64, data, respns, m * 8, ans, n, m, 1, isign);
+}
+int nrfft_convlv_f32(const float* data, size_t n, const float* respns, size_t m, int isign, float* ans) {
+    g_err.clear();
+    int rc = check_convlv(data, n, respns, m, isign, ans);
+    return rc ? rc : host_real(PK_CONVLV, NRFFT_F32, data, respns, m * 4, ans, n, m, 1, isign);
+}
+int nrfft_convlv_batch_f64(const double* data, size_t n, const double* respns, size_t m, size_t batch, int isign,
+                           double* ans) {
+    g_err.clear();
+    int rc = check_convlv(data, n, respns, m, isign, ans);
+    if (rc || batch == 0) return rc;
+    return host_real(PK_CONVLV, NRFFT_F64, data, respns, m * batch * 8, ans, n, m, batch, isign);
+}
+int nrfft_convlv_batch_f32(const float* data, size_t n, const float* respns, size_t m, size_t batch, int isign,
+                           float* ans) {
+    g_err.clear();
+    int rc = check_convlv(data, n, respns, m, isign, ans);
+    if (rc || batch == 0) return rc;
+    return host_real(PK_CONVLV, NRFFT_F32, data, respns, m * batch * 4, ans, n, m, batch, isign);
+}
+
+int nrfft_correl_f64(const double* d1, const double* d2, size_t n, double* ans) {
+    g_err.clear();
+    int rc = check_correl(d1, d2, n, ans);
+    return rc ? rc : host_real(PK_CORREL, NRFFT_F64, d1, d2, n * 8, ans, n, 0, 1, 1);
+}
+int nrfft_correl_f32(const float* d1, const float* d2, size_t n, float* ans) {
+    g_err.clear();
+    int rc = check_correl(d1, d2, n, ans);
+    return rc ? rc : host_real(PK_CORREL, NRFFT_F32, d1, d2, n * 4, ans, n, 0, 1, 1);
+}
+
+int nrfft_plan_realft(nrfft_plan* plan, int dtype, size_t n, size_t batch) {
+    g_err.clear();
+    int rc = check_realft(plan, n, 1);
+    return rc ? rc : make_plan(plan, PK_REALFT, dtype, {n}, batch, 0);
+}
+int nrfft_execute_realft(nrfft_plan plan, void* d_data, int isign, void* stream) {
+    g_err.clear();
+    if (!plan || plan->plan.kind != PK_REALFT) return fail(NRFFT_E_INVALID, "not a realft plan");
+    if (isign != 1 && isign != -1) return fail(NRFFT_E_ISIGN, "isign must be 1 or -1");
+    if (!d_data) return fail(NRFFT_E_INVALID, "null buffer");
+    return exec_plan(plan->plan, d_data, d_data, plan->tmp, nullptr, isign, (hipStream_t)stream);
+}
+
+int nrfft_plan_convlv(nrfft_plan* plan, int dtype, size_t n, size_t m, size_t batch) {
+    g_err.clear();
+    int rc = check_convlv(plan, n, plan, m, 1, plan);
+    return rc ? rc : make_plan(plan, PK_CONVLV, dtype, {n}, batch, m);
+}
+int nrfft_execute_convlv(nrfft_plan plan, const void* d_data, const void* d_respns, void* d_ans, int isign,
+                         void* stream) {
+    g_err.clear();
+    if (!plan || plan->plan.kind != PK_CONVLV) return fail(NRFFT_E_INVALID, "not a convlv plan");
+    if (isign != 1 && isign != -1) return fail(NRFFT_E_ISIGN, "isign must be 1 (convolution) or -1 (deconvolution)");
+    if (!d_data || !d_respns || !d_ans) return fail(NRFFT_E_INVALID, "null buffer");
+    return exec_plan(plan->plan, d_data, d_ans, plan->tmp, d_respns, isign, (hipStream_t)stream);
+}
+
+int nrfft_plan_correl(nrfft_plan* plan, int dtype, size_t n, size_t batch) {
+    g_err.clear();
+    int rc = check_correl(plan, plan, n, plan);
+    return rc ? rc : make_plan(plan, PK_CORREL, dtype, {n}, batch, 0);
+}
+int nrfft_execute_correl(nrfft_plan plan, const void* d1, const void* d2, void* d_ans, void* stream) {
+    g_err.clear();
+    if (!plan || plan->plan.kind != PK_CORREL) return fail(NRFFT_E_INVALID, "not a correl plan");
+    if (!d1 || !d2 || !d_ans) return fail(NRFFT_E_INVALID, "null buffer");
+    return exec_plan(plan->plan, d1, d_ans, plan->tmp, d2, 1, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,240 @@
+// real_fft.hpp -- realft / convlv / correl kernels (numrs Real_FT.rs,
+// Convolve.rs, Correlation.rs) on the Stockham engine.
+//
+// A real signal of n = 2L samples is read as L complex values
+// z_j = x_{2j} + i x_{2j+1} (the same bytes), transformed with an L-point
+// complex FFT and "untangled" into the packed real spectrum
+//   [F_0, F_{n/2}, Re F_1, Im F_1, ..., Re F_{L-1}, Im F_{L-1}],
+//   F_k = sum_j x_j e^{+2 pi i jk/n}                           (NR realft)
+// The untangle pairs bins k and L-k; it is fused into the kernel that runs
+// the FFT (the FFT result is left in LDS and one thread handles each pair).
+// convlv/correl keep the signal spectrum in registers while the response
+// spectrum is computed in the same LDS buffer, apply the spectrum operation,
+// untangle back and run the inverse FFT: R2C -> op -> C2R in one launch.
+#pragma once
+#include "device_fft.hpp"
+
+namespace nrfft {
+
+// (A at k, B at L-k) -> (out_k, out_{L-k}) with c2 = -isign/2 and
+// w = exp(isign i pi k / L):
+//   h1 = (A + conj B)/2,  h2 = c2 i (A - conj B),
+//   out_k = h1 + w h2,  out_{L-k} = conj(h1 - w h2).
+// isign = +1 turns the L-point FFT of z into F (Real_FT.rs:23-117 read with
+// NR's indices); isign = -1 turns a packed spectrum into the z' whose
+// inverse L-point FFT is (n/2) x (Real_FT.rs:119-213).
+template <typename T>
+__device__ __forceinline__ void untangle_pair(cpx<T> A, cpx<T> B, cpx<T> w, T c2, cpx<T>& ok, cpx<T>& olk) {
+    const T half = (T)0.5;
+    cpx<T> h1{half * (A.x + B.x), half * (A.y - B.y)};
+    cpx<T> h2{-c2 * (A.y + B.y), c2 * (A.x - B.x)};
+    cpx<T> wh = cmul(w, h2);
+    ok = h1 + wh;
+    olk = conj(h1 - wh);
+}
+
+// Slot 0 of the packed array: forward (Z_0 -> [F_0, F_{n/2}]) and inverse.
+template <typename T>
+__device__ __forceinline__ cpx<T> dc_forward(cpx<T> z0) { return {z0.x + z0.y, z0.x - z0.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> dc_inverse(cpx<T> f) {
+    const T half = (T)0.5;
+    return {half * (f.x + f.y), half * (f.x - f.y)};
+}
+
+// Spectrum operations of convlv (Convolve.rs:90-138, NR semantics: every bin,
+// DC and Nyquist included) and correl (Correlation.rs:68-110).
+enum SpecOp { OP_CONV = 0, OP_DECONV = 1, OP_CORREL = 2 };
+
+template <typename T>
+__device__ __forceinline__ cpx<T> spec_apply(int op, cpx<T> a, cpx<T> b, T inv_no2) {
+    if (op == OP_CONV) return cscale(cmul(a, b), inv_no2);
+    if (op == OP_CORREL) return cscale(cmulc(a, b), inv_no2);
+    // Deconvolution: a conj(b) / |b|^2 / no2, zero where |b|^2 < 1e-12
+    // (Convolve.rs:117-128).
+    const T mag2 = b.x * b.x + b.y * b.y;
+    if (mag2 < (T)1e-12) return cpx<T>{(T)0, (T)0};
+    const cpx<T> q = cmulc(a, b);
+    return {q.x / mag2 * inv_no2, q.y / mag2 * inv_no2};
+}
+// DC / Nyquist slot: both parts are real bins.
+template <typename T>
+__device__ __forceinline__ T spec_apply_real(int op, T a, T b, T inv_no2) {
+    if (op == OP_CONV || op == OP_CORREL) return a * b * inv_no2;
+    const T mag2 = b * b;
+    if (mag2 < (T)1e-12) return (T)0;
+    return a * b / mag2 * inv_no2;
+}
+
+// Response placement of Convolve.rs:41-63 (prepare_response_function),
+// element e of the n-long padded response, mid = (m-1)/2.
+template <typename T>
+__device__ __forceinline__ T placed_response(const T* r, int64_t e, int64_t n, int64_t m) {
+    const int64_t mid = (m - 1) / 2;
+    if (e < mid) return r[m - mid + e];
+    if (e < m) return r[e];
+    if (e < n - mid) return (T)0;
+    return r[e - (n - mid)];
+}
+
+}  // namespace nrfft
+
+namespace nrfft {
+
+// ---------------------------------------------------------------------------
+// realft: TI signals of n = 2L per group, in place or out of place.
+//   DIR = +1: n reals -> packed spectrum;  DIR = -1: packed -> (n/2) x.
+// ---------------------------------------------------------------------------
+template <typename T, int L, int TI, int NT, int DIR>
+__global__ __launch_bounds__(NT) void k_realft(const cpx<T>* in, cpx<T>* out, int64_t nsig, Twiddles<T> tw) {
+    using E = Engine<T, DIR, L, TI, NT, MAP_J, MAP_J>;
+    constexpr int NP = L / 2;  // pairs per signal
+    constexpr int LOG2L = ilog2c(L) + 1;
+    __shared__ cpx<T> lds[E::LDS_ELEMS];
+    const int tid = threadIdx.x;
+    const int64_t s0 = int64_t(blockIdx.x) * TI;
+    const int nvalid = (nsig - s0) < TI ? int(nsig - s0) : TI;
+    const cpx<T>* ib = in + s0 * L;
+    cpx<T>* ob = out + s0 * L;
+    const T c2 = (T)(-0.5 * DIR);
+    if constexpr (DIR > 0) {
+        E::run_to_lds(tid, lds, tw.coarse, [&](int t, int i) -> cpx<T> {
+            return (TI == 1 || t < nvalid) ? ib[t * L + i] : cpx<T>{(T)0, (T)0};
+        });
+        for (int q = tid; q < TI * NP; q += NT) {
+            const int t = q / NP, p = q % NP;
+            if (TI > 1 && t >= nvalid) continue;
+            if (p == 0) {
+                ob[t * L] = dc_forward(lds[E::lidx(t, 0)]);
+                if (L > 1) ob[t * L + NP] = lds[E::lidx(t, NP)];
+            } else {
+                cpx<T> a, b;
+                untangle_pair(lds[E::lidx(t, p)], lds[E::lidx(t, L - p)], twiddle_big<DIR>(tw, p, LOG2L), c2, a, b);
+                ob[t * L + p] = a;
+                ob[t * L + L - p] = b;
+            }
+        }
+    } else {
+        for (int q = tid; q < TI * NP; q += NT) {
+            const int t = q / NP, p = q % NP;
+            const bool ok = TI == 1 || t < nvalid;
+            if (p == 0) {
+                lds[E::lidx(t, 0)] = ok ? dc_inverse(ib[t * L]) : cpx<T>{(T)0, (T)0};
+                lds[E::lidx(t, NP)] = ok ? ib[t * L + NP] : cpx<T>{(T)0, (T)0};
+            } else {
+                cpx<T> a{(T)0, (T)0}, b{(T)0, (T)0};
+                if (ok) untangle_pair(ib[t * L + p], ib[t * L + L - p], twiddle_big<DIR>(tw, p, LOG2L), c2, a, b);
+                lds[E::lidx(t, p)] = a;
+                lds[E::lidx(t, L - p)] = b;
+            }
+        }
+        __syncthreads();
+        E::run_from_lds(tid, lds, tw.coarse, [&](int t, int k, cpx<T> v) {
+            if (TI == 1 || t < nvalid) ob[t * L + k] = v;
+        });
+    }
+}
+
+// ---------------------------------------------------------------------------
+// convlv / deconvolution / correl: one launch, TI signals of n = 2L per
+// group.  `second` is the per-signal response (m values, placed as
+// Convolve.rs:41-63) for OP_CONV / OP_DECONV, or data2 (n values) for
+// OP_CORREL.  ans may alias data.
+// ---------------------------------------------------------------------------
+template <typename T, int L, int TI, int NT>
+__global__ __launch_bounds__(NT) void k_convlv(const cpx<T>* data, const T* second, cpx<T>* ans, int64_t nsig,
+                                               int64_t m, int op, Twiddles<T> tw) {
+    using EF = Engine<T, 1, L, TI, NT, MAP_J, MAP_J>;
+    using EI = Engine<T, -1, L, TI, NT, MAP_J, MAP_J>;
+    static_assert(EF::LDS_ELEMS == EI::LDS_ELEMS, "same layout both ways");
+    constexpr int NP = L / 2;
+    constexpr int PPT = (TI * NP + NT - 1) / NT;
+    constexpr int LOG2N = ilog2c(L) + 1;
+    constexpr int64_t N = 2 * int64_t(L);
+    __shared__ cpx<T> lds[EF::LDS_ELEMS];
+    const int tid = threadIdx.x;
+    const int64_t s0 = int64_t(blockIdx.x) * TI;
+    const int nvalid = (nsig - s0) < TI ? int(nsig - s0) : TI;
+    const cpx<T>* db = data + s0 * L;
+    cpx<T>* ob = ans + s0 * L;
+    const T inv_no2 = (T)1 / (T)L;
+
+    // 1. Spectrum of the signal, kept in registers pair by pair.
+    EF::run_to_lds(tid, lds, tw.coarse, [&](int t, int i) -> cpx<T> {
+        return (TI == 1 || t < nvalid) ? db[t * L + i] : cpx<T>{(T)0, (T)0};
+    });
+    cpx<T> xa[PPT], xb[PPT];
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        const int q = tid + u * NT;
+        if ((TI * NP) % NT != 0 && q >= TI * NP) continue;
+        const int t = q / NP, p = q % NP;
+        if (p == 0) {
+            xa[u] = dc_forward(lds[EF::lidx(t, 0)]);
+            xb[u] = lds[EF::lidx(t, NP)];
+        } else {
+            untangle_pair(lds[EF::lidx(t, p)], lds[EF::lidx(t, L - p)], twiddle_big<1>(tw, p, LOG2N), (T)-0.5,
+                          xa[u], xb[u]);
+        }
+    }
+    __syncthreads();
+
+    // 2. Spectrum of the response (or of data2).
+    if (op == OP_CORREL) {
+        const cpx<T>* sb = reinterpret_cast<const cpx<T>*>(second) + s0 * L;
+        EF::run_to_lds(tid, lds, tw.coarse, [&](int t, int i) -> cpx<T> {
+            return (TI == 1 || t < nvalid) ? sb[t * L + i] : cpx<T>{(T)0, (T)0};
+        });
+    } else {
+        const T* rb = second + s0 * m;
+        EF::run_to_lds(tid, lds, tw.coarse, [&](int t, int i) -> cpx<T> {
+            if (TI > 1 && t >= nvalid) return cpx<T>{(T)0, (T)0};
+            const T* r = rb + t * m;
+            return cpx<T>{placed_response(r, 2 * int64_t(i), N, m), placed_response(r, 2 * int64_t(i) + 1, N, m)};
+        });
+    }
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        const int q = tid + u * NT;
+        if ((TI * NP) % NT != 0 && q >= TI * NP) continue;
+        const int t = q / NP, p = q % NP;
+        if (p == 0) {
+            const cpx<T> r0 = dc_forward(lds[EF::lidx(t, 0)]);
+            const cpx<T> rh = lds[EF::lidx(t, NP)];
+            cpx<T> y0{spec_apply_real(op, xa[u].x, r0.x, inv_no2), spec_apply_real(op, xa[u].y, r0.y, inv_no2)};
+            xa[u] = dc_inverse(y0);
+            // Bin L/2 pairs with itself: untangle leaves it unchanged both ways.
+            xb[u] = spec_apply(op, xb[u], rh, inv_no2);
+        } else {
+            cpx<T> ra, rb2;
+            const cpx<T> w = twiddle_big<1>(tw, p, LOG2N);
+            untangle_pair(lds[EF::lidx(t, p)], lds[EF::lidx(t, L - p)], w, (T)-0.5, ra, rb2);
+            const cpx<T> ya = spec_apply(op, xa[u], ra, inv_no2);
+            const cpx<T> yb = spec_apply(op, xb[u], rb2, inv_no2);
+            // Inverse untangle: conj(w) = exp(-i pi k / L).
+            untangle_pair(ya, yb, conj(w), (T)0.5, xa[u], xb[u]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        const int q = tid + u * NT;
+        if ((TI * NP) % NT != 0 && q >= TI * NP) continue;
+        const int t = q / NP, p = q % NP;
+        if (p == 0) {
+            lds[EF::lidx(t, 0)] = xa[u];
+            lds[EF::lidx(t, NP)] = xb[u];
+        } else {
+            lds[EF::lidx(t, p)] = xa[u];
+            lds[EF::lidx(t, L - p)] = xb[u];
+        }
+    }
+    __syncthreads();
+
+    // 3. Inverse FFT: (n/2) * (result / no2) = result.
+    EI::run_from_lds(tid, lds, tw.coarse, [&](int t, int k, cpx<T> v) {
+        if (TI == 1 || t < nvalid) ob[t * L + k] = v;
+    });
+}
+
+}  // namespace nrfft

@@ -1,0 +1,112 @@
+"""Device-pointer plans (data resident in HBM), for torch tensors or raw
+device addresses.  Execution is enqueued on the caller's HIP stream."""
+from __future__ import annotations
+
+import ctypes
+from typing import Sequence
+
+from . import _lib
+from .api import NrfftError
+
+
+def _dtype_code(dtype) -> int:
+    name = str(dtype)
+    if name in ("float32", "torch.float32", "f32", "<f4"):
+        return _lib.F32
+    if name in ("float64", "torch.float64", "f64", "<f8"):
+        return _lib.F64
+    raise TypeError(f"unsupported dtype {dtype}")
+
+
+def _check(code: int) -> None:
+    if code != _lib.OK:
+        raise NrfftError(code)
+
+
+def _ptr(x) -> int:
+    """Device address of a torch tensor (or an int address)."""
+    if isinstance(x, int):
+        return x
+    if not x.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    if x.device.type != "cuda":
+        raise ValueError("tensor must live in GPU memory")
+    return x.data_ptr()
+
+
+def _stream(stream) -> int:
+    if stream is None:
+        try:
+            import torch
+
+            return torch.cuda.current_stream().cuda_stream
+        except Exception:  # noqa: BLE001 - no torch: default stream
+            return 0
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+class _Plan:
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        self.destroy()
+
+    def destroy(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.lib().nrfft_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    @property
+    def workspace_bytes(self) -> int:
+        return int(_lib.lib().nrfft_plan_workspace_bytes(self._h))
+
+    @property
+    def num_launches(self) -> int:
+        return int(_lib.lib().nrfft_plan_num_launches(self._h))
+
+
+class C2CPlan(_Plan):
+    """Batched C2C of shape nn (C order); ndim 1 = four1, ndim >= 2 = fourn."""
+
+    def __init__(self, dtype, nn: Sequence[int], batch: int = 1):
+        super().__init__()
+        nn = [int(v) for v in nn]
+        arr = (ctypes.c_size_t * len(nn))(*nn)
+        _check(_lib.lib().nrfft_plan_c2c(ctypes.byref(self._h), _dtype_code(dtype), arr, len(nn), batch))
+        self.nn, self.batch = nn, batch
+
+    def execute(self, x_in, x_out, isign: int, stream=None) -> None:
+        _check(_lib.lib().nrfft_execute_c2c(self._h, _ptr(x_in), _ptr(x_out), isign, _stream(stream)))
+
+
+class RealFTPlan(_Plan):
+    def __init__(self, dtype, n: int, batch: int = 1):
+        super().__init__()
+        _check(_lib.lib().nrfft_plan_realft(ctypes.byref(self._h), _dtype_code(dtype), n, batch))
+        self.n, self.batch = n, batch
+
+    def execute(self, x, isign: int, stream=None) -> None:
+        _check(_lib.lib().nrfft_execute_realft(self._h, _ptr(x), isign, _stream(stream)))
+
+
+class ConvlvPlan(_Plan):
+    def __init__(self, dtype, n: int, m: int, batch: int = 1):
+        super().__init__()
+        _check(_lib.lib().nrfft_plan_convlv(ctypes.byref(self._h), _dtype_code(dtype), n, m, batch))
+        self.n, self.m, self.batch = n, m, batch
+
+    def execute(self, data, respns, ans, isign: int = 1, stream=None) -> None:
+        _check(_lib.lib().nrfft_execute_convlv(self._h, _ptr(data), _ptr(respns), _ptr(ans), isign, _stream(stream)))
+
+
+class CorrelPlan(_Plan):
+    def __init__(self, dtype, n: int, batch: int = 1):
+        super().__init__()
+        _check(_lib.lib().nrfft_plan_correl(ctypes.byref(self._h), _dtype_code(dtype), n, batch))
+        self.n, self.batch = n, batch
+
+    def execute(self, d1, d2, ans, stream=None) -> None:
+        _check(_lib.lib().nrfft_execute_correl(self._h, _ptr(d1), _ptr(d2), _ptr(ans), _stream(stream)))

@@ -1,0 +1,122 @@
+"""CPU: the C-ABI library loads, exports every symbol include/nrfft.h
+declares, and validates arguments (error paths return before any HIP call)."""
+import ctypes
+import subprocess
+
+import numpy as np
+import pytest
+
+import numrs_amd
+from numrs_amd import _lib
+
+
+def test_library_exports_header_symbols():
+    L = _lib.lib()
+    declared = _lib.header_symbols()
+    assert len(declared) >= 25
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    missing = [s for s in declared if s not in exported]
+    assert not missing, missing
+    for s in declared:
+        assert hasattr(L, s)
+
+
+def test_status_strings():
+    assert _lib.status_string(0) == "ok"
+    assert _lib.status_string(_lib.E_DIMS) == "Invalid dimensions"
+    assert _lib.status_string(_lib.E_RESP_TOO_LONG) == "Response function longer than data"
+
+
+def _d(n):
+    return np.zeros(max(n, 1)), np.zeros(max(n, 1), np.float32)
+
+
+def test_four1_validation():
+    L = _lib.lib()
+    d, f = _d(8)
+    assert L.nrfft_four1_f64(d.ctypes.data_as(_lib._dp), 3, 1) == _lib.E_NOT_POW2
+    assert L.nrfft_four1_f64(d.ctypes.data_as(_lib._dp), 4, 0) == _lib.E_ISIGN
+    assert L.nrfft_four1_f64(d.ctypes.data_as(_lib._dp), 0, 1) == _lib.E_EMPTY
+    assert L.nrfft_four1_f32(f.ctypes.data_as(_lib._fp), 6, 1) == _lib.E_NOT_POW2
+    assert L.nrfft_four1_f64(None, 4, 1) == _lib.E_INVALID
+
+
+def test_fourn_validation_matches_reference_order():
+    # Fourn.rs:367-378: ndim, then isign, then sizes.
+    L = _lib.lib()
+    d, _ = _d(64)
+
+    def call(nn, ndim, isign):
+        arr = (ctypes.c_size_t * max(len(nn), 1))(*nn)
+        return L.nrfft_fourn_f64(d.ctypes.data_as(_lib._dp), arr, ndim, isign)
+
+    assert call([1], 1, 1) == _lib.E_DIMS          # Fourn.rs:472
+    assert call([8], 1, 0) == _lib.E_ISIGN         # Fourn.rs:475
+    assert call([8, 8], 0, 1) == _lib.E_DIMS
+    assert call([8, 1], 2, 0) == _lib.E_ISIGN
+    assert call([8, 6], 2, 1) == _lib.E_NOT_POW2
+    with pytest.raises(numrs_amd.FournError):
+        numrs_amd.fourn(np.zeros(16), [1], 1, 1)
+    with pytest.raises(numrs_amd.FournError):
+        numrs_amd.fourn(np.zeros(16), [8], 1, 0)
+
+
+def test_convlv_validation():
+    # Convolve.rs:405-424
+    with pytest.raises(numrs_amd.ConvlvError) as e:
+        numrs_amd.convlv([], [1.0], 1)
+    assert e.value.kind == "EmptyInput"
+    with pytest.raises(numrs_amd.ConvlvError) as e:
+        numrs_amd.convlv([1.0, 2.0], [1.0, 2.0, 3.0], 1)
+    assert e.value.kind == "ResponseTooLong"
+    with pytest.raises(numrs_amd.ConvlvError) as e:
+        numrs_amd.convlv([1.0, 2.0], [1.0], 0)
+    assert e.value.kind == "InvalidIsign"
+    with pytest.raises(numrs_amd.ConvlvError) as e:
+        numrs_amd.convlv([1.0, 2.0, 3.0], [1.0], 1)
+    assert e.value.code == _lib.E_NOT_POW2
+
+
+def test_correl_validation():
+    # Correlation.rs:453-462
+    with pytest.raises(numrs_amd.CorrelError) as e:
+        numrs_amd.correl([], [1.0])
+    assert e.value.kind == "EmptyInput"
+    with pytest.raises(numrs_amd.CorrelError) as e:
+        numrs_amd.correl([1.0, 2.0], [1.0])
+    assert e.value.kind == "LengthMismatch"
+    with pytest.raises(numrs_amd.CorrelError) as e:
+        numrs_amd.correl_normalized([1.0, 1.0], [1.0, 2.0])
+    assert e.value.kind == "ZeroStdDev"
+
+
+def test_realft_validation():
+    L = _lib.lib()
+    d, _ = _d(16)
+    assert L.nrfft_realft_f64(d.ctypes.data_as(_lib._dp), 12, 1) == _lib.E_NOT_POW2
+    assert L.nrfft_realft_f64(d.ctypes.data_as(_lib._dp), 16, 2) == _lib.E_ISIGN
+    with pytest.raises(AssertionError):
+        numrs_amd.realft(np.zeros(8), 7, 1)      # Real_FT.rs:5
+    with pytest.raises(AssertionError):
+        numrs_amd.realft(np.zeros(8), 16, 1)     # Real_FT.rs:6
+
+
+def test_plan_validation():
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    nn = (ctypes.c_size_t * 1)(12)
+    assert L.nrfft_plan_c2c(ctypes.byref(h), 0, nn, 1, 1) == _lib.E_NOT_POW2
+    assert L.nrfft_plan_convlv(ctypes.byref(h), 0, 16, 32, 1) == _lib.E_RESP_TOO_LONG
+    assert L.nrfft_execute_c2c(None, None, None, 1, None) == _lib.E_INVALID
+    assert L.nrfft_plan_destroy(None) == 0
+
+
+def test_host_helpers():
+    c = numrs_amd.real_to_complex([1.0, 2.0])
+    assert np.array_equal(c, [1.0, 0.0, 2.0, 0.0])
+    assert np.array_equal(numrs_amd.complex_to_real(c), [1.0, 2.0])
+    z = np.array([3.0, 4.0, 0.0, 1.0])
+    assert np.allclose(numrs_amd.magnitude_spectrum(z), [5.0, 1.0])
+    assert np.allclose(numrs_amd.power_spectrum(z), [25.0, 1.0])

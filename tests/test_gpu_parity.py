@@ -1,0 +1,246 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the oracle.
+
+Tolerances (relative L2, stated per test):
+  FP64 path vs the oracle: 1e-12 (north-star bar for four1).
+  FP32 path vs the FP64 oracle on the FP32-rounded inputs: 2e-6 for the
+  C2C transforms (radix-16 Stockham, ~1e-7 per stage), 1e-5 for realft
+  packed spectra and convolutions of N(0,1) data.
+"""
+import numpy as np
+import pytest
+
+import numrs_amd
+import oracle
+from conftest import rel_l2
+from numrs_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 2e-6
+
+
+def f32round(x):
+    return np.asarray(x, np.float64).astype(np.float32).astype(np.float64)
+
+
+# ---- four1 -------------------------------------------------------------------
+@pytest.mark.parametrize("logn", list(range(0, 17)))
+def test_four1_f64_all_lengths(logn):
+    nn = 1 << logn
+    x = synth.uniform(logn, 2 * nn)
+    for isign in (1, -1):
+        y = x.copy()
+        numrs_amd.four1(y, nn, isign)
+        assert rel_l2(y, oracle.four1_literal(x, isign)) < TOL64, (nn, isign)
+
+
+def test_four1_config1_roundtrip():
+    # N = 2^16 FP64, U[-1,1) seed 0: forward (+1) then inverse (-1) * 1/N.
+    nn = 1 << 16
+    x = synth.uniform(0, 2 * nn)
+    fwd = x.copy()
+    numrs_amd.four1(fwd, nn, 1)
+    ref = oracle.four1_literal(x, 1)
+    assert rel_l2(fwd, ref) < TOL64
+    back = fwd.copy()
+    numrs_amd.four1(back, nn, -1)
+    back /= nn
+    assert rel_l2(back, x) < TOL64
+    ref_back = oracle.four1_literal(ref, -1) / nn
+    assert rel_l2(back, ref_back) < TOL64
+
+
+@pytest.mark.parametrize("logn", [17, 18, 20, 22])
+def test_four1_f64_large(logn):
+    nn = 1 << logn
+    x = synth.uniform(100 + logn, 2 * nn)
+    y = x.copy()
+    numrs_amd.four1(y, nn, -1)
+    assert rel_l2(y, oracle.four1_nr(x, -1)) < TOL64
+
+
+@pytest.mark.parametrize("logn", [1, 4, 7, 10, 13, 14, 16, 20])
+def test_four1_f32(logn):
+    nn = 1 << logn
+    x = f32round(synth.uniform(200 + logn, 2 * nn))
+    for isign in (1, -1):
+        y = x.astype(np.float32)
+        numrs_amd.four1(y, nn, isign)
+        assert rel_l2(y, oracle.four1_nr(x, isign)) < TOL32, (nn, isign)
+
+
+def test_four1_reference_test_signal():
+    # FFT_1.rs:245-267 on the GPU path.
+    t = np.arange(1024) / 1024.0
+    sig = np.sin(2 * np.pi * 5 * t) + 0.5 * np.cos(2 * np.pi * 20 * t)
+    z = numrs_amd.real_to_complex(sig)
+    numrs_amd.four1(z, 1024, 1)
+    mag = numrs_amd.magnitude_spectrum(z)
+    assert np.allclose(mag * mag, numrs_amd.power_spectrum(z), atol=1e-10)  # FFT_1.rs:304-321
+    numrs_amd.four1(z, 1024, -1)
+    z /= 1024.0
+    assert np.max(np.abs(numrs_amd.complex_to_real(z) - sig)) < 1e-10
+
+
+def test_fft_processor_batch_uneven():
+    # FFTProcessor::fft_batch (FFT_1.rs:185-189) with mixed lengths.
+    proc = numrs_amd.FFTProcessor.new()
+    bufs = [synth.uniform(300 + i, 2 * (1 << (4 + (i % 5)))) for i in range(12)]
+    refs = [oracle.four1_literal(b, 1) for b in bufs]
+    proc.fft_batch(bufs, 1)
+    for b, r in zip(bufs, refs):
+        assert rel_l2(b, r) < TOL64
+
+
+def test_four1_batch_f32_2pow20():
+    # Batched C2C at N = 2^20, FP32 (two-pass four-step), 3 transforms.
+    nn = 1 << 20
+    x = f32round(synth.uniform(3, 2 * nn * 3)).astype(np.float32)
+    y = x.copy()
+    proc = numrs_amd.FFTProcessor.new()
+    bufs = [y[i * 2 * nn:(i + 1) * 2 * nn].copy() for i in range(3)]
+    proc.fft_batch(bufs, -1)
+    for i in range(3):
+        ref = oracle.four1_nr(x[i * 2 * nn:(i + 1) * 2 * nn].astype(np.float64), -1)
+        assert rel_l2(bufs[i], ref) < TOL32
+
+
+# ---- realft ------------------------------------------------------------------
+@pytest.mark.parametrize("logn", list(range(1, 16)) + [17, 19])
+def test_realft_f64(logn):
+    n = 1 << logn
+    x = synth.normal(logn, n)
+    y = x.copy()
+    numrs_amd.realft(y, n, 1)
+    assert rel_l2(y, oracle.realft(x, 1)) < TOL64, n
+    numrs_amd.realft(y, n, -1)
+    assert rel_l2(y, (n / 2) * x) < TOL64, n
+
+
+def test_realft_golden(golden):
+    for n in (16, 256, 2048):
+        y = golden[f"realft_{n}_in"].copy()
+        numrs_amd.realft(y, n, 1)
+        assert rel_l2(y, golden[f"realft_{n}_out"]) < TOL64
+
+
+@pytest.mark.parametrize("logn", [3, 10, 14, 16])
+def test_realft_f32(logn):
+    n = 1 << logn
+    x = f32round(synth.normal(logn, n))
+    y = x.astype(np.float32)
+    numrs_amd.realft(y, n, 1)
+    assert rel_l2(y, oracle.realft(x, 1)) < 1e-5
+    numrs_amd.realft(y, n, -1)
+    assert rel_l2(y, (n / 2) * x) < 1e-5
+
+
+# ---- fourn -------------------------------------------------------------------
+@pytest.mark.parametrize("shape", [(2, 2), (4, 8, 16), (16, 2), (2, 64, 4), (32, 32, 32), (8, 4096), (64, 256, 2),
+                                   (2, 2, 2, 2, 2)])
+def test_fourn_f64(shape):
+    x = synth.normal(sum(shape), 2 * int(np.prod(shape)))
+    for isign in (1, -1):
+        y = x.copy()
+        numrs_amd.fourn(y, list(shape), len(shape), isign)
+        assert rel_l2(y, oracle.fourn(x, list(shape), isign)) < TOL64, (shape, isign)
+
+
+def test_fourn_golden(golden):
+    for isign in (1, -1):
+        y = golden[f"fourn_4x8x16_{isign}_in"].copy()
+        numrs_amd.fourn(y, [4, 8, 16], 3, isign)
+        assert rel_l2(y, golden[f"fourn_4x8x16_{isign}_out"]) < TOL64
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 64), (128, 2048)])
+def test_fourn_f32(shape):
+    x = f32round(synth.normal(1, 2 * int(np.prod(shape))))
+    y = x.astype(np.float32)
+    numrs_amd.fourn(y, list(shape), len(shape), 1)
+    assert rel_l2(y, oracle.fourn(x, list(shape), 1)) < TOL32
+
+
+# ---- convlv ------------------------------------------------------------------
+def test_convlv_reference_case():
+    # Convolve.rs:346-360 (circular: index 0 is 1 + 4 = 5).
+    y = numrs_amd.convlv([1.0, 2.0, 3.0, 4.0], [1.0, 1.0], 1)
+    assert np.allclose(y, [5.0, 3.0, 5.0, 7.0], atol=1e-12)
+    proc = numrs_amd.ConvlvProcessor.new()
+    assert abs(proc.process([1.0, 2.0, 3.0, 4.0], [1.0, 1.0], 1)[1] - 3.0) < 1e-10
+
+
+def test_convlv_golden(golden):
+    d, r = golden["convlv_1024_257_data"], golden["convlv_1024_257_respns"]
+    y = numrs_amd.convlv(d, r, 1)
+    assert rel_l2(y, golden["convlv_1024_257_out"]) < TOL64
+    dec = numrs_amd.convlv(golden["convlv_1024_257_out"], r, -1)
+    assert rel_l2(dec, oracle.convlv(golden["convlv_1024_257_out"], r, -1)) < 1e-10
+
+
+@pytest.mark.parametrize("n,m", [(2, 1), (2, 2), (4, 3), (8, 8), (64, 7), (1024, 257), (16384, 257), (32768, 101),
+                                 (65536, 1000)])
+@pytest.mark.parametrize("isign", [1, -1])
+def test_convlv_f64(n, m, isign):
+    d = synth.normal(n + m, n)
+    r = np.exp(-0.5 * ((np.arange(m) - (m - 1) / 2) / max(m / 6, 0.5)) ** 2) * (1 + 0.1 * synth.normal(3, m))
+    y = numrs_amd.convlv(d, r, isign)
+    ref = oracle.convlv(d, r, isign)
+    assert rel_l2(y, ref) < (TOL64 if isign == 1 else 1e-9), (n, m)
+
+
+def test_convlv_batch_per_signal_f32():
+    # Config-3 shape on a subset: n = 2^14, m = 257 Gaussian-windowed responses.
+    n, m, b = 1 << 14, 257, 16
+    d = f32round(synth.normal(2, n * b)).reshape(b, n)
+    r = f32round(synth.gaussian_windowed_responses(2, b, m, 40.0, start=n * b))
+    y = numrs_amd.convlv_batch_responses(d.astype(np.float32), r.astype(np.float32), 1)
+    for i in range(b):
+        ref = oracle.convlv(d[i], r[i], 1)
+        assert rel_l2(y[i], ref) < 1e-5
+
+
+def test_convlv_batch_shared_response():
+    data = [synth.normal(5 + i, 256) for i in range(4)]
+    resp = [0.5, 1.0, 0.5]
+    out = numrs_amd.convlv_batch(data, resp, 1)
+    for d, y in zip(data, out):
+        assert rel_l2(y, oracle.convlv(d, resp, 1)) < TOL64
+
+
+# ---- correl ------------------------------------------------------------------
+def test_correl_reference_cases():
+    y = numrs_amd.correl([1.0, 2.0, 3.0, 4.0], [1.0, 2.0, 3.0, 4.0])
+    assert abs(y[0] - 30.0) < 1e-10 and y[0] > y[1]
+    res = numrs_amd.correl_batch([([1.0, 2.0], [1.0, 2.0]), ([3.0, 4.0], [3.0, 4.0])])
+    assert abs(res[0][0] - 5.0) < 1e-10 and abs(res[1][0] - 25.0) < 1e-10
+    assert np.allclose(numrs_amd.correl([1.0, 2.0], [1.0, 2.0]), [5.0, 2.0])
+    assert abs(numrs_amd.correl_normalized([1.0, 2.0, 3.0, 4.0], [1.0, 2.0, 3.0, 4.0])[0] - 1.0) < 1e-10
+    assert abs(numrs_amd.correl_normalized_fast([1.0, 2.0, 3.0, 4.0], [1.0, 2.0, 3.0, 4.0])[0] - 1.0) < 1e-10
+
+
+@pytest.mark.parametrize("n", [1, 3, 16, 32, 64, 1024, 16384, 32768, 131072])
+def test_correl_f64(n):
+    a, b = synth.normal(n, n), synth.normal(n + 1, n)
+    y = numrs_amd.correl(a, b)
+    assert rel_l2(y, oracle.correl(a, b)) < TOL64, n
+
+
+def test_correl_golden(golden):
+    assert rel_l2(numrs_amd.correl(golden["correl_64_d1"], golden["correl_64_d2"]), golden["correl_64_out"]) < TOL64
+    assert rel_l2(numrs_amd.correl(golden["correl_16_d1"], golden["correl_16_d2"]), golden["correl_16_out"]) < TOL64
+
+
+def test_autocorrel():
+    x = synth.normal(77, 512)
+    assert rel_l2(numrs_amd.autocorrel(x), oracle.correl(x, x)) < TOL64
+    assert rel_l2(numrs_amd.autocorrel_fast(x), oracle.correl(x, x)) < TOL64
+
+
+# ---- errors on the GPU box equal the CPU-side ones -----------------------------
+def test_errors_on_gpu():
+    with pytest.raises(numrs_amd.ConvlvError):
+        numrs_amd.convlv([1.0, 2.0], [1.0, 2.0, 3.0], 1)
+    with pytest.raises(numrs_amd.FournError):
+        numrs_amd.fourn(np.zeros(16), [8], 1, 0)

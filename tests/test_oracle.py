@@ -1,0 +1,240 @@
+"""CPU: pins the oracle (oracle/nr_oracle.c) before it is trusted.
+
+Each oracle function is checked against (a) the numpy/pocketfft golden
+vectors in tests/golden/, (b) brute force or closed forms, and (c) the
+reference's own unit-test known answers, chosen so that a dropped term, a
+wrong sign, a wrong index or a transposed operand fails at least one check.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_l2
+from numrs_amd import synth
+
+
+def cplx(x):
+    return x[0::2] + 1j * x[1::2]
+
+
+# ---- four1 -----------------------------------------------------------------
+@pytest.mark.parametrize("nn", [8, 64, 1024])
+@pytest.mark.parametrize("isign", [1, -1])
+def test_four1_golden(golden, nn, isign):
+    x = golden[f"four1_{nn}_{isign}_in"]
+    exp = golden[f"four1_{nn}_{isign}_out"]
+    assert rel_l2(oracle.four1_literal(x, isign), exp) < 1e-14
+    assert rel_l2(oracle.four1_nr(x, isign), exp) < 1e-14
+
+
+@pytest.mark.parametrize("nn", [1, 2, 4, 16, 128, 2048])
+def test_four1_brute(nn):
+    x = synth.uniform(5, 2 * nn)
+    for isign in (1, -1):
+        exp = oracle.dft_brute(x, isign)
+        assert rel_l2(oracle.four1_literal(x, isign), exp) < 1e-13
+        assert rel_l2(oracle.four1_nr(x, isign), exp) < 1e-13
+
+
+def test_four1_literal_parallel_stages_equal_nr():
+    # nn = 4096 exercises the direct-table stages (mmax >= 1024, FFT_1.rs:74-107).
+    x = synth.uniform(0, 2 * 4096)
+    assert rel_l2(oracle.four1_literal(x, 1), oracle.four1_nr(x, 1)) < 1e-14
+    assert rel_l2(oracle.four1_literal(x, 1), oracle.dft_brute(x, 1)) < 1e-13
+
+
+def test_four1_closed_forms():
+    nn = 256
+    delta = np.zeros(2 * nn)
+    delta[0] = 1.0
+    y = oracle.four1_literal(delta, 1)
+    assert np.allclose(y[0::2], 1.0) and np.allclose(y[1::2], 0.0)
+    const = np.zeros(2 * nn)
+    const[0::2] = 3.0
+    y = oracle.four1_literal(const, -1)
+    assert abs(y[0] - 3.0 * nn) < 1e-9 and np.max(np.abs(y[2:])) < 1e-9
+    # tone exp(-isign 2 pi i k0 j / N) -> N delta_{k0}
+    k0 = 37
+    j = np.arange(nn)
+    for isign in (1, -1):
+        t = np.exp(-isign * 2j * np.pi * k0 * j / nn)
+        x = np.empty(2 * nn)
+        x[0::2], x[1::2] = t.real, t.imag
+        y = cplx(oracle.four1_literal(x, isign))
+        assert abs(y[k0] - nn) < 1e-9
+        y[k0] = 0
+        assert np.max(np.abs(y)) < 1e-9
+
+
+def test_four1_reference_roundtrip_test():
+    # FFT_1.rs:245-267: sin(2 pi 5 t) + 0.5 cos(2 pi 20 t), N = 1024, 1e-10.
+    t = np.arange(1024) / 1024.0
+    sig = np.sin(2 * np.pi * 5 * t) + 0.5 * np.cos(2 * np.pi * 20 * t)
+    x = np.zeros(2048)
+    x[0::2] = sig
+    y = oracle.four1_literal(oracle.four1_literal(x, 1), -1) / 1024.0
+    assert np.max(np.abs(y[0::2] - sig)) < 1e-10
+
+
+def test_four1_parseval_linearity():
+    nn = 4096
+    a, b = synth.uniform(1, 2 * nn), synth.uniform(2, 2 * nn)
+    fa, fb = oracle.four1_literal(a, 1), oracle.four1_literal(b, 1)
+    assert abs(np.sum(fa ** 2) - nn * np.sum(a ** 2)) / (nn * np.sum(a ** 2)) < 1e-13
+    assert rel_l2(oracle.four1_literal(2.0 * a + b, 1), 2.0 * fa + fb) < 1e-14
+
+
+# ---- realft ----------------------------------------------------------------
+@pytest.mark.parametrize("n", [16, 256, 2048])
+def test_realft_golden(golden, n):
+    x = golden[f"realft_{n}_in"]
+    exp = golden[f"realft_{n}_out"]
+    assert rel_l2(oracle.realft(x, 1), exp) < 1e-14
+    back = oracle.realft(exp, -1)
+    assert rel_l2(back, (n / 2) * x) < 1e-14
+
+
+@pytest.mark.parametrize("n", [2, 4, 8, 32, 4096])
+def test_realft_vs_four1(n):
+    x = synth.normal(3, n)
+    z = np.zeros(2 * n)
+    z[0::2] = x
+    F = cplx(oracle.dft_brute(z, 1))
+    y = oracle.realft(x, 1)
+    assert abs(y[0] - F[0].real) < 1e-12 * n
+    assert abs(y[1] - F[n // 2].real) < 1e-12 * n
+    assert rel_l2(y[2::2] + 1j * y[3::2], F[1:n // 2]) < 1e-13
+    assert rel_l2(oracle.realft(y, -1), (n / 2) * x) < 1e-13
+
+
+def test_realft_literal_documents_defect():
+    # The reference realft (index shift, Real_FT.rs:61-64) does not produce the
+    # real DFT, but its forward->inverse pair still round-trips x (n/2).
+    for n in (16, 256, 2048):
+        x = synth.normal(4, n)
+        buf = np.zeros(2 * n)
+        buf[:n] = x
+        lit = oracle.realft_literal(buf, n, 1)
+        assert rel_l2(lit[:n], oracle.realft(x, 1)) > 0.1
+        back = oracle.realft_literal(lit, n, -1)
+        assert rel_l2(back[:n], (n / 2) * x) < 1e-12
+
+
+# ---- fourn -----------------------------------------------------------------
+@pytest.mark.parametrize("isign", [1, -1])
+def test_fourn_golden(golden, isign):
+    x = golden[f"fourn_4x8x16_{isign}_in"]
+    exp = golden[f"fourn_4x8x16_{isign}_out"]
+    assert rel_l2(oracle.fourn(x, [4, 8, 16], isign), exp) < 1e-14
+
+
+@pytest.mark.parametrize("shape", [(2, 2), (8, 4), (4, 2, 8), (16, 32)])
+def test_fourn_separable(shape):
+    # fourn == four1 along every axis (brute force per axis).
+    x = synth.normal(9, 2 * int(np.prod(shape)))
+    z = cplx(x).reshape(shape)
+    for isign in (1, -1):
+        ref = z.copy()
+        for ax in range(len(shape)):
+            ref = np.apply_along_axis(
+                lambda v: cplx(oracle.dft_brute(np.column_stack([v.real, v.imag]).reshape(-1), isign)), ax, ref)
+        y = cplx(oracle.fourn(x, list(shape), isign)).reshape(shape)
+        assert np.max(np.abs(y - ref)) / np.max(np.abs(ref)) < 1e-13
+
+
+def test_fourn_1d_is_four1():
+    x = synth.uniform(10, 2 * 512)
+    assert rel_l2(oracle.fourn(x, [512], 1), oracle.four1_nr(x, 1)) < 1e-15
+
+
+def test_fourn_validation():
+    # Fourn.rs:466-476 and validate_inputs (Fourn.rs:367-378).
+    assert oracle.fourn_status([1], 1, 1) == -2
+    assert oracle.fourn_status([8], 1, 0) == -1
+    assert oracle.fourn_status([8, 8], 0, 1) == -2
+    assert oracle.fourn_status([8, 8], 2, 1) == 0
+    assert oracle.fourn_status([8, 6], 2, 1) == -3
+
+
+# ---- convlv ----------------------------------------------------------------
+def test_convlv_golden(golden):
+    d = golden["convlv_1024_257_data"]
+    r = golden["convlv_1024_257_respns"]
+    assert np.array_equal(oracle.prepare_response(r, 1024), golden["convlv_1024_257_placed"])
+    assert rel_l2(oracle.convlv(d, r, 1), golden["convlv_1024_257_out"]) < 1e-13
+    assert rel_l2(oracle.convlv_direct(d, r), golden["convlv_1024_257_out"]) < 1e-14
+    dec = oracle.convlv(golden["convlv_1024_257_out"], r, -1)
+    assert rel_l2(dec, golden["deconvlv_1024_257_out"]) < 1e-8
+
+
+def test_prepare_response_layout():
+    # Convolve.rs:41-63 with m = 257: 385 non-zeros, respns[129..257) twice.
+    r = np.arange(1, 258, dtype=np.float64)
+    p = oracle.prepare_response(r, 1024)
+    assert np.count_nonzero(p) == 385
+    assert np.array_equal(p[:128], r[129:257])
+    assert np.array_equal(p[128:257], r[128:257])
+    assert np.array_equal(p[1024 - 128:], r[:128])
+    # m = 2: mid = 0, causal [r0, r1, 0, ...]
+    p = oracle.prepare_response(np.array([1.0, 1.0]), 4)
+    assert np.array_equal(p, [1.0, 1.0, 0.0, 0.0])
+
+
+@pytest.mark.parametrize("n,m", [(4, 2), (16, 5), (64, 64), (256, 33), (2048, 257)])
+def test_convlv_vs_direct(n, m):
+    d = synth.normal(31, n)
+    r = synth.normal(32, m)
+    assert rel_l2(oracle.convlv(d, r, 1), oracle.convlv_direct(d, r)) < 1e-13
+
+
+def test_convlv_reference_cases():
+    # Convolve.rs:346-360: [1,2,3,4]*[1,1]; indices 1..3 are 3,5,7 as the
+    # test expects; index 0 is 1+4 = 5 under circular convolution.
+    y = oracle.convlv([1.0, 2.0, 3.0, 4.0], [1.0, 1.0], 1)
+    assert np.allclose(y, [5.0, 3.0, 5.0, 7.0], atol=1e-12)
+    # Error variants (Convolve.rs:405-424) and power-of-two requirement.
+    assert oracle.convlv_status(0, 1, 1) == -5
+    assert oracle.convlv_status(2, 3, 1) == -6
+    assert oracle.convlv_status(2, 1, 0) == -1
+    assert oracle.convlv_status(3, 2, 1) == -3
+
+
+def test_deconvolution_roundtrip():
+    n, m = 512, 9
+    d = synth.normal(41, n)
+    r = np.exp(-0.5 * ((np.arange(m) - 4) / 1.5) ** 2)
+    c = oracle.convlv(d, r, 1)
+    assert rel_l2(oracle.convlv(c, r, -1), d) < 1e-9
+
+
+# ---- correl ----------------------------------------------------------------
+def test_correl_golden(golden):
+    assert rel_l2(oracle.correl(golden["correl_64_d1"], golden["correl_64_d2"]), golden["correl_64_out"]) < 1e-13
+    assert rel_l2(oracle.correl(golden["correl_16_d1"], golden["correl_16_d2"]), golden["correl_16_out"]) < 1e-14
+
+
+def test_correl_reference_cases():
+    # Correlation.rs:406-418, 464-478, 493-502 (direct path, n <= 32).
+    y = oracle.correl([1.0, 2.0, 3.0, 4.0], [1.0, 2.0, 3.0, 4.0])
+    assert abs(y[0] - 30.0) < 1e-10 and y[0] > y[1]
+    assert abs(oracle.correl([1.0, 2.0], [1.0, 2.0])[0] - 5.0) < 1e-10
+    assert abs(oracle.correl([3.0, 4.0], [3.0, 4.0])[0] - 25.0) < 1e-10
+    assert np.allclose(oracle.correl([1.0, 2.0], [1.0, 2.0]), [5.0, 2.0])
+
+
+@pytest.mark.parametrize("n", [64, 256, 4096])
+def test_correl_vs_direct(n):
+    a, b = synth.normal(51, n), synth.normal(52, n)
+    assert rel_l2(oracle.correl(a, b), oracle.correl_direct_circular(a, b)) < 1e-13
+
+
+# ---- generator ---------------------------------------------------------------
+def test_generator_matches_oracle():
+    assert np.array_equal(synth.uniform(7, 999), oracle.gen_uniform(7, 999))
+    assert np.max(np.abs(synth.normal(7, 999) - oracle.gen_normal(7, 999))) < 1e-15
+    s = oracle.gen_skip(7, 10)
+    assert np.array_equal(oracle.gen_uniform(s, 5), synth.uniform(7, 5, start=10))
+    u = synth.uniform(0, 100000)
+    assert u.min() >= -1.0 and u.max() < 1.0 and abs(u.mean()) < 0.01
+    g = synth.normal(1, 100000)
+    assert abs(g.mean()) < 0.02 and abs(g.std() - 1.0) < 0.02
