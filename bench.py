@@ -1,0 +1,318 @@
+"""Benchmark of the numrs FFT path on one MI355X.
+
+Headline workload (one JSON line on stdout): batched FP32 C2C four1,
+64 transforms x N = 2^20 complex, re/im i.i.d. uniform[-1,1) from seed 3,
+isign = +1, inputs resident in HBM, out of place.  One step = one plan
+execution over the whole batch.  GFLOP/s uses 5 N log2 N per transform.
+
+The same line carries
+  roofline:      the dominant kernel (slower four-step pass), algorithmic bytes
+                 per launch (one read + one write of the batch, 16 B per
+                 point) / its average duration from HIP events on its stream;
+  cpu_baseline:  the oracle's restatement of FFT_1::four1 (FP64, OpenMP over
+                 transforms like FFTProcessor::fft_batch) timed on the host
+                 cores on a bounded sample of the same workload;
+  extra:         the other configurations (four1 2^16 FP64, fourn 256^3,
+                 batched convlv 4096 x 2^14), each with its own GB/s figure.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "batched FFT GFLOP/s + %HBM peak; split-MFMA vs plain FP32 at rel-L2≤1e-6"
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def fill_uniform(t, seed: int, start: int, count: int, chunk: int = 1 << 22) -> None:
+    """Write count uniform[-1,1) draws (stream position start..) into tensor t."""
+    import torch
+
+    from numrs_amd import synth
+
+    for off in range(0, count, chunk):
+        c = min(chunk, count - off)
+        v = synth.uniform(seed, c, start=start + off).astype(np.float32)
+        t[off:off + c].copy_(torch.from_numpy(v))
+
+
+def fill_normal(t, seed: int, count: int, dtype=np.float32, chunk: int = 1 << 22) -> None:
+    import torch
+
+    from numrs_amd import synth
+
+    for off in range(0, count, chunk):
+        c = min(chunk, count - off)
+        v = synth.normal(seed, c, start=off).astype(dtype)
+        t[off:off + c].copy_(torch.from_numpy(v))
+
+
+def time_events(fn, reps: int) -> float:
+    """Average ms of fn() over reps, HIP events on torch's current stream (the
+    stream every launch here is enqueued on)."""
+    import torch
+
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        fn()
+    e1.record(s)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def headline(args) -> dict:
+    import torch
+
+    import numrs_amd
+
+    batch, logn = 64, 20
+    n = 1 << logn
+    pts = batch * n
+    log(f"[bench] c2c fp32 batch {batch} x 2^{logn}: generating inputs")
+    x = torch.empty(2 * pts, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    fill_uniform(x, 3, 0, 2 * pts)
+    plan = numrs_amd.C2CPlan("float32", [n], batch)
+    nl = plan.num_launches
+    for _ in range(args.warmup):
+        plan.execute(x, y, 1)
+    torch.cuda.synchronize()
+    log(f"[bench] timing {args.steps} steps ({nl} launches per step)")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.execute(x, y, 1)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = wall * 1e3 / args.steps
+    flops = 5.0 * n * logn * batch
+    gflops = flops / (ms * 1e-3) / 1e9
+    compulsory = 2.0 * pts * 8  # one read + one write, 8 B per complex
+    # Per-launch durations (HIP events, same stream).
+    reps = max(5, args.steps)
+    per = [time_events(lambda i=i: plan.execute_launches(x, y, 1, i, 1), reps) for i in range(nl)]
+    dom = int(np.argmax(per))
+    pass_bytes = compulsory  # each pass reads and writes every point once
+    achieved = pass_bytes / (per[dom] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(gflops, 1),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: re/im i.i.d. uniform[-1,1) from SplitMix64 seed 3",
+        "config": {"workload": "four1 C2C FP32, batch 64 x N=2^20, isign=+1, out of place, HBM-resident",
+                   "batch": batch, "n": n, "passes": nl, "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": f"launch {dom} of {nl} (four-step pass {'AB'[dom] if nl == 2 else dom})",
+                     "kernel_ms": round(per[dom], 4), "algorithmic_bytes_per_launch": int(pass_bytes)},
+        "hbm_effective": {"GBps": round(compulsory / (ms * 1e-3) / 1e9, 1),
+                          "frac": round(compulsory / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                          "note": "compulsory bytes (one read + one write) / whole step"},
+        "pass_ms": [round(p, 4) for p in per],
+    }
+    del x, y
+    torch.cuda.empty_cache()
+    return out
+
+
+def cpu_baseline(args) -> dict:
+    """The oracle's literal four1 (FFT_1.rs:5-107, FP64) over a bounded sample
+    of the headline workload: `sample` transforms of 2^20 on all host cores."""
+    import oracle
+    from numrs_amd import synth
+
+    n = 1 << 20
+    threads = min(os.cpu_count() or 1, 16)
+    sample = args.cpu_sample
+    data = np.empty(2 * n * sample, np.float64)
+    for i in range(sample):
+        data[2 * n * i:2 * n * (i + 1)] = synth.uniform(3, 2 * n, start=2 * n * i).astype(np.float32)
+    t0 = time.perf_counter()
+    used = oracle.four1_batch_inplace(data, n, sample, 1, threads)
+    dt = time.perf_counter() - t0
+    gflops = 5.0 * n * 20 * sample / dt / 1e9
+    return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": used, "kind": "port",
+            "sample": f"{sample} of the 64 transforms (N=2^20, FP64 literal four1), {dt:.2f} s wall"}
+
+
+def copy_calibration() -> dict:
+    """A plain 512 MiB device-to-device copy: the reference kernel whose
+    FETCH_SIZE / WRITE_SIZE counters scale the FFT kernels' counters to bytes."""
+    import torch
+
+    nbytes = 512 << 20
+    a = torch.zeros(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    ms = time_events(lambda: b.copy_(a), 10)
+    del a, b
+    torch.cuda.empty_cache()
+    return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": round(ms, 4),
+            "GBps_rw": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
+
+
+def extra_configs(args, only: str = "all") -> dict:
+    import torch
+
+    import numrs_amd
+
+    res = {}
+    if only in ("all", "four1"):
+        res.update(_extra_four1())
+    if only in ("all", "fourn"):
+        res.update(_extra_fourn())
+    if only in ("all", "convlv"):
+        res.update(_extra_convlv())
+    return res
+
+
+def _extra_four1() -> dict:
+    import torch
+
+    import numrs_amd
+
+    res = {}
+    # four1 N=2^16 FP64, single transform, device resident (config 1 shape).
+    n = 1 << 16
+    x = torch.empty(2 * n, dtype=torch.float64, device="cuda")
+    fill_uniform(x, 0, 0, 2 * n)
+    x = x.double()
+    y = torch.empty_like(x)
+    plan = numrs_amd.C2CPlan("float64", [n], 1)
+    for _ in range(5):
+        plan.execute(x, y, 1)
+    ms = time_events(lambda: plan.execute(x, y, 1), 50)
+    res["four1_f64_2p16"] = {"us": round(ms * 1e3, 2), "GFLOPs": round(5 * n * 16 / (ms * 1e-3) / 1e9, 2),
+                             "GBps": round(2 * n * 16 / (ms * 1e-3) / 1e9, 1)}
+    del plan, x, y
+    return res
+
+
+def _extra_fourn() -> dict:
+    import torch
+
+    import numrs_amd
+
+    res = {}
+    # fourn 256^3 FP32 and FP64 (config 2 shape): per pass and total.
+    for dt, tdt, es in (("float32", torch.float32, 8), ("float64", torch.float64, 16)):
+        shape = [256, 256, 256]
+        pts = 256 ** 3
+        x = torch.empty(2 * pts, dtype=tdt, device="cuda")
+        fill_normal(x, 1, 2 * pts, np.float32 if es == 8 else np.float64)
+        plan = numrs_amd.C2CPlan(dt, shape, 1)
+        for _ in range(3):
+            plan.execute(x, x, 1)
+        total = time_events(lambda: plan.execute(x, x, 1), 10)
+        per = [time_events(lambda i=i: plan.execute_launches(x, x, 1, i, 1), 10) for i in range(plan.num_launches)]
+        byt = 2.0 * pts * es
+        res[f"fourn_256cubed_{dt}"] = {
+            "ms": round(total, 4), "GFLOPs": round(5 * pts * 24 / (total * 1e-3) / 1e9, 1),
+            "GBps_total_compulsory": round(byt / (total * 1e-3) / 1e9, 1),
+            "pass_GBps": [round(byt / (p * 1e-3) / 1e9, 1) for p in per]}
+        del plan, x
+    torch.cuda.empty_cache()
+    return res
+
+
+def _extra_convlv() -> dict:
+    import torch
+
+    import numrs_amd
+    from numrs_amd import synth
+
+    res = {}
+    # batched convlv 4096 x 2^14 FP32, m = 257 per-signal responses (config 3 shape).
+
+    b, n, m = 4096, 1 << 14, 257
+    d = torch.empty(b * n, dtype=torch.float32, device="cuda")
+    fill_normal(d, 2, b * n)
+    r = torch.from_numpy(synth.gaussian_windowed_responses(2, b, m, 40.0, start=b * n).astype(np.float32)).cuda()
+    a = torch.empty_like(d)
+    plan = numrs_amd.ConvlvPlan("float32", n, m, b)
+    for _ in range(3):
+        plan.execute(d, r, a, 1)
+    ms = time_events(lambda: plan.execute(d, r, a, 1), 20)
+    byt = (2.0 * n + m) * 4 * b
+    res["convlv_f32_4096x2p14_m257"] = {
+        "ms": round(ms, 4), "GBps": round(byt / (ms * 1e-3) / 1e9, 1),
+        "frac_hbm": round(byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "GFLOPs": round(3 * 2.5 * n * 14 * b / (ms * 1e-3) / 1e9, 1), "launches": plan.num_launches}
+    del plan, d, r, a
+    torch.cuda.empty_cache()
+    return res
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=16, help="transforms in the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--only", choices=["headline", "four1", "fourn", "convlv"], default=None,
+                    help="run one workload only (profiling runs); the JSON line then holds just that workload")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a 512 MiB copy kernel (counter calibration for the traffic figure)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the FFT path shards by transform with no exchange, and this build "
+              "runs and reports the single-GPU figure only (use --gpus 1).", file=sys.stderr)
+        return 2
+    import torch
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        return 1
+    from numrs_amd import _lib
+
+    rc = _lib.lib().nrfft_init(0)
+    if rc:
+        raise RuntimeError(_lib.last_error())
+    calib = copy_calibration() if args.calib_copy else None
+    if args.only and args.only != "headline":
+        out = {"metric": METRIC, "only": args.only, "extra": extra_configs(args, args.only)}
+        if calib:
+            out["copy_calibration"] = calib
+        print(json.dumps(out), flush=True)
+        return 0
+    out = headline(args)
+    if calib:
+        out["copy_calibration"] = calib
+    if not args.no_extra and args.only is None:
+        log("[bench] secondary configurations")
+        out["extra"] = extra_configs(args)
+    if not args.no_cpu and args.only is None:
+        log("[bench] cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(args)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

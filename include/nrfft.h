@@ -125,6 +125,29 @@ int nrfft_execute_convlv(nrfft_plan plan, const void* d_data, const void* d_resp
 int nrfft_plan_correl(nrfft_plan* plan, int dtype, size_t n, size_t batch);
 int nrfft_execute_correl(nrfft_plan plan, const void* d_data1, const void* d_data2, void* d_ans, void* stream);
 
+/* Profiling aid (no counterpart in the reference): enqueue only launches
+ * [first, first + count) of a C2C plan, e.g. one pass of a four-step plan. */
+int nrfft_execute_c2c_launches(nrfft_plan plan, const void* d_in, void* d_out, int isign, void* stream, int first,
+                               int count);
+
+/* ---- autotuner (no counterpart in the reference) -------------------------
+ * A C2C plan's tunables -- tune[0] log2 of the four-step column length,
+ * tune[1] / tune[2] log2 tile widths of the column / transpose passes (which
+ * set threads, LDS and so work-groups per CU), tune[3] transforms per batch
+ * chunk (0 = whole batch), tune[4] resident work-groups per CU of the
+ * persistent passes, tune[5] XCD-contiguous tile order (0/1) -- are cached per
+ * (shape, batch, dtype).  best_tune receives 6 values.
+ * nrfft_autotune_c2c times every candidate on the current device with HIP
+ * events (iters executions each) and caches the fastest; plans created
+ * afterwards for that key use it.  -1 leaves an entry at the planner default. */
+int nrfft_plan_c2c_tuned(nrfft_plan* plan, int dtype, const size_t* nn, size_t ndim, size_t batch, const int* tune,
+                         size_t ntune);
+int nrfft_autotune_c2c(int dtype, const size_t* nn, size_t ndim, size_t batch, int iters, int* best_tune,
+                       double* best_ms);
+/* Text cache file; also read at the first plan creation from $NRFFT_TUNE_CACHE. */
+int nrfft_tune_cache_save(const char* path);
+int nrfft_tune_cache_load(const char* path);
+
 int nrfft_plan_destroy(nrfft_plan plan);
 /* Bytes of HBM workspace the plan owns. */
 size_t nrfft_plan_workspace_bytes(nrfft_plan plan);

@@ -33,6 +33,14 @@ from .api import (  # noqa: F401
     realft,
     realft_optimized,
 )
-from .device import C2CPlan, ConvlvPlan, CorrelPlan, RealFTPlan  # noqa: F401
+from .device import (  # noqa: F401
+    C2CPlan,
+    ConvlvPlan,
+    CorrelPlan,
+    RealFTPlan,
+    autotune_c2c,
+    tune_cache_load,
+    tune_cache_save,
+)
 
 __version__ = "0.1.0"

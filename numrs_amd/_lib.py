@@ -64,6 +64,11 @@ _SIGS = {
     "nrfft_execute_convlv": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
     "nrfft_plan_correl": (_i, [ctypes.POINTER(_vp), _i, _sz, _sz]),
     "nrfft_execute_correl": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "nrfft_execute_c2c_launches": (_i, [_vp, _vp, _vp, _i, _vp, _i, _i]),
+    "nrfft_plan_c2c_tuned": (_i, [ctypes.POINTER(_vp), _i, _szp, _sz, _sz, ctypes.POINTER(_i), _sz]),
+    "nrfft_autotune_c2c": (_i, [_i, _szp, _sz, _sz, _i, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_double)]),
+    "nrfft_tune_cache_save": (_i, [ctypes.c_char_p]),
+    "nrfft_tune_cache_load": (_i, [ctypes.c_char_p]),
     "nrfft_plan_destroy": (_i, [_vp]),
     "nrfft_plan_workspace_bytes": (_sz, [_vp]),
     "nrfft_plan_num_launches": (_i, [_vp]),

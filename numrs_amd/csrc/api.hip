@@ -49,6 +49,34 @@ __global__ void k_twiddles(c64* c64c, c64* c64f, c32* c32c, c32* c32f) {
     c32f[i] = c32{(float)c, (float)s};
 }
 
+// Per-stage tables: entry e in [2L, 4L) belongs to length L = 2^(floor(log2 e) - 1);
+// within L's block, stage s holds NS_s rows of R_s values exp(2 pi i r k / (NS_s R_s)).
+__global__ void k_stage_twiddles(c64* t64, c32* t32) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= TW_STAGE_ELEMS) return;
+    double s = 0.0, c = 1.0;
+    if (e >= 4) {
+        int lg = 0;
+        while ((2 << lg) <= e) ++lg;  // 2^lg <= e < 2^(lg+1)
+        const int logl = lg - 1, L = 1 << logl;
+        int o = e - 2 * L;
+        int ns = 1;
+        const int nst = L < 16 ? 1 : (logl + 3) / 4;
+        for (int st = 0; st < nst; ++st) {
+            const int R = L < 16 ? L : (st == 0 ? ((logl % 4) == 0 ? 16 : (1 << (logl % 4))) : 16);
+            if (o < ns * R) {
+                const int k = o / R, r = o % R;
+                sincospi(2.0 * (double)(r * k) / (double)(ns * R), &s, &c);
+                break;
+            }
+            o -= ns * R;
+            ns *= R;
+        }
+    }
+    t64[e] = c64{c, s};
+    t32[e] = c32{(float)c, (float)s};
+}
+
 static std::mutex g_ctx_mu;
 static std::map<int, DeviceCtx*> g_ctx;
 
@@ -67,7 +95,12 @@ int get_ctx(DeviceCtx** out) {
     HIPCHK(hipMalloc(&c->c64f, TW_COARSE * sizeof(c64)));
     HIPCHK(hipMalloc(&c->c32c, TW_COARSE * sizeof(c32)));
     HIPCHK(hipMalloc(&c->c32f, TW_COARSE * sizeof(c32)));
+    HIPCHK(hipMalloc(&c->c64s, TW_STAGE_ELEMS * sizeof(c64)));
+    HIPCHK(hipMalloc(&c->c32s, TW_STAGE_ELEMS * sizeof(c32)));
+    HIPCHK(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev));
     hipLaunchKernelGGL(k_twiddles, dim3(TW_COARSE / 256), dim3(256), 0, 0, c->c64c, c->c64f, c->c32c, c->c32f);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_stage_twiddles, dim3(TW_STAGE_ELEMS / 256), dim3(256), 0, 0, c->c64s, c->c32s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     g_ctx[dev] = c;
@@ -108,14 +141,25 @@ int plan_add_c2c_pass(Plan& p, int mode, int logl, int tilog, const PassDesc& d,
     l.fn[1] = km.fn;
     l.nt = kp.nt;
     const int64_t tiles0 = (d.nb0 + kp.ti - 1) / kp.ti;
-    const int64_t grid = tiles0 * d.nb1 * d.nb2;
-    if (grid <= 0 || grid > 0x7fffffffLL) return fail(NRFFT_E_UNSUPPORTED, "grid too large");
+    const int64_t ntiles = tiles0 * d.nb1 * d.nb2;
+    if (ntiles <= 0) return fail(NRFFT_E_UNSUPPORTED, "empty pass");
+    // Persistent grid: as many groups as are resident at once (or the tuned
+    // groups per CU), never more than there are tiles.
+    DeviceCtx* ctx = nullptr;
+    int rc = get_ctx(&ctx);
+    if (rc) return rc;
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp.fn, kp.nt, 0));
+    if (per_cu < 1) per_cu = 1;
+    if (p.tune.wg_per_cu > 0 && p.tune.wg_per_cu < per_cu) per_cu = p.tune.wg_per_cu;
+    const int64_t grid = std::min<int64_t>(ntiles, int64_t(per_cu) * ctx->ncu);
     // In-tile offsets must fit 32 bits.
     const int64_t span_in = int64_t(kp.ti - 1) * d.si0 + int64_t((1 << logl) - 1) * d.esi;
     const int64_t span_out = int64_t(kp.ti - 1) * d.so0 + int64_t((1 << logl) - 1) * d.eso;
     if (span_in > 0x7fffffffLL || span_out > 0x7fffffffLL) return fail(NRFFT_E_UNSUPPORTED, "tile span too large");
     l.grid = (unsigned)grid;
     l.desc = d;
+    l.desc.xcd_remap = p.tune.xcd_remap > 0 ? 1 : 0;
     l.src = src;
     l.dst = dst;
     l.lds = kp.lds_bytes;
@@ -168,44 +212,47 @@ int plan_c2c_1d(Plan& p, int logn, int64_t batch, int src, int dst, size_t* tmp_
         l2 = logn - l1;
     }
     const int64_t N1 = int64_t(1) << l1, N2 = int64_t(1) << l2;
-    *tmp_elems = std::max<size_t>(*tmp_elems, size_t(N) * size_t(batch));
-    // Pass A: FFT over n1 (stride N2), tile over n2, twiddle W_N^{k1 n2}; -> tmp[k1][n2].
-    {
-        PassDesc d{};
-        d.nb0 = N2;
-        d.si0 = 1;
-        d.so0 = 1;
-        d.esi = (int32_t)N2;
-        d.eso = (int32_t)N2;
-        d.nb1 = batch;
-        d.si1 = N;
-        d.so1 = N;
-        d.nb2 = 1;
-        d.tw_logn = logn;
-        d.tw_b0_shift = 0;
-        int want = p.tune.tilog_a >= 0 ? p.tune.tilog_a : col_tilog(p.dtype);
-        int t = pick_tilog(p.dtype, MODE_TT, l1, N2, want);
-        if (t < 0) return fail(NRFFT_E_UNSUPPORTED, "no column kernel");
-        int rc = plan_add_c2c_pass(p, MODE_TT, l1, t, d, src, B_TMP);
+    // Batch chunking: pass A and pass B of one chunk run back to back so the
+    // chunk's intermediate can stay in the last-level cache (256 MB MALL).
+    const int64_t chunk = (p.tune.chunk > 0 && p.tune.chunk < batch) ? p.tune.chunk : batch;
+    *tmp_elems = std::max<size_t>(*tmp_elems, size_t(N) * size_t(chunk));
+    const int want_a = p.tune.tilog_a >= 0 ? p.tune.tilog_a : col_tilog(p.dtype);
+    const int want_b = p.tune.tilog_b >= 0 ? p.tune.tilog_b : col_tilog(p.dtype);
+    const int ta = pick_tilog(p.dtype, MODE_TT, l1, N2, want_a);
+    const int tb = pick_tilog(p.dtype, MODE_JT, l2, N1, want_b);
+    if (ta < 0 || tb < 0) return fail(NRFFT_E_UNSUPPORTED, "no four-step kernel");
+    for (int64_t c0 = 0; c0 < batch; c0 += chunk) {
+        const int64_t cb = std::min<int64_t>(chunk, batch - c0);
+        // Pass A: FFT over n1 (stride N2), tile over n2, twiddle W_N^{k1 n2}; -> tmp[k1][n2].
+        PassDesc a{};
+        a.nb0 = N2;
+        a.si0 = 1;
+        a.so0 = 1;
+        a.esi = (int32_t)N2;
+        a.eso = (int32_t)N2;
+        a.nb1 = cb;
+        a.si1 = N;
+        a.so1 = N;
+        a.nb2 = 1;
+        a.tw_logn = logn;
+        a.tw_b0_shift = 0;
+        int rc = plan_add_c2c_pass(p, MODE_TT, l1, ta, a, src, B_TMP);
         if (rc) return rc;
-    }
-    // Pass B: FFT over n2 (contiguous rows of tmp), rows k1 tiled; X[k2 N1 + k1].
-    {
-        PassDesc d{};
-        d.nb0 = N1;
-        d.si0 = (int32_t)N2;
-        d.esi = 1;
-        d.so0 = 1;
-        d.eso = (int32_t)N1;
-        d.nb1 = batch;
-        d.si1 = N;
-        d.so1 = N;
-        d.nb2 = 1;
-        int want = p.tune.tilog_b >= 0 ? p.tune.tilog_b : col_tilog(p.dtype);
-        int t = pick_tilog(p.dtype, MODE_JT, l2, N1, want);
-        if (t < 0) return fail(NRFFT_E_UNSUPPORTED, "no transpose kernel");
-        int rc = plan_add_c2c_pass(p, MODE_JT, l2, t, d, B_TMP, dst);
+        p.launches.back().off_src = c0 * N;
+        // Pass B: FFT over n2 (contiguous rows of tmp), rows k1 tiled; X[k2 N1 + k1].
+        PassDesc b{};
+        b.nb0 = N1;
+        b.si0 = (int32_t)N2;
+        b.esi = 1;
+        b.so0 = 1;
+        b.eso = (int32_t)N1;
+        b.nb1 = cb;
+        b.si1 = N;
+        b.so1 = N;
+        b.nb2 = 1;
+        rc = plan_add_c2c_pass(p, MODE_JT, l2, tb, b, B_TMP, dst);
         if (rc) return rc;
+        p.launches.back().off_dst = c0 * N;
     }
     return NRFFT_OK;
 }
@@ -271,18 +318,22 @@ int plan_c2c_nd(Plan& p, const std::vector<size_t>& nn, int64_t batch) {
 // ---------------------------------------------------------------------------
 // Execution.
 // ---------------------------------------------------------------------------
-int exec_plan(const Plan& p, const void* in, void* out, void* tmp, const void* aux, int isign, hipStream_t stream) {
+int exec_plan(const Plan& p, const void* in, void* out, void* tmp, const void* aux, int isign, hipStream_t stream,
+              int first, int count) {
     DeviceCtx* ctx = nullptr;
     int rc = get_ctx(&ctx);
     if (rc) return rc;
+    int idx = -1;
     void* tmp2 = tmp ? static_cast<char*>(tmp) + p.tmp2_offset * csize(p.dtype) : nullptr;
     void* bufs[B_COUNT] = {const_cast<void*>(in), out, tmp, const_cast<void*>(aux), tmp2};
     for (const Launch& l : p.launches) {
+        ++idx;
+        if (idx < first || (count >= 0 && idx >= first + count)) continue;
         if (l.args[7] != 0 && ((l.args[7] > 0) != (isign > 0))) continue;
         const int eff = l.args[6] != 0 ? (int)l.args[6] : isign;
         const int si = eff > 0 ? 0 : 1;
-        void* a = bufs[l.src];
-        void* b = bufs[l.dst];
+        void* a = static_cast<char*>(bufs[l.src]) + l.off_src * (int64_t)csize(p.dtype);
+        void* b = static_cast<char*>(bufs[l.dst]) + l.off_dst * (int64_t)csize(p.dtype);
         if (l.kind == LK_COPY) {
             if (a != b) HIPCHK(hipMemcpyAsync(b, a, l.bytes, hipMemcpyDeviceToDevice, stream));
             continue;
@@ -290,11 +341,11 @@ int exec_plan(const Plan& p, const void* in, void* out, void* tmp, const void* a
         if (l.kind == LK_C2C) {
             PassDesc d = l.desc;
             if (p.dtype == NRFFT_F32) {
-                Twiddles<float> tw{ctx->c32c, ctx->c32f};
+                Twiddles<float> tw = ctx_twiddles<float>(ctx);
                 void* args[] = {&a, &b, &d, &tw};
                 HIPCHK(hipLaunchKernel(l.fn[si], dim3(l.grid), dim3(l.nt), args, 0, stream));
             } else {
-                Twiddles<double> tw{ctx->c64c, ctx->c64f};
+                Twiddles<double> tw = ctx_twiddles<double>(ctx);
                 void* args[] = {&a, &b, &d, &tw};
                 HIPCHK(hipLaunchKernel(l.fn[si], dim3(l.grid), dim3(l.nt), args, 0, stream));
             }
@@ -338,9 +389,6 @@ int cached_plan(int kind, int dtype, const std::vector<size_t>& nn, size_t batch
     *out = p;
     return NRFFT_OK;
 }
-
-// Tunables: defaults here; the autotuner (tune.hip) overrides them.
-__attribute__((weak)) void tune_lookup(Plan& p) { p.tune = Tune{}; }
 
 int build_plan(Plan& p) {
     switch (p.kind) {
@@ -527,6 +575,15 @@ int nrfft_execute_c2c(nrfft_plan plan, const void* d_in, void* d_out, int isign,
     if (isign != 1 && isign != -1) return fail(NRFFT_E_ISIGN, "isign must be 1 or -1");
     if (!d_in || !d_out) return fail(NRFFT_E_INVALID, "null buffer");
     return exec_plan(plan->plan, d_in, d_out, plan->tmp, nullptr, isign, (hipStream_t)stream);
+}
+
+int nrfft_execute_c2c_launches(nrfft_plan plan, const void* d_in, void* d_out, int isign, void* stream, int first,
+                               int count) {
+    g_err.clear();
+    if (!plan || plan->plan.kind != PK_C2C) return fail(NRFFT_E_INVALID, "not a c2c plan");
+    if (isign != 1 && isign != -1) return fail(NRFFT_E_ISIGN, "isign must be 1 or -1");
+    if (!d_in || !d_out || first < 0 || count < 0) return fail(NRFFT_E_INVALID, "bad argument");
+    return exec_plan(plan->plan, d_in, d_out, plan->tmp, nullptr, isign, (hipStream_t)stream, first, count);
 }
 
 int nrfft_plan_destroy(nrfft_plan plan) {

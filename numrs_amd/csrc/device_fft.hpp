@@ -194,14 +194,21 @@ enum { MAP_J = 0, MAP_T = 1 };
 //   coarse[m] = exp(2 pi i m / 2^13),       m < 2^13
 //   fine[m]   = exp(2 pi i m / 2^26),       m < 2^13
 // exp(2 pi i M / 2^26) = coarse[M >> 13] * fine[M & 8191] for M < 2^26.
+//   stage: per-stage rows for every single-pass length L = 2^1..2^13; the
+//   stage-s table of L starts at 2L + off_s and holds NS_s rows of R_s entries,
+//   row k = (exp(2 pi i r k / (NS_s R_s)), r = 0..R_s-1), so one thread's
+//   twiddles are contiguous (wide loads) and consecutive k are adjacent.
 constexpr int TW_LOG_COARSE = 13;
 constexpr int TW_COARSE = 1 << TW_LOG_COARSE;
 constexpr int TW_LOG_VIRT = 26;
+constexpr int TW_STAGE_MAX_LOGL = 13;
+constexpr int TW_STAGE_ELEMS = 4 << TW_STAGE_MAX_LOGL;
 
 template <typename T>
 struct Twiddles {
     const cpx<T>* coarse;
     const cpx<T>* fine;
+    const cpx<T>* stage;
 };
 
 // exp(S 2 pi i m / 2^logn), 0 <= m < 2^logn <= 2^26.
@@ -212,9 +219,36 @@ __device__ __forceinline__ cpx<T> twiddle_big(const Twiddles<T>& tw, uint32_t m,
     return dirw<S>(w);
 }
 
+// out[r] = W^{S (m0 + r m1)} for r < R (W = exp(2 pi i / 2^logn)), from table
+// values of W^{m0} and W^{2^q m1}; out[r] = out[r - lowbit(r)] * W^{lowbit(r) m1},
+// so every value is at most popcount(r) + 1 products away from the tables.
+template <int R, int S, typename T>
+__device__ __forceinline__ void twiddle_run(const Twiddles<T>& tw, uint32_t m0, uint32_t m1, int logn, cpx<T>* out) {
+    constexpr int LR = ilog2c(R);
+    cpx<T> p[LR > 0 ? LR : 1];
+#pragma unroll
+    for (int q = 0; q < LR; ++q) p[q] = twiddle_big<S>(tw, (m1 << q) & ((1u << logn) - 1u), logn);
+    out[0] = twiddle_big<S>(tw, m0, logn);
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+        const int lb = r & (-r);
+        out[r] = cmul(out[r - lb], p[ilog2c(lb)]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // The stage engine.
 // ---------------------------------------------------------------------------
+template <int L>
+struct StageTable {
+    // offset of stage s inside L's block (which starts at 2L)
+    static constexpr int off(int s) {
+        int o = 0;
+        for (int q = 0; q < s; ++q) o += RadixPlan<L>::ns(q + 1);
+        return o;
+    }
+};
+
 template <typename T, int S, int L, int TI, int NT, int MAPL, int MAPS>
 struct Engine {
     using P = RadixPlan<L>;
@@ -225,25 +259,36 @@ struct Engine {
     // LDS layout follows the mapping used after the first stage.
     static constexpr int LAYOUT = (NST == 1) ? MAPL : MAPS;
     static constexpr int map_of(int s) { return s == 0 ? MAPL : MAPS; }
+    static constexpr int LDS_ELEMS = TI * L;
 
-    // Padded LDS index: one pad slot per 16 complex values breaks the
-    // power-of-two strides of the early Stockham stages across banks.
-    static __device__ __forceinline__ int pad(int a) { return a + (a >> 4); }
-    static constexpr int LDS_ELEMS = TI * L + (TI * L) / 16 + 1;
-
+    // Bank-conflict-avoiding LDS index of element i of tile transform t.
+    //   MAP_T layout ([i][TI]): the column index is XOR-swizzled with a hash of
+    //   the row, so 16 lanes that touch one row, or rows i, i+1.., i+4.., i+16..
+    //   of one column, fall in distinct slots.
+    //   MAP_J layout ([t][L]): the low 4 bits of i are XOR-ed with bits 4..7 of
+    //   i plus t, with the same effect along a row and across rows.
     static __device__ __forceinline__ int lidx(int t, int i) {
-        if constexpr (LAYOUT == MAP_T) return pad(i * TI + t);
-        else return pad(t * L + i);
+        if constexpr (LAYOUT == MAP_T) {
+            if constexpr (TI == 1) return i;
+            else {
+                const unsigned h = unsigned(i) + (unsigned(i) >> 4) + (unsigned(i) >> 8);
+                return i * TI + (t ^ int(h & unsigned(TI - 1)));
+            }
+        } else {
+            if constexpr (L < 16) return t * L + i;
+            else return t * L + (i ^ int(((unsigned(i) >> 4) + unsigned(t)) & 15u));
+        }
     }
 
     template <int MAP, int NB>
     static __device__ __forceinline__ void bmap(int q, int& t, int& j) {
+        const unsigned uq = unsigned(q);
         if constexpr (MAP == MAP_J) {
-            t = q / NB;
-            j = q % NB;
+            t = int(uq / unsigned(NB));
+            j = int(uq % unsigned(NB));
         } else {
-            t = q % TI;
-            j = q / TI;
+            t = int(uq % unsigned(TI));
+            j = int(uq / unsigned(TI));
         }
     }
 
@@ -276,18 +321,17 @@ struct Engine {
     }
 
     template <int s>
-    static __device__ __forceinline__ void compute(cpx<T>* v, int tid, const cpx<T>* __restrict__ coarse) {
+    static __device__ __forceinline__ void compute(cpx<T>* v, int tid, const cpx<T>* __restrict__ stage_tw) {
         constexpr int R = P::radix(s), NB = L / R, BPT = EPT / R, NS = P::ns(s);
 #pragma unroll
         for (int b = 0; b < BPT; ++b) {
             if constexpr (NS > 1) {
                 int t, j;
                 bmap<map_of(s), NB>(tid + b * NT, t, j);
-                const int k = j % NS;
-                // W_{NS*R}^{r k} = coarse[r k 2^13 / (NS R)]
-                constexpr int STEP = TW_COARSE / (NS * R);
+                // Row k = j mod NS of this stage's table: W_{NS R}^{r k}, r < R.
+                const cpx<T>* row = stage_tw + 2 * L + StageTable<L>::off(s) + (j % NS) * R;
 #pragma unroll
-                for (int r = 1; r < R; ++r) v[b * R + r] = cmul(v[b * R + r], dirw<S>(coarse[r * k * STEP]));
+                for (int r = 1; r < R; ++r) v[b * R + r] = cmul(v[b * R + r], dirw<S>(row[r]));
             }
             dft<R, S>(v + b * R);
         }
@@ -308,17 +352,19 @@ struct Engine {
     }
 
     // Last-stage outputs (natural order k = j + r L/R) through a store
-    // functor st(t, k, value).
+    // functor st(t, j, r, value) -- the functor sees the butterfly (t, j) and
+    // the output index k = j + r * (L / R_last).
+    static constexpr int RLAST = P::radix(NST - 1);
+    static constexpr int NBLAST = L / RLAST;
     template <class ST>
     static __device__ __forceinline__ void store_last(const cpx<T>* v, int tid, ST&& st) {
         constexpr int s = NST - 1;
-        constexpr int R = P::radix(s), NB = L / R, BPT = EPT / R;
+        constexpr int R = RLAST, NB = NBLAST, BPT = EPT / R;
 #pragma unroll
         for (int b = 0; b < BPT; ++b) {
             int t, j;
             bmap<map_of(s), NB>(tid + b * NT, t, j);
-#pragma unroll
-            for (int r = 0; r < R; ++r) st(t, j + r * NB, v[b * R + r]);
+            st(t, j, v + b * R);
         }
     }
 
@@ -326,36 +372,52 @@ struct Engine {
     // of each exchange protects the LDS buffer against the previous stage's
     // readers (WAR); the trailing one publishes the writes (RAW).
     template <int s = 1>
-    static __device__ __forceinline__ void middle(cpx<T>* v, int tid, cpx<T>* lds, const cpx<T>* coarse,
+    static __device__ __forceinline__ void middle(cpx<T>* v, int tid, cpx<T>* lds, const cpx<T>* stage_tw,
                                                   bool lead_barrier) {
         if constexpr (s < NST) {
             if (lead_barrier) __syncthreads();
             store_lds<s - 1>(v, tid, lds);
             __syncthreads();
             load_lds<s>(v, tid, lds);
-            compute<s>(v, tid, coarse);
-            middle<s + 1>(v, tid, lds, coarse, true);
+            compute<s>(v, tid, stage_tw);
+            middle<s + 1>(v, tid, lds, stage_tw, true);
         }
     }
 
-    // Whole transform: load0 -> stages -> store_last.
+    // Store functor adaptor for element-wise stores st(t, k, value).
+    template <class ST>
+    struct PerElem {
+        ST st;
+        __device__ __forceinline__ void operator()(int t, int j, const cpx<T>* w) const {
+#pragma unroll
+            for (int r = 0; r < RLAST; ++r) st(t, j + r * NBLAST, w[r]);
+        }
+    };
+    template <class ST>
+    static __device__ __forceinline__ PerElem<ST> per_elem(ST st) { return PerElem<ST>{st}; }
+
+    // Whole transform: load0 -> stages -> store_last (element-wise store).
     template <class LD, class ST>
-    static __device__ __forceinline__ void run(int tid, cpx<T>* lds, const cpx<T>* coarse, LD&& ld, ST&& st) {
+    static __device__ __forceinline__ void run(int tid, cpx<T>* lds, const cpx<T>* stage_tw, LD&& ld, ST&& st) {
         cpx<T> v[EPT];
         load0(v, tid, ld);
-        compute<0>(v, tid, coarse);
-        middle<1>(v, tid, lds, coarse, false);
-        store_last(v, tid, st);
+        compute<0>(v, tid, stage_tw);
+        middle<1>(v, tid, lds, stage_tw, false);
+        store_last(v, tid, per_elem(st));
     }
 
     // Transform whose output is left in LDS in natural order (index lidx(t, k)).
+    // lead_barrier: a barrier is needed before the first LDS write (the buffer
+    // was read earlier by other threads).
     template <class LD>
-    static __device__ __forceinline__ void run_to_lds(int tid, cpx<T>* lds, const cpx<T>* coarse, LD&& ld) {
+    static __device__ __forceinline__ void run_to_lds(int tid, cpx<T>* lds, const cpx<T>* stage_tw, LD&& ld,
+                                                      bool lead_barrier = false) {
         cpx<T> v[EPT];
         load0(v, tid, ld);
-        compute<0>(v, tid, coarse);
-        middle<1>(v, tid, lds, coarse, false);
+        compute<0>(v, tid, stage_tw);
+        middle<1>(v, tid, lds, stage_tw, lead_barrier);
         if constexpr (NST > 1) __syncthreads();
+        else if (lead_barrier) __syncthreads();
         store_lds<NST - 1>(v, tid, lds);
         __syncthreads();
     }
@@ -363,12 +425,12 @@ struct Engine {
     // Transform whose natural-order input is already in LDS (a barrier must
     // separate the writes of that input from this call).
     template <class ST>
-    static __device__ __forceinline__ void run_from_lds(int tid, cpx<T>* lds, const cpx<T>* coarse, ST&& st) {
+    static __device__ __forceinline__ void run_from_lds(int tid, cpx<T>* lds, const cpx<T>* stage_tw, ST&& st) {
         cpx<T> v[EPT];
         load_lds<0>(v, tid, lds);
-        compute<0>(v, tid, coarse);
-        middle<1>(v, tid, lds, coarse, true);
-        store_last(v, tid, st);
+        compute<0>(v, tid, stage_tw);
+        middle<1>(v, tid, lds, stage_tw, true);
+        store_last(v, tid, per_elem(st));
     }
 };
 

@@ -24,7 +24,15 @@ struct DeviceCtx {
     c64* c64f = nullptr;
     c32* c32c = nullptr;
     c32* c32f = nullptr;
+    c64* c64s = nullptr;  // per-stage tables (TW_STAGE_ELEMS entries)
+    c32* c32s = nullptr;
+    int ncu = 256;        // compute units
 };
+template <typename T>
+inline Twiddles<T> ctx_twiddles(const DeviceCtx* c) {
+    if constexpr (sizeof(T) == 4) return Twiddles<T>{c->c32c, c->c32f, c->c32s};
+    else return Twiddles<T>{c->c64c, c->c64f, c->c64s};
+}
 int get_ctx(DeviceCtx** out);
 
 // Buffers a launch reads or writes.
@@ -40,6 +48,12 @@ struct Tune {
     int tilog_a = -1;     // tile width (log2) of column passes
     int tilog_b = -1;     // tile width (log2) of transpose passes
     int chunk = -1;       // transforms per four-step chunk (0/-1 = whole batch)
+    int wg_per_cu = -1;   // resident groups per CU of persistent passes (-1 = occupancy limit)
+    int xcd_remap = -1;   // 1 = contiguous tiles per XCD (-1 = default)
+    bool operator==(const Tune& o) const {
+        return split_log1 == o.split_log1 && tilog_a == o.tilog_a && tilog_b == o.tilog_b && chunk == o.chunk &&
+               wg_per_cu == o.wg_per_cu && xcd_remap == o.xcd_remap;
+    }
 };
 
 struct Launch {
@@ -54,6 +68,8 @@ struct Launch {
     PassDesc desc{};
     int src = B_IN;
     int dst = B_OUT;
+    int64_t off_src = 0;  // element offsets (complex units) into src / dst
+    int64_t off_dst = 0;
     size_t bytes = 0;   // LK_COPY
     int special = 0;    // LK_REAL: which real-FFT kernel
     // args[0..5]: kernel arguments of LK_REAL launches.
@@ -84,9 +100,11 @@ int build_plan(Plan& p);
 int build_plan_real(Plan& p);
 int exec_special(const Plan& p, const Launch& l, void* const* bufs, int isign, hipStream_t stream, DeviceCtx* ctx);
 void tune_lookup(Plan& p);
+Tune default_tune(const Plan& p);
 int plan_c2c_1d(Plan& p, int logn, int64_t batch, int src, int dst, size_t* tmp_elems);
 int plan_add_c2c_pass(Plan& p, int mode, int logl, int tilog, const PassDesc& d, int src, int dst);
-int exec_plan(const Plan& p, const void* in, void* out, void* tmp, const void* aux, int isign, hipStream_t stream);
+int exec_plan(const Plan& p, const void* in, void* out, void* tmp, const void* aux, int isign, hipStream_t stream,
+              int first = 0, int count = -1);
 int cached_plan(int kind, int dtype, const std::vector<size_t>& nn, size_t batch, size_t extra,
                 std::shared_ptr<Plan>* out);
 

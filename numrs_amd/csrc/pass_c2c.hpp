@@ -20,43 +20,136 @@ struct PassDesc {
     int32_t so0, eso;
     int32_t tw_logn;        // 0 = no four-step twiddle
     int32_t tw_b0_shift;
+    int32_t xcd_remap;      // 1: give each XCD a contiguous run of tiles
 };
 
+// Persistent pass kernel: group g walks tiles g, g + G, g + 2G, ... (G =
+// gridDim.x, sized to the groups that fit on the chip) and prefetches the
+// next tile's stage-0 inputs into registers before computing the current
+// one, so the global loads of one tile overlap the LDS/VALU work of the
+// previous one.  With xcd_remap, group ids are renumbered so the groups of
+// one XCD (blockIdx mod 8 on MI355X) take adjacent tiles and share their L2
+// lines.
+// Register prefetch of the next tile (1) costs occupancy that the no-prefetch
+// form (0) turns into more resident groups; 0 measured faster on MI355X.
+#ifndef NRFFT_PREFETCH
+#define NRFFT_PREFETCH 0
+#endif
+
+// Waves per SIMD a kernel is compiled for: as many groups as its LDS lets
+// share a CU (MI355X: 160 KB LDS, 4 SIMDs), capped at 4 with register
+// prefetch (128 VGPRs per lane) and at 8 otherwise.
+template <int LDSB, int NT, bool PF>
+constexpr int waves_per_eu_target() {
+    constexpr int per_cu = LDSB > 0 ? (163840 / LDSB) : 16;
+    constexpr int w = (per_cu * (NT < 64 ? 1 : NT / 64)) / 4;
+    constexpr int cap = PF ? 4 : 8;
+    return w < 1 ? 1 : (w > cap ? cap : w);
+}
+template <typename T, int L, int TI, int NT>
+constexpr int kc2c_waves =
+    waves_per_eu_target<(RadixPlan<L>::NST > 1 ? TI * L * (int)sizeof(cpx<T>) : 0), NT, NRFFT_PREFETCH != 0>();
+
+// Thread index laundered through a register move the compiler cannot hoist,
+// so per-tile address arithmetic is recomputed inside the persistent loop
+// rather than kept live across it.
+__device__ __forceinline__ int fresh_tid(int tid) {
+    int r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(tid));
+    return r;
+}
+
 template <typename T, int S, int L, int TI, int NT, int ML, int MS>
-__global__ __launch_bounds__(NT) void k_c2c(const cpx<T>* in, cpx<T>* out, PassDesc d, Twiddles<T> tw) {
+__global__ __attribute__((amdgpu_flat_work_group_size(1, NT), amdgpu_waves_per_eu(kc2c_waves<T, L, TI, NT>))) void
+k_c2c(const cpx<T>* in, cpx<T>* out, PassDesc d, Twiddles<T> tw) {
     using E = Engine<T, S, L, TI, NT, ML, MS>;
+    constexpr int EPT = E::EPT;
     __shared__ cpx<T> lds[E::NST > 1 ? E::LDS_ELEMS : 1];
+    const int tid = threadIdx.x;
     const int64_t tiles0 = (d.nb0 + TI - 1) / TI;
-    int64_t blk = blockIdx.x;
-    const int64_t tile0 = blk % tiles0;
-    blk /= tiles0;
-    const int64_t b1 = blk % d.nb1;
-    const int64_t b2 = blk / d.nb1;
-    const int64_t b0base = tile0 * TI;
-    const int64_t rem = d.nb0 - b0base;
-    const int nvalid = rem < TI ? (int)rem : TI;
-    const cpx<T>* ib = in + b0base * d.si0 + b1 * d.si1 + b2 * d.si2;
-    cpx<T>* ob = out + b0base * d.so0 + b1 * d.so1 + b2 * d.so2;
+    const int64_t ntiles = tiles0 * d.nb1 * d.nb2;
     const int si0 = d.si0, esi = d.esi, so0 = d.so0, eso = d.eso;
-    auto ld = [&](int t, int i) -> cpx<T> {
-        if (TI == 1 || t < nvalid) return ib[t * si0 + i * esi];
-        return cpx<T>{(T)0, (T)0};
+    int64_t G = gridDim.x;
+    int64_t tile = blockIdx.x;
+    if (d.xcd_remap && (G % 8) == 0) tile = (tile % 8) * (G / 8) + tile / 8;
+    if (tile >= ntiles) return;
+
+    struct Ctx {
+        const cpx<T>* ib;
+        cpx<T>* ob;
+        int64_t b0base;
+        int nvalid;
     };
-    if (d.tw_logn) {
-        const int logn = d.tw_logn, sh = d.tw_b0_shift;
-        auto st = [&](int t, int k, cpx<T> v) {
-            if (TI == 1 || t < nvalid) {
-                const uint32_t n2 = (uint32_t)((b0base + t) >> sh);
-                v = cmul(v, twiddle_big<S>(tw, (uint32_t)k * n2, logn));
-                ob[t * so0 + k * eso] = v;
-            }
-        };
-        E::run(threadIdx.x, lds, tw.coarse, ld, st);
-    } else {
-        auto st = [&](int t, int k, cpx<T> v) {
-            if (TI == 1 || t < nvalid) ob[t * so0 + k * eso] = v;
-        };
-        E::run(threadIdx.x, lds, tw.coarse, ld, st);
+    auto ctx_of = [&](int64_t tl) {
+        Ctx c;
+        const int64_t t0 = tl % tiles0;
+        const int64_t rest = tl / tiles0;
+        const int64_t b1 = rest % d.nb1, b2 = rest / d.nb1;
+        c.b0base = t0 * TI;
+        const int64_t rem = d.nb0 - c.b0base;
+        c.nvalid = rem < TI ? (int)rem : TI;
+        c.ib = in + c.b0base * si0 + b1 * d.si1 + b2 * d.si2;
+        c.ob = out + c.b0base * so0 + b1 * d.so1 + b2 * d.so2;
+        return c;
+    };
+    auto load = [&](cpx<T>* v, const Ctx& c, int ltid) {
+        E::load0(v, ltid, [&](int t, int i) -> cpx<T> {
+            if (TI == 1 || t < c.nvalid) return c.ib[t * si0 + i * esi];
+            return cpx<T>{(T)0, (T)0};
+        });
+    };
+
+#if NRFFT_PREFETCH
+    Ctx cn = ctx_of(tile);
+    cpx<T> vn[EPT];
+    load(vn, cn, fresh_tid(tid));
+#endif
+    bool first = true;
+    for (;;) {
+        cpx<T> v[EPT];
+#if NRFFT_PREFETCH
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) v[e] = vn[e];
+        const Ctx c = cn;
+        const int64_t next = tile + G;
+        const bool has_next = next < ntiles;
+        if (has_next) {
+            cn = ctx_of(next);
+            load(vn, cn, fresh_tid(tid));
+        }
+        const int ltid = fresh_tid(tid);
+#else
+        const Ctx c = ctx_of(tile);
+        const int64_t next = tile + G;
+        const bool has_next = next < ntiles;
+        const int ltid = fresh_tid(tid);
+        load(v, c, ltid);
+#endif
+        E::template compute<0>(v, ltid, tw.stage);
+        E::template middle<1>(v, ltid, lds, tw.stage, !first);
+        if (d.tw_logn) {
+            const int logn = d.tw_logn, sh = d.tw_b0_shift;
+            const uint32_t mask = (1u << logn) - 1u;
+            E::store_last(v, ltid, [&](int t, int j, const cpx<T>* w) {
+                if (TI == 1 || t < c.nvalid) {
+                    const uint32_t n2 = uint32_t((c.b0base + t) >> sh);
+                    cpx<T> f[E::RLAST];
+                    twiddle_run<E::RLAST, S>(tw, (uint32_t(j) * n2) & mask, (uint32_t(E::NBLAST) * n2) & mask, logn, f);
+#pragma unroll
+                    for (int r = 0; r < E::RLAST; ++r) c.ob[t * so0 + (j + r * E::NBLAST) * eso] = cmul(w[r], f[r]);
+                }
+            });
+        } else {
+            E::store_last(v, ltid, [&](int t, int j, const cpx<T>* w) {
+                if (TI == 1 || t < c.nvalid) {
+#pragma unroll
+                    for (int r = 0; r < E::RLAST; ++r) c.ob[t * so0 + (j + r * E::NBLAST) * eso] = w[r];
+                }
+            });
+        }
+        if (!has_next) break;
+        first = false;
+        tile = next;
     }
 }
 

@@ -249,6 +249,10 @@ int build_plan_real(Plan& p) {
     p.launches.push_back(real_launch(RS_UNTANGLE, B_TMP2, B_TMP2, {batch, L, 1}));
     if ((rc = plan_c2c_1d(p, logl, batch, B_IN, B_OUT, &tmp))) return rc;
     p.launches.push_back(real_launch(RS_UNTANGLE, B_OUT, B_OUT, {batch, L, 1}));
+    // Both forward spectra use isign = +1 whatever isign the call carries
+    // (for convlv isign selects convolution or deconvolution).
+    for (auto& l : p.launches)
+        if (l.kind == LK_C2C) l.args[6] = 1;
     p.launches.push_back(real_launch(RS_SPEC, B_OUT, B_OUT, {batch, L}));
     // The inverse C2C of the last step runs with isign = -1 whatever isign
     // the call carries (isign selects conv/deconv); args[6] marks it.
@@ -271,9 +275,7 @@ static unsigned grid_for(int64_t work, int nt) {
 template <typename T>
 static int exec_real_t(const Plan& p, const Launch& l, void* const* bufs, int isign, hipStream_t stream,
                        DeviceCtx* ctx) {
-    Twiddles<T> tw;
-    if constexpr (sizeof(T) == 4) tw = Twiddles<T>{ctx->c32c, ctx->c32f};
-    else tw = Twiddles<T>{ctx->c64c, ctx->c64f};
+    Twiddles<T> tw = ctx_twiddles<T>(ctx);
     void* src = bufs[l.src];
     void* dst = bufs[l.dst];
     const int64_t* a = l.args;

@@ -98,7 +98,7 @@ __global__ __launch_bounds__(NT) void k_realft(const cpx<T>* in, cpx<T>* out, in
     cpx<T>* ob = out + s0 * L;
     const T c2 = (T)(-0.5 * DIR);
     if constexpr (DIR > 0) {
-        E::run_to_lds(tid, lds, tw.coarse, [&](int t, int i) -> cpx<T> {
+        E::run_to_lds(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
             return (TI == 1 || t < nvalid) ? ib[t * L + i] : cpx<T>{(T)0, (T)0};
         });
         for (int q = tid; q < TI * NP; q += NT) {
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(NT) void k_realft(const cpx<T>* in, cpx<T>* out, in
             }
         }
         __syncthreads();
-        E::run_from_lds(tid, lds, tw.coarse, [&](int t, int k, cpx<T> v) {
+        E::run_from_lds(tid, lds, tw.stage, [&](int t, int k, cpx<T> v) {
             if (TI == 1 || t < nvalid) ob[t * L + k] = v;
         });
     }
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(NT) void k_convlv(const cpx<T>* data, const T* seco
     const T inv_no2 = (T)1 / (T)L;
 
     // 1. Spectrum of the signal, kept in registers pair by pair.
-    EF::run_to_lds(tid, lds, tw.coarse, [&](int t, int i) -> cpx<T> {
+    EF::run_to_lds(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
         return (TI == 1 || t < nvalid) ? db[t * L + i] : cpx<T>{(T)0, (T)0};
     });
     cpx<T> xa[PPT], xb[PPT];
@@ -182,12 +182,12 @@ __global__ __launch_bounds__(NT) void k_convlv(const cpx<T>* data, const T* seco
     // 2. Spectrum of the response (or of data2).
     if (op == OP_CORREL) {
         const cpx<T>* sb = reinterpret_cast<const cpx<T>*>(second) + s0 * L;
-        EF::run_to_lds(tid, lds, tw.coarse, [&](int t, int i) -> cpx<T> {
+        EF::run_to_lds(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
             return (TI == 1 || t < nvalid) ? sb[t * L + i] : cpx<T>{(T)0, (T)0};
         });
     } else {
         const T* rb = second + s0 * m;
-        EF::run_to_lds(tid, lds, tw.coarse, [&](int t, int i) -> cpx<T> {
+        EF::run_to_lds(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
             if (TI > 1 && t >= nvalid) return cpx<T>{(T)0, (T)0};
             const T* r = rb + t * m;
             return cpx<T>{placed_response(r, 2 * int64_t(i), N, m), placed_response(r, 2 * int64_t(i) + 1, N, m)};
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(NT) void k_convlv(const cpx<T>* data, const T* seco
     __syncthreads();
 
     // 3. Inverse FFT: (n/2) * (result / no2) = result.
-    EI::run_from_lds(tid, lds, tw.coarse, [&](int t, int k, cpx<T> v) {
+    EI::run_from_lds(tid, lds, tw.stage, [&](int t, int k, cpx<T> v) {
         if (TI == 1 || t < nvalid) ob[t * L + k] = v;
     });
 }

@@ -52,7 +52,10 @@ class _Plan:
         self._h = ctypes.c_void_p()
 
     def __del__(self):
-        self.destroy()
+        try:
+            self.destroy()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
 
     def destroy(self) -> None:
         if getattr(self, "_h", None) and self._h.value:
@@ -69,17 +72,50 @@ class _Plan:
 
 
 class C2CPlan(_Plan):
-    """Batched C2C of shape nn (C order); ndim 1 = four1, ndim >= 2 = fourn."""
+    """Batched C2C of shape nn (C order); ndim 1 = four1, ndim >= 2 = fourn.
+    ``tune`` = (split_log1, tilog_a, tilog_b, chunk, wg_per_cu, xcd_remap)
+    (a prefix is enough; -1 = default) overrides the cached or default
+    tunables for this key."""
 
-    def __init__(self, dtype, nn: Sequence[int], batch: int = 1):
+    def __init__(self, dtype, nn: Sequence[int], batch: int = 1, tune: Sequence[int] | None = None):
         super().__init__()
         nn = [int(v) for v in nn]
         arr = (ctypes.c_size_t * len(nn))(*nn)
-        _check(_lib.lib().nrfft_plan_c2c(ctypes.byref(self._h), _dtype_code(dtype), arr, len(nn), batch))
+        if tune is None:
+            _check(_lib.lib().nrfft_plan_c2c(ctypes.byref(self._h), _dtype_code(dtype), arr, len(nn), batch))
+        else:
+            t = (ctypes.c_int * len(tune))(*[int(v) for v in tune])
+            _check(_lib.lib().nrfft_plan_c2c_tuned(ctypes.byref(self._h), _dtype_code(dtype), arr, len(nn), batch,
+                                                   t, len(tune)))
         self.nn, self.batch = nn, batch
 
     def execute(self, x_in, x_out, isign: int, stream=None) -> None:
         _check(_lib.lib().nrfft_execute_c2c(self._h, _ptr(x_in), _ptr(x_out), isign, _stream(stream)))
+
+    def execute_launches(self, x_in, x_out, isign: int, first: int, count: int, stream=None) -> None:
+        """Enqueue only launches [first, first+count) (e.g. one pass)."""
+        _check(_lib.lib().nrfft_execute_c2c_launches(self._h, _ptr(x_in), _ptr(x_out), isign, _stream(stream),
+                                                      first, count))
+
+
+def autotune_c2c(dtype, nn: Sequence[int], batch: int = 1, iters: int = 5):
+    """Time every candidate tuning of a C2C plan on the current GPU and cache
+    the fastest.  Returns ((split_log1, tilog_a, tilog_b, chunk, wg_per_cu,
+    xcd_remap), ms)."""
+    nn = [int(v) for v in nn]
+    arr = (ctypes.c_size_t * len(nn))(*nn)
+    best = (ctypes.c_int * 6)()
+    ms = ctypes.c_double()
+    _check(_lib.lib().nrfft_autotune_c2c(_dtype_code(dtype), arr, len(nn), batch, iters, best, ctypes.byref(ms)))
+    return tuple(best), ms.value
+
+
+def tune_cache_save(path: str) -> None:
+    _check(_lib.lib().nrfft_tune_cache_save(path.encode()))
+
+
+def tune_cache_load(path: str) -> None:
+    _check(_lib.lib().nrfft_tune_cache_load(path.encode()))
 
 
 class RealFTPlan(_Plan):

@@ -216,7 +216,11 @@ def test_correl_reference_cases():
     res = numrs_amd.correl_batch([([1.0, 2.0], [1.0, 2.0]), ([3.0, 4.0], [3.0, 4.0])])
     assert abs(res[0][0] - 5.0) < 1e-10 and abs(res[1][0] - 25.0) < 1e-10
     assert np.allclose(numrs_amd.correl([1.0, 2.0], [1.0, 2.0]), [5.0, 2.0])
-    assert abs(numrs_amd.correl_normalized([1.0, 2.0, 3.0, 4.0], [1.0, 2.0, 3.0, 4.0])[0] - 1.0) < 1e-10
+    # correl_normalized correlates z-scores without dividing by n
+    # (Correlation.rs:189-223), so lag 0 is n * rho = 4 here; the reference's
+    # own test_normalized_correlation (Correlation.rs:435-450) expects 1.0,
+    # which its code cannot produce.
+    assert abs(numrs_amd.correl_normalized([1.0, 2.0, 3.0, 4.0], [1.0, 2.0, 3.0, 4.0])[0] - 4.0) < 1e-10
     assert abs(numrs_amd.correl_normalized_fast([1.0, 2.0, 3.0, 4.0], [1.0, 2.0, 3.0, 4.0])[0] - 1.0) < 1e-10
 
 
