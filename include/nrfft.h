@@ -110,6 +110,15 @@ int nrfft_plan_c2c(nrfft_plan* plan, int dtype, const size_t* nn, size_t ndim, s
 /* Out of place or in place (d_out == d_in). */
 int nrfft_execute_c2c(nrfft_plan plan, const void* d_in, void* d_out, int isign, void* stream);
 
+/* Split-precision form of an FP32 C2C plan (no counterpart in the reference):
+ * radix-16 stages as 16x16 complex DFT-matrix products on MFMA with FP16
+ * hi/lo (NRFFT_SPLIT_F16, 3 products) or BF16 hi/mid/lo (NRFFT_SPLIT_BF16,
+ * 6 products) operands and FP32 accumulation; twiddles in FP32 from FP64
+ * tables.  Passes whose length has no MFMA kernel run the plain FP32 kernels.
+ * Executed with nrfft_execute_c2c. */
+enum { NRFFT_SPLIT_F16 = 1, NRFFT_SPLIT_BF16 = 2 };
+int nrfft_plan_c2c_split(nrfft_plan* plan, const size_t* nn, size_t ndim, size_t batch, int prec);
+
 /* realft on `batch` signals of n reals, in place. */
 int nrfft_plan_realft(nrfft_plan* plan, int dtype, size_t n, size_t batch);
 int nrfft_execute_realft(nrfft_plan plan, void* d_data, int isign, void* stream);

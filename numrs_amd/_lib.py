@@ -29,6 +29,8 @@ E_INVALID = -10
 
 F32 = 0
 F64 = 1
+SPLIT_F16 = 1
+SPLIT_BF16 = 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -58,6 +60,7 @@ _SIGS = {
     "nrfft_correl_f32": (_i, [_fp, _fp, _sz, _fp]),
     "nrfft_plan_c2c": (_i, [ctypes.POINTER(_vp), _i, _szp, _sz, _sz]),
     "nrfft_execute_c2c": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "nrfft_plan_c2c_split": (_i, [ctypes.POINTER(_vp), _szp, _sz, _sz, _i]),
     "nrfft_plan_realft": (_i, [ctypes.POINTER(_vp), _i, _sz, _sz]),
     "nrfft_execute_realft": (_i, [_vp, _vp, _i, _vp]),
     "nrfft_plan_convlv": (_i, [ctypes.POINTER(_vp), _i, _sz, _sz, _sz]),

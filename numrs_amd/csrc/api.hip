@@ -102,6 +102,7 @@ int get_ctx(DeviceCtx** out) {
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_stage_twiddles, dim3(TW_STAGE_ELEMS / 256), dim3(256), 0, 0, c->c64s, c->c32s);
     HIPCHK(hipGetLastError());
+    if (int rc = build_mfma_tables(c)) return rc;
     HIPCHK(hipDeviceSynchronize());
     g_ctx[dev] = c;
     *out = c;
@@ -131,8 +132,10 @@ int pick_tilog(int dtype, int mode, int logl, int64_t nb0, int want_log) {
 // Plans.
 // ---------------------------------------------------------------------------
 int plan_add_c2c_pass(Plan& p, int mode, int logl, int tilog, const PassDesc& d, int src, int dst) {
-    const KernelRef& kp = c2c_lookup(p.dtype, +1, mode, logl, tilog);
-    const KernelRef& km = c2c_lookup(p.dtype, -1, mode, logl, tilog);
+    const int prec = (p.kind == PK_C2C && p.dtype == NRFFT_F32) ? (int)p.extra : 0;
+    const bool use_mfma = prec != 0 && mfma_lookup(prec, +1, mode, logl).fn != nullptr;
+    const KernelRef& kp = use_mfma ? mfma_lookup(prec, +1, mode, logl) : c2c_lookup(p.dtype, +1, mode, logl, tilog);
+    const KernelRef& km = use_mfma ? mfma_lookup(prec, -1, mode, logl) : c2c_lookup(p.dtype, -1, mode, logl, tilog);
     if (!kp.fn || !km.fn) return fail(NRFFT_E_UNSUPPORTED, "no kernel for pass (mode " + std::to_string(mode) +
                                                                ", log2 L " + std::to_string(logl) + ")");
     Launch l;
@@ -140,6 +143,7 @@ int plan_add_c2c_pass(Plan& p, int mode, int logl, int tilog, const PassDesc& d,
     l.fn[0] = kp.fn;
     l.fn[1] = km.fn;
     l.nt = kp.nt;
+    l.prec = use_mfma ? prec : 0;
     const int64_t tiles0 = (d.nb0 + kp.ti - 1) / kp.ti;
     const int64_t ntiles = tiles0 * d.nb1 * d.nb2;
     if (ntiles <= 0) return fail(NRFFT_E_UNSUPPORTED, "empty pass");
@@ -209,6 +213,11 @@ int plan_c2c_1d(Plan& p, int logn, int64_t batch, int src, int dst, size_t* tmp_
     if (p.tune.split_log1 > 0 && p.tune.split_log1 < logn && logn - p.tune.split_log1 <= C2C_MAX_LOGL_ &&
         p.tune.split_log1 <= C2C_MAX_LOGL_) {
         l1 = p.tune.split_log1;
+        l2 = logn - l1;
+    }
+    if (p.kind == PK_C2C && p.extra != 0 && p.dtype == NRFFT_F32 && logn <= 20 && p.tune.split_log1 <= 0) {
+        // split-precision plans: both passes on MFMA kernels (L <= 1024)
+        l1 = logn / 2;
         l2 = logn - l1;
     }
     const int64_t N1 = int64_t(1) << l1, N2 = int64_t(1) << l2;
@@ -340,7 +349,12 @@ int exec_plan(const Plan& p, const void* in, void* out, void* tmp, const void* a
         }
         if (l.kind == LK_C2C) {
             PassDesc d = l.desc;
-            if (p.dtype == NRFFT_F32) {
+            if (p.dtype == NRFFT_F32 && l.prec) {
+                Twiddles<float> tw = ctx_twiddles<float>(ctx);
+                const void* amat = l.prec == NRFFT_SPLIT_F16 ? ctx->mfma_f16 : ctx->mfma_bf16;
+                void* args[] = {&a, &b, &d, &tw, &amat};
+                HIPCHK(hipLaunchKernel(l.fn[si], dim3(l.grid), dim3(l.nt), args, 0, stream));
+            } else if (p.dtype == NRFFT_F32) {
                 Twiddles<float> tw = ctx_twiddles<float>(ctx);
                 void* args[] = {&a, &b, &d, &tw};
                 HIPCHK(hipLaunchKernel(l.fn[si], dim3(l.grid), dim3(l.nt), args, 0, stream));
@@ -567,6 +581,15 @@ int nrfft_plan_c2c(nrfft_plan* plan, int dtype, const size_t* nn, size_t ndim, s
     int rc = check_fourn(nn, nn, ndim, 1, &d);
     if (rc) return rc;
     return make_plan(plan, PK_C2C, dtype, d, batch, 0);
+}
+
+int nrfft_plan_c2c_split(nrfft_plan* plan, const size_t* nn, size_t ndim, size_t batch, int prec) {
+    g_err.clear();
+    if (prec != NRFFT_SPLIT_F16 && prec != NRFFT_SPLIT_BF16) return fail(NRFFT_E_INVALID, "precision");
+    std::vector<size_t> d;
+    int rc = check_fourn(nn, nn, ndim, 1, &d);
+    if (rc) return rc;
+    return make_plan(plan, PK_C2C, NRFFT_F32, d, batch, (size_t)prec);
 }
 
 int nrfft_execute_c2c(nrfft_plan plan, const void* d_in, void* d_out, int isign, void* stream) {

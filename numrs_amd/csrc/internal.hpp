@@ -26,8 +26,12 @@ struct DeviceCtx {
     c32* c32f = nullptr;
     c64* c64s = nullptr;  // per-stage tables (TW_STAGE_ELEMS entries)
     c32* c32s = nullptr;
+    void* mfma_f16 = nullptr;   // split-precision DFT-matrix operands
+    void* mfma_bf16 = nullptr;
     int ncu = 256;        // compute units
 };
+int build_mfma_tables(DeviceCtx* c);
+const KernelRef& mfma_lookup(int prec, int sign, int mode, int logl);
 template <typename T>
 inline Twiddles<T> ctx_twiddles(const DeviceCtx* c) {
     if constexpr (sizeof(T) == 4) return Twiddles<T>{c->c32c, c->c32f, c->c32s};
@@ -68,6 +72,7 @@ struct Launch {
     PassDesc desc{};
     int src = B_IN;
     int dst = B_OUT;
+    int prec = 0;         // 0 = plain kernel, else split-precision MFMA kernel (MfmaPrec)
     int64_t off_src = 0;  // element offsets (complex units) into src / dst
     int64_t off_dst = 0;
     size_t bytes = 0;   // LK_COPY
@@ -83,7 +88,7 @@ struct Plan {
     int dtype = 0;
     std::vector<size_t> nn;
     size_t batch = 1;
-    size_t extra = 0;  // convlv: response length m
+    size_t extra = 0;  // convlv: response length m; C2C: split precision (0 = plain FP32/FP64)
     int device = 0;
     Tune tune;
     std::vector<Launch> launches;

@@ -73,15 +73,28 @@ class _Plan:
 
 class C2CPlan(_Plan):
     """Batched C2C of shape nn (C order); ndim 1 = four1, ndim >= 2 = fourn.
+    ``split`` = "f16" / "bf16" selects the split-precision MFMA form (float32).
     ``tune`` = (split_log1, tilog_a, tilog_b, chunk, wg_per_cu, xcd_remap)
     (a prefix is enough; -1 = default) overrides the cached or default
     tunables for this key."""
 
-    def __init__(self, dtype, nn: Sequence[int], batch: int = 1, tune: Sequence[int] | None = None):
+    def __init__(self, dtype, nn: Sequence[int], batch: int = 1, tune: Sequence[int] | None = None,
+                 split: str | None = None):
         super().__init__()
         nn = [int(v) for v in nn]
         arr = (ctypes.c_size_t * len(nn))(*nn)
-        if tune is None:
+        if split is not None:
+            if _dtype_code(dtype) != _lib.F32:
+                raise TypeError("split precision applies to float32 plans")
+            prec = {"f16": _lib.SPLIT_F16, "bf16": _lib.SPLIT_BF16}[split]
+            if tune is not None:
+                t = (ctypes.c_int * len(tune))(*[int(v) for v in tune])
+                _check(_lib.lib().nrfft_plan_c2c_tuned(ctypes.byref(self._h), _lib.F32, arr, len(nn), batch, t,
+                                                       len(tune)))
+                self.destroy()
+                self._h = ctypes.c_void_p()
+            _check(_lib.lib().nrfft_plan_c2c_split(ctypes.byref(self._h), arr, len(nn), batch, prec))
+        elif tune is None:
             _check(_lib.lib().nrfft_plan_c2c(ctypes.byref(self._h), _dtype_code(dtype), arr, len(nn), batch))
         else:
             t = (ctypes.c_int * len(tune))(*[int(v) for v in tune])
