@@ -63,7 +63,7 @@ __global__ void k_stage_twiddles(c64* t64, c32* t32) {
         int ns = 1;
         const int nst = L < 16 ? 1 : (logl + 3) / 4;
         for (int st = 0; st < nst; ++st) {
-            const int R = L < 16 ? L : (st == 0 ? ((logl % 4) == 0 ? 16 : (1 << (logl % 4))) : 16);
+            const int R = plan_radix(L, logl, nst, st);
             if (o < ns * R) {
                 const int k = o / R, r = o % R;
                 sincospi(2.0 * (double)(r * k) / (double)(ns * R), &s, &c);

@@ -168,18 +168,33 @@ __device__ __forceinline__ void dft(cpx<T>* v) {
 }
 
 // ---------------------------------------------------------------------------
-// Radix plan: log2(L) bits covered by radix-16 stages, the remainder (2, 4 or
-// 8) taken first.  L < 16 is a single stage.
-// ---------------------------------------------------------------------------
+// Radix plan: log2(L) bits covered by radix-16 stages plus one radix-2/4/8
+// stage for the remainder, taken first (NRFFT_SMALL_RADIX_FIRST = 1, 10-15%
+// faster at L = 512..2048 on MI355X) or last.  Padded LDS addresses are affine
+// in the butterfly slot r whenever the stage's stride is a multiple of 16,
+// and are computed per element otherwise.  L < 16 is a single stage.
 __host__ __device__ constexpr int ilog2c(long long v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
+
+// Build switches: order of the small radix, and the LDS index scheme (padding
+// with affine offsets, or an XOR swizzle -- equal speed on MI355X).
+#ifndef NRFFT_SMALL_RADIX_FIRST
+#define NRFFT_SMALL_RADIX_FIRST 1
+#endif
+#ifndef NRFFT_LDS_SWIZZLE
+#define NRFFT_LDS_SWIZZLE 0
+#endif
+
+__host__ __device__ constexpr int plan_radix(int L, int LOG, int NST, int s) {
+    return (L < 16) ? L
+                    : (NRFFT_SMALL_RADIX_FIRST ? (s == 0 ? ((LOG % 4) == 0 ? 16 : (1 << (LOG % 4))) : 16)
+                                               : (s < NST - 1 ? 16 : ((LOG % 4) == 0 ? 16 : (1 << (LOG % 4)))));
+}
 
 template <int L>
 struct RadixPlan {
     static constexpr int LOG = ilog2c(L);
     static constexpr int NST = (L < 16) ? 1 : (LOG + 3) / 4;
-    static constexpr int radix(int s) {
-        return (L < 16) ? L : (s == 0 ? ((LOG % 4) == 0 ? 16 : (1 << (LOG % 4))) : 16);
-    }
+    static constexpr int radix(int s) { return plan_radix(L, LOG, NST, s); }
     static constexpr int ns(int s) {
         int p = 1;
         for (int i = 0; i < s; ++i) p *= radix(i);
@@ -259,25 +274,33 @@ struct Engine {
     // LDS layout follows the mapping used after the first stage.
     static constexpr int LAYOUT = (NST == 1) ? MAPL : MAPS;
     static constexpr int map_of(int s) { return s == 0 ? MAPL : MAPS; }
-    static constexpr int LDS_ELEMS = TI * L;
 
-    // Bank-conflict-avoiding LDS index of element i of tile transform t.
-    //   MAP_T layout ([i][TI]): the column index is XOR-swizzled with a hash of
-    //   the row, so 16 lanes that touch one row, or rows i, i+1.., i+4.., i+16..
-    //   of one column, fall in distinct slots.
-    //   MAP_J layout ([t][L]): the low 4 bits of i are XOR-ed with bits 4..7 of
-    //   i plus t, with the same effect along a row and across rows.
+    // Padded LDS index of element i of tile transform t: one pad slot per 16
+    // complex values, so strides that are multiples of 16 land in distinct
+    // banks (MAP_T layout [i][TI]: lanes across t are contiguous, lanes along
+    // i are 17 * TI / 16 apart; MAP_J layout [t][L]: lanes along i are
+    // contiguous, stride-16 runs are 17 apart).
+    // With NRFFT_LDS_SWIZZLE the MAP_T layout is unpadded and XOR-swizzles the
+    // column with a hash of the row instead (no affine offsets).
+    static constexpr int LSI = (LAYOUT == MAP_T) ? TI : 1;  // raw distance between i and i + 1
+    static constexpr bool SWZ = NRFFT_LDS_SWIZZLE && LAYOUT == MAP_T && TI > 1;
+    static constexpr int LDS_ELEMS = SWZ ? TI * L : TI * L + (TI * L) / 16 + 1;
     static __device__ __forceinline__ int lidx(int t, int i) {
-        if constexpr (LAYOUT == MAP_T) {
-            if constexpr (TI == 1) return i;
-            else {
-                const unsigned h = unsigned(i) + (unsigned(i) >> 4) + (unsigned(i) >> 8);
-                return i * TI + (t ^ int(h & unsigned(TI - 1)));
-            }
+        if constexpr (SWZ) {
+            const unsigned h = unsigned(i) + (unsigned(i) >> 4) + (unsigned(i) >> 8);
+            return i * TI + (t ^ int(h & unsigned(TI - 1)));
         } else {
-            if constexpr (L < 16) return t * L + i;
-            else return t * L + (i ^ int(((unsigned(i) >> 4) + unsigned(t)) & 15u));
+            const int raw = (LAYOUT == MAP_T) ? i * TI + t : t * L + i;
+            return raw + (raw >> 4);
         }
+    }
+    // lidx(t, i0 + r * STRIDE) = lidx(t, i0) + loff<STRIDE>(r) whenever the raw
+    // step STRIDE * LSI is a multiple of 16 (then the pad term is affine too).
+    template <int STRIDE>
+    static constexpr bool affine = !SWZ && ((STRIDE * LSI) % 16) == 0;
+    template <int STRIDE>
+    static constexpr int loff(int r) {
+        return r * (STRIDE * LSI + (STRIDE * LSI) / 16);
     }
 
     template <int MAP, int NB>
@@ -315,8 +338,14 @@ struct Engine {
         for (int b = 0; b < BPT; ++b) {
             int t, j;
             bmap<map_of(s), NB>(tid + b * NT, t, j);
+            if constexpr (affine<NB>) {
+                const cpx<T>* base = lds + lidx(t, j);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[b * R + r] = lds[lidx(t, j + r * NB)];
+                for (int r = 0; r < R; ++r) v[b * R + r] = base[loff<NB>(r)];
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[b * R + r] = lds[lidx(t, j + r * NB)];
+            }
         }
     }
 
@@ -345,9 +374,15 @@ struct Engine {
         for (int b = 0; b < BPT; ++b) {
             int t, j;
             bmap<map_of(s), NB>(tid + b * NT, t, j);
-            const int base = (j / NS) * NS * R + (j % NS);
+            const int i0 = (j / NS) * NS * R + (j % NS);
+            if constexpr (affine<NS>) {
+                cpx<T>* base = lds + lidx(t, i0);
 #pragma unroll
-            for (int r = 0; r < R; ++r) lds[lidx(t, base + r * NS)] = v[b * R + r];
+                for (int r = 0; r < R; ++r) base[loff<NS>(r)] = v[b * R + r];
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) lds[lidx(t, i0 + r * NS)] = v[b * R + r];
+            }
         }
     }
 

@@ -102,11 +102,11 @@ __device__ __forceinline__ f4v split_product(const typename MfmaTraits<PREC>::V*
 }
 
 template <int PREC, int L, int NT>
-constexpr int kmfma_waves = waves_per_eu_target<16 * L * 8, NT, true>();
+constexpr int kmfma_waves = waves_per_eu_target<2 * 17 * L * 4, NT, true>();
 
-// Tile: 16 transforms (TI = 16) of length L, LDS planes re/im in the
-// swizzled MAP_T layout of the VALU engine: position p of transform t at
-// p * 16 + (t ^ (h(p) & 15)).
+// Tile: 16 transforms (TI = 16) of length L, LDS planes re/im in the padded
+// MAP_T layout of the VALU engine: position p of transform t at p * 17 + t,
+// so every access below is a per-lane base plus compile-time offsets.
 template <int PREC, int S, int L, int NT, int ML, int MS>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, NT), amdgpu_waves_per_eu(kmfma_waves<PREC, L, NT>))) void
 k_c2c_mfma(const c32* in, c32* out, PassDesc d, Twiddles<float> tw, const void* amat) {
@@ -123,8 +123,8 @@ k_c2c_mfma(const c32* in, c32* out, PassDesc d, Twiddles<float> tw, const void* 
     static_assert(A >= 1 && G % W == 0 && GPW >= 1, "tile must split into whole groups per wave");
     static_assert(NB % 16 == 0 || ML == MAP_T, "row groups need 16 | L/16");
 
-    __shared__ float pre[TI * L];
-    __shared__ float pim[TI * L];
+    __shared__ float pre[17 * L];
+    __shared__ float pim[17 * L];
     __shared__ float red[W];
 
     const int tid0 = threadIdx.x;
@@ -153,10 +153,7 @@ k_c2c_mfma(const c32* in, c32* out, PassDesc d, Twiddles<float> tw, const void* 
         }
     }
 
-    auto lpos = [](int t, int p) -> int {
-        const unsigned h = unsigned(p) + (unsigned(p) >> 4) + (unsigned(p) >> 8);
-        return p * TI + (t ^ int(h & 15u));
-    };
+    auto lpos = [](int t, int p) -> int { return p * 17 + t; };
 
     bool first = true;
     for (;;) {
@@ -243,9 +240,9 @@ k_c2c_mfma(const c32* in, c32* out, PassDesc d, Twiddles<float> tw, const void* 
                         t = n;
                         j = gid;
                     }
-                    const float* plane = (kb >> 1) ? pim : pre;
+                    const float* src = ((kb >> 1) ? pim : pre) + lpos(t, j + 8 * (kb & 1) * NB);
 #pragma unroll
-                    for (int e = 0; e < 8; ++e) x[g][e] = plane[lpos(t, j + (8 * (kb & 1) + e) * NB)];
+                    for (int e = 0; e < 8; ++e) x[g][e] = src[e * NB * 17];
                 }
                 __syncthreads();  // all reads done before this stage's writes
             }

@@ -48,7 +48,10 @@ constexpr int waves_per_eu_target() {
 }
 template <typename T, int L, int TI, int NT>
 constexpr int kc2c_waves =
-    waves_per_eu_target<(RadixPlan<L>::NST > 1 ? TI * L * (int)sizeof(cpx<T>) : 0), NT, NRFFT_PREFETCH != 0>();
+    waves_per_eu_target<(RadixPlan<L>::NST > 1 ? (TI * L + (NRFFT_LDS_SWIZZLE ? 0 : TI * L / 16 + 1)) *
+                                                     (int)sizeof(cpx<T>)
+                                               : 0),
+                        NT, NRFFT_PREFETCH != 0>();
 
 // Thread index laundered through a register move the compiler cannot hoist,
 // so per-tile address arithmetic is recomputed inside the persistent loop
