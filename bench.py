@@ -75,6 +75,41 @@ def time_events(fn, reps: int) -> float:
     return e0.elapsed_time(e1) / reps
 
 
+def measured_traffic_ratio():
+    """HBM bytes / algorithmic bytes of the FP32 four-step passes, from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated)."""
+    path = os.path.join(ROOT, "profiles", "c2c_fp32_batch64_n2p20_traffic.txt")
+    try:
+        ratios = []
+        for line in open(path):
+            parts = line.split()
+            if len(parts) >= 5 and "nrfft::k_c2c<float," in line and parts[0].isdigit():
+                ratios.append(float(parts[3]) / 256.0)
+        return (max(ratios), os.path.relpath(path, ROOT)) if ratios else (None, None)
+    except OSError:
+        return None, None
+
+
+def _rel_l2_gpu(a, ref) -> float:
+    import torch
+
+    d = a.double() - ref
+    return float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(ref))
+
+
+def _time_plan(plan, x, y, steps: int, warmup: int) -> float:
+    import torch
+
+    for _ in range(warmup):
+        plan.execute(x, y, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.execute(x, y, 1)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
 def headline(args) -> dict:
     import torch
 
@@ -89,25 +124,23 @@ def headline(args) -> dict:
     fill_uniform(x, 3, 0, 2 * pts)
     plan = numrs_amd.C2CPlan("float32", [n], batch)
     nl = plan.num_launches
-    for _ in range(args.warmup):
-        plan.execute(x, y, 1)
-    torch.cuda.synchronize()
     log(f"[bench] timing {args.steps} steps ({nl} launches per step)")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.execute(x, y, 1)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = wall * 1e3 / args.steps
+    ms = _time_plan(plan, x, y, args.steps, args.warmup)
     flops = 5.0 * n * logn * batch
     gflops = flops / (ms * 1e-3) / 1e9
     compulsory = 2.0 * pts * 8  # one read + one write, 8 B per complex
-    # Per-launch durations (HIP events, same stream).
+    # Per-launch durations (HIP events on the stream the library launches on).
+    # Four-step plans alternate pass A / pass B over batch chunks; each launch
+    # reads and writes every point of its chunk once.
     reps = max(5, args.steps)
     per = [time_events(lambda i=i: plan.execute_launches(x, y, 1, i, 1), reps) for i in range(nl)]
-    dom = int(np.argmax(per))
-    pass_bytes = compulsory  # each pass reads and writes every point once
-    achieved = pass_bytes / (per[dom] * 1e-3) / 1e9
+    kinds = ["A", "B"] if nl % 2 == 0 and nl > 1 else ["pass"]
+    groups = {k: [per[i] for i in range(nl) if kinds[i % len(kinds)] == k] for k in kinds}
+    dom = max(groups, key=lambda k: sum(groups[k]))
+    launch_bytes = compulsory / (nl // len(kinds))
+    dom_ms = sum(groups[dom]) / len(groups[dom])
+    achieved = launch_bytes / (dom_ms * 1e-3) / 1e9
+    ratio, tsrc = measured_traffic_ratio()
     out = {
         "metric": METRIC,
         "value": round(gflops, 1),
@@ -122,17 +155,38 @@ def headline(args) -> dict:
         "dtype": "f32",
         "data": "synthetic: re/im i.i.d. uniform[-1,1) from SplitMix64 seed 3",
         "config": {"workload": "four1 C2C FP32, batch 64 x N=2^20, isign=+1, out of place, HBM-resident",
-                   "batch": batch, "n": n, "passes": nl, "parallelism": "single-gpu"},
+                   "batch": batch, "n": n, "launches": nl, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"launch {dom} of {nl} (four-step pass {'AB'[dom] if nl == 2 else dom})",
-                     "kernel_ms": round(per[dom], 4), "algorithmic_bytes_per_launch": int(pass_bytes)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": int(ratio * launch_bytes) if ratio else None,
+                     "traffic_source": tsrc,
+                     "kernel": f"four-step pass {dom} (k_c2c), {len(groups[dom])} launches per step",
+                     "kernel_ms": round(dom_ms, 4), "algorithmic_bytes_per_launch": int(launch_bytes)},
         "hbm_effective": {"GBps": round(compulsory / (ms * 1e-3) / 1e9, 1),
                           "frac": round(compulsory / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                          "note": "compulsory bytes (one read + one write) / whole step"},
-        "pass_ms": [round(p, 4) for p in per],
+                          "note": "compulsory bytes (one read + one write of the batch) / whole step"},
+        "launch_ms": [round(p, 4) for p in per],
     }
-    del x, y
+    # FP64 reference of the same FP32 inputs from the FP64 path (parity-tested
+    # against the oracle at 1e-12), for the relative L2 of every FP32 form.
+    log("[bench] fp64 reference and split-precision MFMA forms")
+    x64 = x.double()
+    y64 = torch.empty_like(x64)
+    p64 = numrs_amd.C2CPlan("float64", [n], batch)
+    p64.execute(x64, y64, 1)
+    torch.cuda.synchronize()
+    del x64, p64
+    out["rel_l2_vs_fp64"] = _rel_l2_gpu(y, y64)
+    split = {}
+    for name in ("f16", "bf16"):
+        sp = numrs_amd.C2CPlan("float32", [n], batch, split=name)
+        sms = _time_plan(sp, x, y, args.steps, args.warmup)
+        split[name] = {"ms": round(sms, 4), "GFLOPs": round(flops / (sms * 1e-3) / 1e9, 1),
+                       "rel_l2_vs_fp64": _rel_l2_gpu(y, y64),
+                       "speedup_vs_plain_fp32": round(ms / sms, 3), "launches": sp.num_launches}
+        del sp
+    out["split_mfma"] = split
+    del x, y, y64
     torch.cuda.empty_cache()
     return out
 
@@ -149,12 +203,17 @@ def cpu_baseline(args) -> dict:
     data = np.empty(2 * n * sample, np.float64)
     for i in range(sample):
         data[2 * n * i:2 * n * (i + 1)] = synth.uniform(3, 2 * n, start=2 * n * i).astype(np.float32)
-    t0 = time.perf_counter()
-    used = oracle.four1_batch_inplace(data, n, sample, 1, threads)
-    dt = time.perf_counter() - t0
-    gflops = 5.0 * n * 20 * sample / dt / 1e9
+    best = None
+    used = 1
+    for _ in range(args.cpu_reps):
+        t0 = time.perf_counter()
+        used = oracle.four1_batch_inplace(data, n, sample, 1, threads)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    gflops = 5.0 * n * 20 * sample / best / 1e9
     return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": used, "kind": "port",
-            "sample": f"{sample} of the 64 transforms (N=2^20, FP64 literal four1), {dt:.2f} s wall"}
+            "sample": f"{sample} transforms of N=2^20 (the whole batch, FP64 literal four1 of FFT_1.rs), "
+                      f"best of {args.cpu_reps} runs: {best:.2f} s wall on {used} threads"}
 
 
 def copy_calibration() -> dict:
@@ -270,7 +329,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-sample", type=int, default=16, help="transforms in the CPU baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=64, help="transforms in the CPU baseline sample")
+    ap.add_argument("--cpu-reps", type=int, default=3, help="repetitions of the CPU sample (best kept)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--only", choices=["headline", "four1", "fourn", "convlv"], default=None,
