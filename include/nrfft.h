@@ -139,6 +139,37 @@ int nrfft_execute_correl(nrfft_plan plan, const void* d_data1, const void* d_dat
 int nrfft_execute_c2c_launches(nrfft_plan plan, const void* d_in, void* d_out, int isign, void* stream, int first,
                                int count);
 
+/* ---- uneven batches -------------------------------------------------------
+ * Many transforms of different lengths and dimensionality in one persistent
+ * work-queue launch (replayed as a captured hipGraph).  Item i has ndim[i]
+ * (1 or 2) and dims dims[2 i] (, dims[2 i + 1]) in C order, powers of two
+ * >= 64 (2-D sides up to 2048 in FP32 / 1024 in FP64, 1-D up to 2^20), and
+ * lives at complex-element offset offs[i] of both d_in and d_out (in place
+ * allowed).  FFTProcessor::fft_batch (FFT_1.rs:185-189) is the reference
+ * counterpart for 1-D items. */
+typedef struct nrfft_uneven_s* nrfft_uneven;
+int nrfft_plan_uneven(nrfft_uneven* plan, int dtype, size_t nitems, const int* ndim, const size_t* dims,
+                      const size_t* offs);
+int nrfft_execute_uneven(nrfft_uneven plan, const void* d_in, void* d_out, int isign, void* stream);
+/* Same work as two plain launches instead of a graph replay. */
+int nrfft_execute_uneven_nograph(nrfft_uneven plan, const void* d_in, void* d_out, int isign, void* stream);
+/* Waits for the stream and reports NRFFT_E_HIP if a dependency wait of the queue timed out. */
+int nrfft_uneven_check(nrfft_uneven plan, void* stream);
+int nrfft_uneven_info(nrfft_uneven plan, size_t* tasks_phase1, size_t* tasks_phase2, size_t* workspace_bytes,
+                      unsigned* groups);
+int nrfft_uneven_destroy(nrfft_uneven plan);
+
+/* FFTProcessor::fft_batch (FFT_1.rs:185-189): `count` host buffers, buffer i
+ * holding 2*nn[i] interleaved values, transformed in place in one launch. */
+int nrfft_fft_batch_f64(double* const* data, const size_t* nn, size_t count, int isign);
+int nrfft_fft_batch_f32(float* const* data, const size_t* nn, size_t count, int isign);
+
+/* Seeded synthetic inputs generated in HBM (bench and test utility): n draws
+ * of uniform[-1,1) from SplitMix64(seed), starting at draw `start`, the same
+ * stream as numrs_amd/synth.py; FP32 values are the FP64 draws rounded. */
+int nrfft_gen_uniform_f32(void* d, size_t n, uint64_t seed, uint64_t start, void* stream);
+int nrfft_gen_uniform_f64(void* d, size_t n, uint64_t seed, uint64_t start, void* stream);
+
 /* ---- autotuner (no counterpart in the reference) -------------------------
  * A C2C plan's tunables -- tune[0] log2 of the four-step column length,
  * tune[1] / tune[2] log2 tile widths of the column / transpose passes (which

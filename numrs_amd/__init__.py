@@ -38,7 +38,9 @@ from .device import (  # noqa: F401
     ConvlvPlan,
     CorrelPlan,
     RealFTPlan,
+    UnevenPlan,
     autotune_c2c,
+    gen_uniform,
     tune_cache_load,
     tune_cache_save,
 )

@@ -72,6 +72,16 @@ _SIGS = {
     "nrfft_autotune_c2c": (_i, [_i, _szp, _sz, _sz, _i, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_double)]),
     "nrfft_tune_cache_save": (_i, [ctypes.c_char_p]),
     "nrfft_tune_cache_load": (_i, [ctypes.c_char_p]),
+    "nrfft_plan_uneven": (_i, [ctypes.POINTER(_vp), _i, _sz, ctypes.POINTER(_i), _szp, _szp]),
+    "nrfft_execute_uneven": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "nrfft_execute_uneven_nograph": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "nrfft_uneven_check": (_i, [_vp, _vp]),
+    "nrfft_uneven_info": (_i, [_vp, _szp, _szp, _szp, ctypes.POINTER(ctypes.c_uint)]),
+    "nrfft_uneven_destroy": (_i, [_vp]),
+    "nrfft_fft_batch_f64": (_i, [ctypes.POINTER(_dp), _szp, _sz, _i]),
+    "nrfft_fft_batch_f32": (_i, [ctypes.POINTER(_fp), _szp, _sz, _i]),
+    "nrfft_gen_uniform_f32": (_i, [_vp, _sz, ctypes.c_uint64, ctypes.c_uint64, _vp]),
+    "nrfft_gen_uniform_f64": (_i, [_vp, _sz, ctypes.c_uint64, ctypes.c_uint64, _vp]),
     "nrfft_plan_destroy": (_i, [_vp]),
     "nrfft_plan_workspace_bytes": (_sz, [_vp]),
     "nrfft_plan_num_launches": (_i, [_vp]),

@@ -143,16 +143,20 @@ class FFTProcessor:
         four1(data, data.size // 2, isign)
 
     def fft_batch(self, batches: Sequence[np.ndarray], isign: int) -> None:
-        groups: dict = {}
-        for i, b in enumerate(batches):
-            groups.setdefault((b.dtype.str, b.size), []).append(i)
-        for (_, size), idx in groups.items():
-            nn = size // 2
-            stack = np.ascontiguousarray(np.stack([batches[i][:2 * nn] for i in idx]))
-            p, sfx = _buf(stack)
-            _check(getattr(_lib.lib(), f"nrfft_four1_batch_{sfx}")(p, nn, len(idx), isign))
-            for k, i in enumerate(idx):
-                batches[i][:2 * nn] = stack[k]
+        """All buffers of one dtype go to the GPU in one uneven-batch launch
+        (nrfft_fft_batch_*), whatever their lengths."""
+        for dt, ctype, sfx in ((np.float64, ctypes.c_double, "f64"), (np.float32, ctypes.c_float, "f32")):
+            idx = [i for i, b in enumerate(batches) if b.dtype == dt]
+            if not idx:
+                continue
+            for i in idx:
+                _inplace_view(batches[i], 2 * (batches[i].size // 2))
+            ptrs = (ctypes.POINTER(ctype) * len(idx))(*[batches[i].ctypes.data_as(ctypes.POINTER(ctype)) for i in idx])
+            nns = (ctypes.c_size_t * len(idx))(*[batches[i].size // 2 for i in idx])
+            _check(getattr(_lib.lib(), f"nrfft_fft_batch_{sfx}")(ptrs, nns, len(idx), isign))
+        other = [b for b in batches if b.dtype not in (np.float32, np.float64)]
+        if other:
+            raise TypeError("fft_batch buffers must be float32 or float64")
 
 
 def real_to_complex(real_data) -> np.ndarray:

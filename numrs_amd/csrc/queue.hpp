@@ -1,0 +1,169 @@
+// queue.hpp -- uneven batches: one persistent work-queue kernel for many
+// transforms of different lengths and dimensionality.
+//
+// The host turns every item into tiles (tasks) of one Stockham pass each:
+//   1-D, N <= QPTS            one row tile (the whole transform)
+//   1-D, N  > QPTS            four-step: N1 column tiles with the W_N twiddle
+//                             (in -> workspace), then N2 row tiles stored
+//                             transposed (workspace -> out)
+//   2-D, n0 x n1              row tiles along n1 (in -> out), then column
+//                             tiles along n0 (in place in out)
+// Phase-1 tasks (first passes and single-pass items) are ordered largest
+// first; phase-2 tasks (second passes) follow in the same item order.  Groups
+// take tasks from one atomic counter (dynamic load balance).  A phase-2 task
+// waits for its own item's phase-1 tasks through a per-item completion
+// counter: every task it waits for was taken from the counter before it, by a
+// group that is running it, so the wait needs no co-resident grid and cannot
+// deadlock; it is still bounded (a flag reports a timeout instead of hanging).
+// Device-scope release/acquire fences around the counters make the phase-1
+// results written on one XCD visible to readers on another.
+#pragma once
+#include "pass_c2c.hpp"
+
+namespace nrfft {
+
+// Tile points (TI x L) of the queue kernel: EPT = 16 values per thread.
+template <typename T>
+constexpr int qpts() {
+    return sizeof(T) == 4 ? 8192 : 4096;
+}
+constexpr int Q_MIN_LOGL = 6;
+constexpr int Q_MODES = 3;  // 0: row (JJ), 1: column (TT), 2: row load / column store (JT)
+
+template <typename T>
+constexpr int q_nclasses() {
+    return (ilog2c(qpts<T>()) - Q_MIN_LOGL + 1) * Q_MODES;
+}
+
+template <typename T>
+constexpr int q_lds_elems() {
+    return qpts<T>() + qpts<T>() / 16 + 1;
+}
+
+struct QTask {
+    int32_t cls;        // (log2 L - Q_MIN_LOGL) * Q_MODES + mode
+    int32_t nvalid;     // transforms of the tile that exist
+    int32_t src, dst;   // 0 = in, 1 = out, 2 = workspace
+    int64_t ioff, ooff; // element offsets of the tile base
+    int32_t si0, esi, so0, eso;
+    int32_t tw_logn;    // four-step twiddle W_N^{k (tw_n2base + t)}, N = 2^tw_logn (0 = none)
+    int32_t tw_n2base;
+    int32_t dep;        // phase-2 task: item whose phase-1 tasks must be done (-1: none)
+    int32_t need;       // ... their count
+    int32_t sig;        // phase-1 task: item whose completion counter it bumps (-1: none)
+};
+
+struct QState {
+    uint32_t next;   // task counter
+    uint32_t error;  // 1 = a dependency wait timed out
+    uint32_t pad[2];
+    // followed by one completion counter per item
+};
+
+template <typename T, int S, int C>
+__device__ __forceinline__ void q_run_class(const QTask& q, cpx<T>* const* bufs, cpx<T>* lds, const Twiddles<T>& tw,
+                                            int tid) {
+    constexpr int LOGL = C / Q_MODES + Q_MIN_LOGL;
+    constexpr int MODE = C % Q_MODES;
+    constexpr int L = 1 << LOGL;
+    if constexpr (L > qpts<T>()) {
+        return;
+    } else {
+        constexpr int TI = qpts<T>() / L;
+        constexpr int NT = qpts<T>() / 16;
+        constexpr int ML = MODE == 1 ? MAP_T : MAP_J;
+        constexpr int MS = MODE == 0 ? MAP_J : MAP_T;
+        using E = Engine<T, S, L, TI, NT, ML, MS>;
+        static_assert(E::LDS_ELEMS <= q_lds_elems<T>(), "queue LDS too small for a tile class");
+        static_assert(TI * L == qpts<T>() && E::EPT == 16, "queue tile shape");
+        const cpx<T>* ib = bufs[q.src] + q.ioff;
+        cpx<T>* ob = bufs[q.dst] + q.ooff;
+        const int nvalid = q.nvalid, si0 = q.si0, esi = q.esi, so0 = q.so0, eso = q.eso;
+        cpx<T> v[E::EPT];
+        E::load0(v, tid, [&](int t, int i) -> cpx<T> {
+            if (t < nvalid) return ib[t * si0 + i * esi];
+            return cpx<T>{(T)0, (T)0};
+        });
+        E::template compute<0>(v, tid, tw.stage);
+        E::template middle<1>(v, tid, lds, tw.stage, false);
+        if (q.tw_logn) {
+            const int logn = q.tw_logn;
+            const uint32_t mask = (1u << logn) - 1u;
+            E::store_last(v, tid, [&](int t, int j, const cpx<T>* w) {
+                if (t < nvalid) {
+                    const uint32_t n2 = uint32_t(q.tw_n2base + t);
+                    cpx<T> f[E::RLAST];
+                    twiddle_run<E::RLAST, S>(tw, (uint32_t(j) * n2) & mask, (uint32_t(E::NBLAST) * n2) & mask, logn, f);
+#pragma unroll
+                    for (int r = 0; r < E::RLAST; ++r) ob[t * so0 + (j + r * E::NBLAST) * eso] = cmul(w[r], f[r]);
+                }
+            });
+        } else {
+            E::store_last(v, tid, [&](int t, int j, const cpx<T>* w) {
+                if (t < nvalid) {
+#pragma unroll
+                    for (int r = 0; r < E::RLAST; ++r) ob[t * so0 + (j + r * E::NBLAST) * eso] = w[r];
+                }
+            });
+        }
+    }
+}
+
+template <typename T, int S, int... Cs>
+__device__ __forceinline__ void q_dispatch(int cls, const QTask& q, cpx<T>* const* bufs, cpx<T>* lds,
+                                           const Twiddles<T>& tw, int tid, std::integer_sequence<int, Cs...>) {
+    ((cls == Cs ? q_run_class<T, S, Cs>(q, bufs, lds, tw, tid) : void()), ...);
+}
+
+// Reset of the queue counters (a fill kernel, so the sequence can live in a
+// captured graph).
+__global__ void k_queue_reset(QState* st, uint32_t* done, int nitems) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        st->next = 0;
+        st->error = 0;
+    }
+    for (int k = i; k < nitems; k += gridDim.x * blockDim.x) done[k] = 0;
+}
+
+template <typename T, int S>
+__global__ __launch_bounds__(qpts<T>() / 16) void k_queue(cpx<T>* in, cpx<T>* out, cpx<T>* ws, const QTask* tasks,
+                                                            int ntasks, QState* st, uint32_t* done, Twiddles<T> tw,
+                                                            uint32_t spin_limit) {
+    __shared__ cpx<T> lds[q_lds_elems<T>()];
+    __shared__ int sh_task;
+    cpx<T>* bufs[3] = {in, out, ws};
+    const int tid = threadIdx.x;
+    for (;;) {
+        __syncthreads();  // previous task's LDS readers are done; sh_task free
+        if (tid == 0) sh_task = (int)atomicAdd(&st->next, 1u);
+        __syncthreads();
+        const int k = sh_task;
+        if (k >= ntasks) break;
+        const QTask q = tasks[k];
+        if (q.dep >= 0) {
+            if (tid == 0) {
+                uint32_t spins = 0;
+                while (__hip_atomic_load(&done[q.dep], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
+                       (uint32_t)q.need) {
+                    if (++spins > spin_limit) {
+                        atomicOr(&st->error, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(4);
+                }
+            }
+            __syncthreads();
+            __threadfence();
+        }
+        q_dispatch<T, S>(q.cls, q, bufs, lds, tw, fresh_tid(tid), std::make_integer_sequence<int, q_nclasses<T>()>{});
+        if (q.sig >= 0) {
+            // release this group's stores device-wide, then count the task done
+            __threadfence();
+            __syncthreads();
+            if (tid == 0) __hip_atomic_fetch_add(&done[q.sig], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+}  // namespace nrfft

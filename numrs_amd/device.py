@@ -5,6 +5,8 @@ from __future__ import annotations
 import ctypes
 from typing import Sequence
 
+import numpy as np
+
 from . import _lib
 from .api import NrfftError
 
@@ -159,3 +161,58 @@ class CorrelPlan(_Plan):
 
     def execute(self, d1, d2, ans, stream=None) -> None:
         _check(_lib.lib().nrfft_execute_correl(self._h, _ptr(d1), _ptr(d2), _ptr(ans), _stream(stream)))
+
+
+class UnevenPlan:
+    """Uneven batch: items of different lengths / dimensionality in one
+    persistent work-queue launch (replayed as a captured hipGraph).
+    ``items`` is a list of shapes, (N,) or (n0, n1); ``offsets`` are the items'
+    complex-element offsets in the buffers (default: packed back to back)."""
+
+    def __init__(self, dtype, items, offsets=None):
+        self._h = ctypes.c_void_p()
+        items = [tuple(int(v) for v in it) for it in items]
+        n = len(items)
+        nd = (ctypes.c_int * n)(*[len(it) for it in items])
+        dims = (ctypes.c_size_t * (2 * n))(*[v for it in items for v in (it[0], it[1] if len(it) > 1 else 1)])
+        if offsets is None:
+            offsets, acc = [], 0
+            for it in items:
+                offsets.append(acc)
+                acc += int(np.prod(it))
+        self.offsets = [int(o) for o in offsets]
+        self.items = items
+        offs = (ctypes.c_size_t * n)(*self.offsets)
+        _check(_lib.lib().nrfft_plan_uneven(ctypes.byref(self._h), _dtype_code(dtype), n, nd, dims, offs))
+        self.total = max(o + int(np.prod(it)) for o, it in zip(self.offsets, items))
+
+    def __del__(self):
+        try:
+            if self._h.value:
+                _lib.lib().nrfft_uneven_destroy(self._h)
+                self._h = ctypes.c_void_p()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+    def execute(self, x_in, x_out, isign: int, stream=None, graph: bool = True) -> None:
+        fn = _lib.lib().nrfft_execute_uneven if graph else _lib.lib().nrfft_execute_uneven_nograph
+        _check(fn(self._h, _ptr(x_in), _ptr(x_out), isign, _stream(stream)))
+
+    def check(self, stream=None) -> None:
+        """Wait for the stream; raise if the queue's grid barrier timed out."""
+        _check(_lib.lib().nrfft_uneven_check(self._h, _stream(stream)))
+
+    def info(self) -> dict:
+        a, b, w = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
+        g = ctypes.c_uint()
+        _check(_lib.lib().nrfft_uneven_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(w),
+                                            ctypes.byref(g)))
+        return {"tasks_phase1": a.value, "tasks_phase2": b.value, "workspace_bytes": w.value, "groups": g.value}
+
+
+def gen_uniform(x, seed: int, start: int = 0, stream=None) -> None:
+    """Fill tensor x (float32/float64, contiguous, on the GPU) with the seeded
+    uniform[-1,1) stream of numrs_amd.synth, generated in HBM."""
+    n = x.numel()
+    fn = _lib.lib().nrfft_gen_uniform_f32 if str(x.dtype) == "torch.float32" else _lib.lib().nrfft_gen_uniform_f64
+    _check(fn(_ptr(x), n, seed, start, _stream(stream)))
