@@ -13,7 +13,8 @@ The same line carries
                  transforms like FFTProcessor::fft_batch) timed on the host
                  cores on a bounded sample of the same workload;
   extra:         the other configurations (four1 2^16 FP64, fourn 256^3,
-                 batched convlv 4096 x 2^14), each with its own GB/s figure.
+                 batched convlv 4096 x 2^14, the uneven batch of 10,000
+                 mixed 1-D/2-D transforms), each with its own GB/s figure.
 """
 from __future__ import annotations
 
@@ -244,6 +245,8 @@ def extra_configs(args, only: str = "all") -> dict:
         res.update(_extra_fourn())
     if only in ("all", "convlv"):
         res.update(_extra_convlv())
+    if only in ("all", "uneven"):
+        res.update(_extra_uneven())
     return res
 
 
@@ -324,6 +327,77 @@ def _extra_convlv() -> dict:
     return res
 
 
+def _segment_sums(v, lengths):
+    """Sums of consecutive segments of v (lengths in elements), on the GPU,
+    from differences of an FP64 prefix sum."""
+    import torch
+
+    c = torch.cumsum(v, 0)
+    ends = torch.as_tensor(np.cumsum(lengths) - 1, device=v.device)
+    tot = c[ends]
+    return torch.cat([tot[:1], tot[1:] - tot[:-1]])
+
+
+def _extra_uneven() -> dict:
+    import torch
+
+    import numrs_amd
+    from numrs_amd import synth
+
+    res = {}
+    # 10,000 transforms, 80% 1-D (log2 N on 8..20), 20% 2-D (sides 2^6..2^11),
+    # values uniform[-1,1) from seed 4, FP32, one graph-replayed queue launch.
+    items = synth.uneven_shapes(4, 10000)
+    plan = numrs_amd.UnevenPlan("float32", items)
+    total = plan.total
+    x = torch.empty(2 * total, dtype=torch.float32, device="cuda")
+    numrs_amd.gen_uniform(x, 4)
+    y = torch.empty_like(x)
+    for _ in range(2):
+        plan.execute(x, y, 1)
+    plan.check()
+    ms = time_events(lambda: plan.execute(x, y, 1), 5)
+    plan.check()
+    flops = sum(5.0 * int(np.prod(it)) * math.log2(int(np.prod(it))) for it in items)
+    byt = 2.0 * total * 8
+    info = plan.info()
+    del plan
+    # accuracy: the FP64 queue path on the same (FP32-rounded) inputs
+    y64 = x.double()
+    del x
+    p64 = numrs_amd.UnevenPlan("float64", items)
+    p64.execute(y64, y64, 1)
+    p64.check()
+    del p64
+    worst, acc_e, acc_r = 0.0, 0.0, 0.0
+    lens = [2 * int(np.prod(it)) for it in items]
+    pos, i0 = 0, 0
+    while i0 < len(items):
+        i1, span = i0, 0
+        while i1 < len(items) and (span == 0 or span + lens[i1] <= (1 << 28)):
+            span += lens[i1]
+            i1 += 1
+        d = y[pos:pos + span].double() - y64[pos:pos + span]
+        e = _segment_sums(d * d, lens[i0:i1])
+        r = _segment_sums(y64[pos:pos + span] ** 2, lens[i0:i1])
+        worst = max(worst, float(torch.sqrt(e / r).max()))
+        acc_e += float(e.sum())
+        acc_r += float(r.sum())
+        pos += span
+        i0 = i1
+    del y, y64
+    torch.cuda.empty_cache()
+    n1d = sum(1 for it in items if len(it) == 1)
+    res["uneven_f32_10000"] = {
+        "items": len(items), "items_1d": n1d, "items_2d": len(items) - n1d, "points": total,
+        "ms": round(ms, 4), "GFLOPs": round(flops / (ms * 1e-3) / 1e9, 1),
+        "GBps": round(byt / (ms * 1e-3) / 1e9, 1), "frac_hbm": round(byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "max_rel_l2_vs_fp64": worst, "rel_l2_all_vs_fp64": math.sqrt(acc_e / acc_r),
+        "tasks": [info["tasks_phase1"], info["tasks_phase2"]], "groups": info["groups"],
+        "launch": "graph replay: counter-reset fill kernel + the persistent queue kernel, one dispatch per pass phase"}
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -333,7 +407,7 @@ def main() -> int:
     ap.add_argument("--cpu-reps", type=int, default=3, help="repetitions of the CPU sample (best kept)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--only", choices=["headline", "four1", "fourn", "convlv"], default=None,
+    ap.add_argument("--only", choices=["headline", "four1", "fourn", "convlv", "uneven"], default=None,
                     help="run one workload only (profiling runs); the JSON line then holds just that workload")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 512 MiB copy kernel (counter calibration for the traffic figure)")

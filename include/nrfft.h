@@ -141,19 +141,20 @@ int nrfft_execute_c2c_launches(nrfft_plan plan, const void* d_in, void* d_out, i
 
 /* ---- uneven batches -------------------------------------------------------
  * Many transforms of different lengths and dimensionality in one persistent
- * work-queue launch (replayed as a captured hipGraph).  Item i has ndim[i]
- * (1 or 2) and dims dims[2 i] (, dims[2 i + 1]) in C order, powers of two
- * >= 64 (2-D sides up to 2048 in FP32 / 1024 in FP64, 1-D up to 2^20), and
- * lives at complex-element offset offs[i] of both d_in and d_out (in place
- * allowed).  FFTProcessor::fft_batch (FFT_1.rs:185-189) is the reference
- * counterpart for 1-D items. */
+ * work-queue kernel (two dispatches, one per pass phase, replayed as a
+ * captured hipGraph).  Item i has ndim[i] (1 or 2) and dims dims[2 i]
+ * (, dims[2 i + 1]) in C order, powers of two >= 64 (2-D sides up to 4096 in
+ * FP32 / 2048 in FP64; 1-D up to 2^22 in FP32 / 2^20 in FP64), and lives at
+ * complex-element offset offs[i] of both d_in and d_out (in place allowed).
+ * FFTProcessor::fft_batch (FFT_1.rs:185-189) is the reference counterpart
+ * for 1-D items. */
 typedef struct nrfft_uneven_s* nrfft_uneven;
 int nrfft_plan_uneven(nrfft_uneven* plan, int dtype, size_t nitems, const int* ndim, const size_t* dims,
                       const size_t* offs);
 int nrfft_execute_uneven(nrfft_uneven plan, const void* d_in, void* d_out, int isign, void* stream);
-/* Same work as two plain launches instead of a graph replay. */
+/* Same work as plain launches (reset, phase 1, phase 2) instead of a graph replay. */
 int nrfft_execute_uneven_nograph(nrfft_uneven plan, const void* d_in, void* d_out, int isign, void* stream);
-/* Waits for the stream and reports NRFFT_E_HIP if a dependency wait of the queue timed out. */
+/* Waits for the stream and reports NRFFT_E_HIP if the last execution left tasks untaken. */
 int nrfft_uneven_check(nrfft_uneven plan, void* stream);
 int nrfft_uneven_info(nrfft_uneven plan, size_t* tasks_phase1, size_t* tasks_phase2, size_t* workspace_bytes,
                       unsigned* groups);

@@ -9,14 +9,14 @@
 //   2-D, n0 x n1              row tiles along n1 (in -> out), then column
 //                             tiles along n0 (in place in out)
 // Phase-1 tasks (first passes and single-pass items) are ordered largest
-// first; phase-2 tasks (second passes) follow in the same item order.  Groups
-// take tasks from one atomic counter (dynamic load balance).  A phase-2 task
-// waits for its own item's phase-1 tasks through a per-item completion
-// counter: every task it waits for was taken from the counter before it, by a
-// group that is running it, so the wait needs no co-resident grid and cannot
-// deadlock; it is still bounded (a flag reports a timeout instead of hanging).
-// Device-scope release/acquire fences around the counters make the phase-1
-// results written on one XCD visible to readers on another.
+// first; phase-2 tasks (second passes) follow in the same item order.  Each
+// phase is one dispatch of the persistent kernel; its groups take tasks from
+// an atomic counter (dynamic load balance).  The two dispatches are the only
+// synchronisation: the kernel boundary makes phase-1 results written on one
+// XCD's L2 visible to phase-2 readers on another.  (Per-task release/acquire
+// fences inside one dispatch write back / invalidate a whole L2 each on this
+// chip -- hundreds of thousands of them cost far more than a kernel
+// boundary.)
 #pragma once
 #include "pass_c2c.hpp"
 
@@ -48,16 +48,10 @@ struct QTask {
     int32_t si0, esi, so0, eso;
     int32_t tw_logn;    // four-step twiddle W_N^{k (tw_n2base + t)}, N = 2^tw_logn (0 = none)
     int32_t tw_n2base;
-    int32_t dep;        // phase-2 task: item whose phase-1 tasks must be done (-1: none)
-    int32_t need;       // ... their count
-    int32_t sig;        // phase-1 task: item whose completion counter it bumps (-1: none)
 };
 
 struct QState {
-    uint32_t next;   // task counter
-    uint32_t error;  // 1 = a dependency wait timed out
-    uint32_t pad[2];
-    // followed by one completion counter per item
+    uint32_t next[2];  // task counters of the two phases
 };
 
 template <typename T, int S, int C>
@@ -117,52 +111,27 @@ __device__ __forceinline__ void q_dispatch(int cls, const QTask& q, cpx<T>* cons
 
 // Reset of the queue counters (a fill kernel, so the sequence can live in a
 // captured graph).
-__global__ void k_queue_reset(QState* st, uint32_t* done, int nitems) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        st->next = 0;
-        st->error = 0;
-    }
-    for (int k = i; k < nitems; k += gridDim.x * blockDim.x) done[k] = 0;
+__global__ void k_queue_reset(QState* st) {
+    if (threadIdx.x < 2) st->next[threadIdx.x] = 0;
 }
 
+// One phase: tasks [base, base + count) of the task list.
 template <typename T, int S>
 __global__ __launch_bounds__(qpts<T>() / 16) void k_queue(cpx<T>* in, cpx<T>* out, cpx<T>* ws, const QTask* tasks,
-                                                            int ntasks, QState* st, uint32_t* done, Twiddles<T> tw,
-                                                            uint32_t spin_limit) {
+                                                            int base, int count, int phase, QState* st,
+                                                            Twiddles<T> tw) {
     __shared__ cpx<T> lds[q_lds_elems<T>()];
     __shared__ int sh_task;
     cpx<T>* bufs[3] = {in, out, ws};
     const int tid = threadIdx.x;
     for (;;) {
         __syncthreads();  // previous task's LDS readers are done; sh_task free
-        if (tid == 0) sh_task = (int)atomicAdd(&st->next, 1u);
+        if (tid == 0) sh_task = (int)atomicAdd(&st->next[phase], 1u);
         __syncthreads();
         const int k = sh_task;
-        if (k >= ntasks) break;
-        const QTask q = tasks[k];
-        if (q.dep >= 0) {
-            if (tid == 0) {
-                uint32_t spins = 0;
-                while (__hip_atomic_load(&done[q.dep], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
-                       (uint32_t)q.need) {
-                    if (++spins > spin_limit) {
-                        atomicOr(&st->error, 1u);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(4);
-                }
-            }
-            __syncthreads();
-            __threadfence();
-        }
+        if (k >= count) break;
+        const QTask q = tasks[base + k];
         q_dispatch<T, S>(q.cls, q, bufs, lds, tw, fresh_tid(tid), std::make_integer_sequence<int, q_nclasses<T>()>{});
-        if (q.sig >= 0) {
-            // release this group's stores device-wide, then count the task done
-            __threadfence();
-            __syncthreads();
-            if (tid == 0) __hip_atomic_fetch_add(&done[q.sig], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
     }
 }
 

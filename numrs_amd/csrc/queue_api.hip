@@ -19,10 +19,6 @@ namespace nrfft {
         if (e_ != hipSuccess) return fail(NRFFT_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-// Bounded dependency wait (iterations of an atomic load and a 4x64-cycle
-// sleep; a few seconds -- far longer than any pass takes).
-constexpr uint32_t Q_SPIN_LIMIT = 1u << 22;
-
 struct UnevenItem {
     int ndim;
     size_t n0, n1;  // ndim == 1: n0 = N, n1 = 1
@@ -37,7 +33,7 @@ struct UnevenState {
     size_t ws_elems = 0;
     size_t total_elems = 0;
     QTask* d_tasks = nullptr;
-    QState* d_state = nullptr;  // followed by items.size() completion counters
+    QState* d_state = nullptr;
     void* d_ws = nullptr;
     unsigned grid[2] = {0, 0};  // per direction (different kernels)
     // graph cache
@@ -86,9 +82,6 @@ static void add_pass(std::vector<QTask>& out, int dtype, int logl, int mode, int
         q.eso = oeso;
         q.tw_logn = tw_logn;
         q.tw_n2base = (int32_t)b0;
-        q.dep = -1;
-        q.need = 0;
-        q.sig = -1;
         out.push_back(q);
     }
 }
@@ -97,8 +90,7 @@ static int build_uneven(UnevenState& u) {
     const int qlog = ilog2(u.dtype == NRFFT_F32 ? qpts<float>() : qpts<double>());
     std::vector<std::pair<int64_t, std::vector<QTask>>> p1, p2;  // (work, tasks) per item
     size_t ws = 0;
-    for (size_t idx = 0; idx < u.items.size(); ++idx) {
-        const UnevenItem& it = u.items[idx];
+    for (const UnevenItem& it : u.items) {
         std::vector<QTask> a, b;
         if (it.ndim == 1) {
             const int logn = ilog2(it.n0);
@@ -128,13 +120,6 @@ static int build_uneven(UnevenState& u) {
             add_pass(b, u.dtype, l0, 1, (int64_t)it.n1, B_OUT, B_OUT, it.off, 1, (int32_t)it.n1, it.off, 1,
                      (int32_t)it.n1, 0);
         }
-        if (!b.empty()) {
-            for (QTask& q : a) q.sig = (int32_t)idx;
-            for (QTask& q : b) {
-                q.dep = (int32_t)idx;
-                q.need = (int32_t)a.size();
-            }
-        }
         const int64_t work = int64_t(it.n0 * it.n1);
         p1.emplace_back(work, std::move(a));
         if (!b.empty()) p2.emplace_back(work, std::move(b));
@@ -159,7 +144,7 @@ static int uneven_setup_device(UnevenState& u) {
     HIPCHK(hipMalloc(&u.d_tasks, std::max<size_t>(1, u.tasks.size()) * sizeof(QTask)));
     if (!u.tasks.empty())
         HIPCHK(hipMemcpy(u.d_tasks, u.tasks.data(), u.tasks.size() * sizeof(QTask), hipMemcpyHostToDevice));
-    HIPCHK(hipMalloc(&u.d_state, sizeof(QState) + u.items.size() * sizeof(uint32_t)));
+    HIPCHK(hipMalloc(&u.d_state, sizeof(QState)));
     if (u.ws_elems) {
         const size_t bytes = u.ws_elems * csize(u.dtype);
         if (hipMalloc(&u.d_ws, bytes) != hipSuccess) return fail(NRFFT_E_NOMEM, "uneven workspace");
@@ -187,18 +172,18 @@ static int uneven_enqueue(UnevenState& u, const void* in, void* out, int isign, 
     cpx<T>* pout = static_cast<cpx<T>*>(out);
     cpx<T>* pws = static_cast<cpx<T>*>(u.d_ws);
     const QTask* tk = u.d_tasks;
-    int ntasks = u.n1 + u.n2;
     QState* st = u.d_state;
-    uint32_t* done = reinterpret_cast<uint32_t*>(st + 1);
-    const int nitems = (int)u.items.size();
-    uint32_t spin = Q_SPIN_LIMIT;
     const int d = isign > 0 ? 0 : 1;
-    const unsigned rb = (unsigned)std::min(1024, (nitems + 255) / 256 + 1);
-    hipLaunchKernelGGL(k_queue_reset, dim3(rb), dim3(256), 0, s, st, done, nitems);
+    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, s, st);
     HIPCHK(hipGetLastError());
-    void* args[] = {&pin, &pout, &pws, &tk, &ntasks, &st, &done, &tw, &spin};
     const int nt = (int)(qpts<T>() / 16);
-    HIPCHK(hipLaunchKernel(queue_kernel<T>(isign), dim3(u.grid[d]), dim3(nt), args, 0, s));
+    for (int phase = 0; phase < 2; ++phase) {
+        int base = phase ? u.n1 : 0, count = phase ? u.n2 : u.n1, ph = phase;
+        if (count == 0) continue;
+        const unsigned grid = std::min<unsigned>(u.grid[d], (unsigned)count);
+        void* args[] = {&pin, &pout, &pws, &tk, &base, &count, &ph, &st, &tw};
+        HIPCHK(hipLaunchKernel(queue_kernel<T>(isign), dim3(grid), dim3(nt), args, 0, s));
+    }
     return NRFFT_OK;
 }
 
@@ -356,7 +341,9 @@ int nrfft_uneven_check(nrfft_uneven plan, void* stream) {
     QState st{};
     HIPCHK(hipMemcpyAsync(&st, plan->u.d_state, sizeof st, hipMemcpyDeviceToHost, (hipStream_t)stream));
     HIPCHK(hipStreamSynchronize((hipStream_t)stream));
-    if (st.error) return fail(NRFFT_E_HIP, "uneven queue: dependency wait timed out");
+    // every group leaves after drawing one index past the end of its phase
+    if (st.next[0] < (uint32_t)plan->u.n1 || st.next[1] < (uint32_t)plan->u.n2)
+        return fail(NRFFT_E_HIP, "uneven queue: a phase did not drain its task list");
     return NRFFT_OK;
 }
 

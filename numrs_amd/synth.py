@@ -61,3 +61,16 @@ def gaussian_windowed_responses(seed: int, count: int, m: int, sigma: float, sta
     r = g * w
     r /= np.abs(r).sum(axis=1, keepdims=True)
     return r
+
+
+def uneven_shapes(seed: int, count: int, frac_1d: float = 0.8, log_1d=(8, 20), log_2d=(6, 11)) -> list:
+    """Shapes of an uneven batch: each item is 1-D with probability frac_1d,
+    log2 N uniform on log_1d (inclusive), else 2-D with each side's log2
+    uniform on log_2d.  Drawn with numpy's PCG64 generator from ``seed``
+    (shape draws only; values come from the SplitMix64 stream)."""
+    rng = np.random.default_rng(seed)
+    is1d = rng.random(count) < frac_1d
+    l1 = rng.integers(log_1d[0], log_1d[1] + 1, count)
+    a = rng.integers(log_2d[0], log_2d[1] + 1, count)
+    b = rng.integers(log_2d[0], log_2d[1] + 1, count)
+    return [(1 << int(l1[i]),) if is1d[i] else (1 << int(a[i]), 1 << int(b[i])) for i in range(count)]

@@ -116,3 +116,23 @@ def test_gen_uniform_matches_synth():
         if dt == torch.float32:
             ref = ref.astype(np.float32)
         assert np.array_equal(x.cpu().numpy(), ref)
+
+
+def test_uneven_full_size_range():
+    # The size classes of the 10,000-item benchmark batch (1-D 2^8..2^20,
+    # 2-D sides 2^6..2^11), 40 items, FP32 against the FP64 oracle.
+    import torch
+
+    items = synth.uneven_shapes(4, 40)
+    assert max(int(np.prod(it)) for it in items if len(it) == 1) >= 1 << 17
+    plan = numrs_amd.UnevenPlan("float32", items)
+    x = torch.empty(2 * plan.total, dtype=torch.float32, device="cuda")
+    numrs_amd.gen_uniform(x, 4)
+    y = torch.empty_like(x)
+    plan.execute(x, y, 1)
+    plan.check()
+    xh, yh = x.cpu().numpy(), y.cpu().numpy()
+    for it, off in zip(items, plan.offsets):
+        n = int(np.prod(it))
+        seg = slice(2 * off, 2 * (off + n))
+        assert rel_l2(yh[seg], _ref(xh[seg].astype(np.float64), it, 1)) < 2e-6, it
