@@ -143,8 +143,8 @@ int nrfft_execute_c2c_launches(nrfft_plan plan, const void* d_in, void* d_out, i
  * Many transforms of different lengths and dimensionality in one persistent
  * work-queue kernel (two dispatches, one per pass phase, replayed as a
  * captured hipGraph).  Item i has ndim[i] (1 or 2) and dims dims[2 i]
- * (, dims[2 i + 1]) in C order, powers of two >= 64 (2-D sides up to 4096 in
- * FP32 / 2048 in FP64; 1-D up to 2^22 in FP32 / 2^20 in FP64), and lives at
+ * (, dims[2 i + 1]) in C order, powers of two >= 64 (2-D sides up to 8192 in
+ * FP32 / 4096 in FP64; 1-D up to 2^24 in FP32 / 2^22 in FP64), and lives at
  * complex-element offset offs[i] of both d_in and d_out (in place allowed).
  * FFTProcessor::fft_batch (FFT_1.rs:185-189) is the reference counterpart
  * for 1-D items. */

@@ -268,6 +268,7 @@ template <typename T, int S, int L, int TI, int NT, int MAPL, int MAPS>
 struct Engine {
     using P = RadixPlan<L>;
     static constexpr int EPT = TI * L / NT;
+    static_assert(L <= (1 << TW_STAGE_MAX_LOGL), "per-stage twiddle tables cover L <= 2^13 only");
     static_assert(TI * L % NT == 0, "tile must split evenly over threads");
     static constexpr int NST = P::NST;
     static_assert(EPT % P::EPT_MIN == 0 || L < 16, "EPT must cover the largest radix");

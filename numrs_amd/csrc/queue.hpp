@@ -22,17 +22,30 @@
 
 namespace nrfft {
 
-// Tile points (TI x L) of the queue kernel: EPT = 16 values per thread.
+// Tile points (TI x L) of the queue kernel: EPT = 16 values per thread, so a
+// group has qpts / 16 threads.  Build switches: NRFFT_QPTS_F32 / _F64.
+#ifndef NRFFT_QPTS_F32
+#define NRFFT_QPTS_F32 16384
+#endif
+#ifndef NRFFT_QPTS_F64
+#define NRFFT_QPTS_F64 8192
+#endif
 template <typename T>
 constexpr int qpts() {
-    return sizeof(T) == 4 ? 8192 : 4096;
+    return sizeof(T) == 4 ? NRFFT_QPTS_F32 : NRFFT_QPTS_F64;
 }
 constexpr int Q_MIN_LOGL = 6;
+// Longest pass of a tile: the per-stage twiddle tables' limit, and at most
+// the tile itself.
+template <typename T>
+constexpr int q_max_logl() {
+    return ilog2c(qpts<T>()) < TW_STAGE_MAX_LOGL ? ilog2c(qpts<T>()) : TW_STAGE_MAX_LOGL;
+}
 constexpr int Q_MODES = 3;  // 0: row (JJ), 1: column (TT), 2: row load / column store (JT)
 
 template <typename T>
 constexpr int q_nclasses() {
-    return (ilog2c(qpts<T>()) - Q_MIN_LOGL + 1) * Q_MODES;
+    return (q_max_logl<T>() - Q_MIN_LOGL + 1) * Q_MODES;
 }
 
 template <typename T>

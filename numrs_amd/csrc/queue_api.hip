@@ -87,21 +87,25 @@ static void add_pass(std::vector<QTask>& out, int dtype, int logl, int mode, int
 }
 
 static int build_uneven(UnevenState& u) {
-    const int qlog = ilog2(u.dtype == NRFFT_F32 ? qpts<float>() : qpts<double>());
+    const bool f32 = u.dtype == NRFFT_F32;
+    const int qlog = ilog2(f32 ? qpts<float>() : qpts<double>());
+    const int maxl = f32 ? q_max_logl<float>() : q_max_logl<double>();
+    // a column tile (TT / JT) holds at least 4 transforms
+    const int maxl_col = std::min(maxl, qlog - 2);
     std::vector<std::pair<int64_t, std::vector<QTask>>> p1, p2;  // (work, tasks) per item
     size_t ws = 0;
     for (const UnevenItem& it : u.items) {
         std::vector<QTask> a, b;
         if (it.ndim == 1) {
             const int logn = ilog2(it.n0);
-            if (logn <= qlog) {
+            if (logn <= maxl) {
                 if (logn < Q_MIN_LOGL) return fail(NRFFT_E_UNSUPPORTED, "uneven items need N >= 64");
                 add_pass(a, u.dtype, logn, 0, 1, B_IN, B_OUT, it.off, it.n0, 1, it.off, it.n0, 1, 0);
             } else {
                 int l1 = logn / 2;
                 if (l1 < Q_MIN_LOGL) l1 = Q_MIN_LOGL;
                 const int l2 = logn - l1;
-                if (l2 > qlog - 2 || l1 > qlog - 2)
+                if (l2 > maxl_col || l1 > maxl_col)
                     return fail(NRFFT_E_UNSUPPORTED, "uneven 1-D item longer than the queue supports");
                 const int64_t N1 = int64_t(1) << l1, N2 = int64_t(1) << l2;
                 // pass A: columns n2 (stride N2), length N1, twiddle -> workspace [k1][n2]
@@ -113,8 +117,8 @@ static int build_uneven(UnevenState& u) {
             }
         } else {
             const int l0 = ilog2(it.n0), l1 = ilog2(it.n1);
-            if (l0 < Q_MIN_LOGL || l1 < Q_MIN_LOGL || l0 > qlog - 1 || l1 > qlog - 1)
-                return fail(NRFFT_E_UNSUPPORTED, "uneven 2-D sides must be 64..QPTS/2");
+            if (l0 < Q_MIN_LOGL || l1 < Q_MIN_LOGL || l1 > maxl || l0 > std::min(maxl, qlog - 1))
+                return fail(NRFFT_E_UNSUPPORTED, "uneven 2-D sides out of the supported range");
             // rows along n1 (contiguous), then columns along n0 (stride n1) in place in out
             add_pass(a, u.dtype, l1, 0, (int64_t)it.n0, B_IN, B_OUT, it.off, it.n1, 1, it.off, it.n1, 1, 0);
             add_pass(b, u.dtype, l0, 1, (int64_t)it.n1, B_OUT, B_OUT, it.off, 1, (int32_t)it.n1, it.off, 1,

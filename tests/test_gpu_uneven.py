@@ -136,3 +136,33 @@ def test_uneven_full_size_range():
         n = int(np.prod(it))
         seg = slice(2 * off, 2 * (off + n))
         assert rel_l2(yh[seg], _ref(xh[seg].astype(np.float64), it, 1)) < 2e-6, it
+
+
+@pytest.mark.parametrize("dtype,items,tol", [
+    ("float32", [(1 << 24,), (8192, 64), (64, 8192), (1 << 13,), (1 << 14,)], 3e-6),
+    ("float64", [(1 << 22,), (4096, 64), (64, 8192), (1 << 13,), (1 << 14,)], 1e-12),
+])
+def test_uneven_size_limits(dtype, items, tol):
+    # Largest items the queue takes (1-D four-step at the limit, long rows and
+    # long columns, the longest single-pass 1-D length and the shortest
+    # four-step one).
+    import torch
+
+    plan = numrs_amd.UnevenPlan(dtype, items)
+    x = torch.empty(2 * plan.total, dtype=getattr(torch, dtype), device="cuda")
+    numrs_amd.gen_uniform(x, 8)
+    y = torch.empty_like(x)
+    plan.execute(x, y, -1)
+    plan.check()
+    xh, yh = x.cpu().numpy(), y.cpu().numpy()
+    for it, off in zip(items, plan.offsets):
+        n = int(np.prod(it))
+        seg = slice(2 * off, 2 * (off + n))
+        assert rel_l2(yh[seg], _ref(xh[seg].astype(np.float64), it, -1)) < tol, it
+
+
+@pytest.mark.parametrize("dtype,item", [("float32", (1 << 25,)), ("float32", (16384, 64)),
+                                        ("float64", (1 << 23,)), ("float64", (8192, 64)), ("float32", (32,))])
+def test_uneven_rejects_out_of_range(dtype, item):
+    with pytest.raises(numrs_amd.NrfftError):
+        numrs_amd.UnevenPlan(dtype, [item])
