@@ -256,6 +256,7 @@ def extra_configs(args, only: str = "all") -> dict:
     res = {}
     if only in ("all", "four1"):
         res.update(_extra_four1())
+        res.update(_extra_host_call())
     if only in ("all", "fourn"):
         res.update(_extra_fourn())
     if only in ("all", "convlv"):
@@ -285,6 +286,31 @@ def _extra_four1() -> dict:
                              "GBps": round(2 * n * 16 / (ms * 1e-3) / 1e9, 1)}
     del plan, x, y
     return res
+
+
+def _extra_host_call() -> dict:
+    """The headline batch through the host-pointer drop-in call
+    (nrfft_four1_batch_f32: host -> HBM, transform, HBM -> host inside the
+    call), i.e. the PCIe-inclusive rate.  Never the headline value."""
+    from numrs_amd import _lib
+
+    batch, n = 64, 1 << 20
+    src = np.tile(np.linspace(-1.0, 1.0, 2 * n, dtype=np.float32), batch)
+    h = src.copy()
+    p = h.ctypes.data_as(_lib._fp)
+    rc = _lib.lib().nrfft_four1_batch_f32(p, n, batch, 1)
+    if rc:
+        raise RuntimeError(_lib.last_error())
+    times = []
+    for _ in range(3):
+        h[:] = src
+        t0 = time.perf_counter()
+        _lib.lib().nrfft_four1_batch_f32(p, n, batch, 1)
+        times.append(time.perf_counter() - t0)
+    ms = float(np.median(times)) * 1e3
+    return {"four1_batch_host_f32_64x2p20": {
+        "ms": round(ms, 2), "GFLOPs": round(5.0 * n * 20 * batch / (ms * 1e-3) / 1e9, 1),
+        "note": "host-pointer call on pageable numpy memory: 512 MiB copied in and out over PCIe per call"}}
 
 
 def _extra_fourn() -> dict:
