@@ -52,6 +52,10 @@ def lib() -> ctypes.CDLL:
             "orc_correl": (i32, [_dp, _dp, sz, _dp]),
             "orc_correl_direct_circular": (None, [_dp, _dp, sz, _dp]),
             "orc_four1_batch": (i32, [_dp, sz, sz, i32, i32]),
+            "orc_twofft": (None, [_dp, _dp, sz, _dp, _dp]),
+            "orc_cosft1": (None, [_dp, sz]),
+            "orc_cosft2": (None, [_dp, sz, i32]),
+            "orc_rlft3": (i32, [_dp, _dp, sz, sz, sz, i32]),
             "orc_convlv_batch": (i32, [_dp, sz, _dp, sz, sz, i32, _dp, i32]),
         }
         for name, (res, args) in sig.items():
@@ -199,3 +203,35 @@ def four1_batch_inplace(data: np.ndarray, nn: int, batch: int, isign: int, nthre
 def convlv_batch(data: np.ndarray, n: int, respns: np.ndarray, m: int, batch: int, isign: int,
                  out: np.ndarray, nthreads: int = 0) -> int:
     return int(lib().orc_convlv_batch(_p(data), n, _p(respns), m, batch, isign, _p(out), nthreads))
+
+
+def twofft(d1, d2):
+    """FFT_2.rs:3-132 (NR twofft): the isign=+1 spectra (bins 0..n-1,
+    interleaved) of two real sequences from one complex FFT."""
+    a, b = _f64(d1), _f64(d2)
+    n = a.size
+    f1, f2 = np.zeros(2 * n), np.zeros(2 * n)
+    lib().orc_twofft(_p(a), _p(b), n, _p(f1), _p(f2))
+    return f1, f2
+
+
+def cosft1(y):
+    """Cos_FT.rs:7-68 (NR cosft1, DCT-I): y has n + 1 values."""
+    d = _f64(y).copy()
+    lib().orc_cosft1(_p(d), d.size - 1)
+    return d
+
+
+def cosft2(y, isign: int):
+    """Cos_FT2.rs:7-173 (NR cosft2): DCT-II (isign=1) / DCT-III (isign=-1)."""
+    d = _f64(y).copy()
+    lib().orc_cosft2(_p(d), d.size, isign)
+    return d
+
+
+def rlft3(data, speq, nn1: int, nn2: int, nn3: int, isign: int):
+    """Real_FT3.rs:8-141 (NR rlft3): returns (data, speq) after the call."""
+    d = _f64(data).copy()
+    s = _f64(speq).copy()
+    _chk(lib().orc_rlft3(_p(d), _p(s), nn1, nn2, nn3, isign))
+    return d, s

@@ -554,6 +554,216 @@ void orc_correl_direct_circular(const double *d1, const double *d2, size_t n, do
 /* ------------------------------------------------------------------------ */
 
 /* Returns the number of OpenMP threads used. */
+/* ------------------------------------------------------------------------
+ * twofft: FFT_2.rs:3-17 (pack), 52-132 (separation) -- NR twofft.
+ * NR: the two real inputs packed as z_j = d1_j + i d2_j, one n-point four1
+ * (isign +1), then F1[k] = (Z_k + conj Z_{n-k})/2, F2[k] = (Z_k - conj Z_{n-k})/(2i).
+ * Divergences of the Rust source corrected here: it runs four1 over a
+ * 2n+2-long buffer (four1's loop bound is data.len(), FFT_1.rs:52-56) and
+ * pairs bin k with index 2n+2-2k (NR's 1-based mirror, FFT_2.rs:76-78), which
+ * is bin n+1-k in 0-based storage.  fft1 / fft2 receive bins 0..n-1 (2n
+ * values each).
+ * ------------------------------------------------------------------------ */
+void orc_twofft(const double *d1, const double *d2, size_t n, double *fft1, double *fft2) {
+    for (size_t j = 0; j < n; ++j) {
+        fft1[2 * j] = d1[j];
+        fft1[2 * j + 1] = d2[j];
+    }
+    orc_four1_nr(fft1, n, 1);
+    fft2[0] = fft1[1];
+    fft1[1] = fft2[1] = 0.0;
+    for (size_t k = 1; k <= n / 2; ++k) {
+        const size_t r = n - k;
+        const double rep = 0.5 * (fft1[2 * k] + fft1[2 * r]);
+        const double rem = 0.5 * (fft1[2 * k] - fft1[2 * r]);
+        const double aip = 0.5 * (fft1[2 * k + 1] + fft1[2 * r + 1]);
+        const double aim = 0.5 * (fft1[2 * k + 1] - fft1[2 * r + 1]);
+        fft1[2 * k] = rep;
+        fft1[2 * k + 1] = aim;
+        fft1[2 * r] = rep;
+        fft1[2 * r + 1] = -aim;
+        fft2[2 * k] = aip;
+        fft2[2 * k + 1] = -rem;
+        fft2[2 * r] = aip;
+        fft2[2 * r + 1] = rem;
+    }
+}
+
+/* ------------------------------------------------------------------------
+ * cosft1: Cos_FT.rs:7-68 -- NR cosft1 (DCT-I), 0-based: y[0..n] (n + 1
+ * values) becomes Y_k = (y_0 + (-1)^k y_n)/2 + sum_{j=1}^{n-1} y_j cos(pi jk/n).
+ * Pre-pass with recurrence twiddles, realft(n, +1), running sum over the odd
+ * slots.  Divergences of the Rust source corrected here: 1-based indices in a
+ * 0-based slice (Cos_FT.rs:16-17), realft left unimplemented (Cos_FT.rs:71-75),
+ * and the missing y[2] = sum step of NR after realft (Cos_FT.rs:58-64).
+ * ------------------------------------------------------------------------ */
+void orc_cosft1(double *y, size_t n) {
+    const double theta = ORC_PI / (double)n;
+    double wtemp = sin(0.5 * theta);
+    const double wpr = -2.0 * wtemp * wtemp, wpi = sin(theta);
+    double wr = 1.0, wi = 0.0;
+    double sum = 0.5 * (y[0] - y[n]);
+    y[0] = 0.5 * (y[0] + y[n]);
+    for (size_t j = 1; j < n / 2; ++j) {
+        wtemp = wr;
+        wr = wtemp * wpr - wi * wpi + wr;
+        wi = wi * wpr + wtemp * wpi + wi;
+        const double y1 = 0.5 * (y[j] + y[n - j]);
+        const double y2 = y[j] - y[n - j];
+        y[j] = y1 - wi * y2;
+        y[n - j] = y1 + wi * y2;
+        sum += wr * y2;
+    }
+    orc_realft(y, n, 1);
+    y[n] = y[1];
+    y[1] = sum;
+    for (size_t j = 3; j < n; j += 2) {
+        sum += y[j];
+        y[j] = sum;
+    }
+}
+
+/* ------------------------------------------------------------------------
+ * cosft2: Cos_FT2.rs:7-173 -- NR cosft2, 0-based y[0..n-1].
+ *   isign = +1: Y_k = sum_j y_j cos(pi k (j + 1/2) / n)            (DCT-II)
+ *   isign = -1: y_j = Y_0/2 + sum_{k>=1} Y_k cos(pi k (j + 1/2) / n)  (DCT-III,
+ *               so inverse(forward(y)) = (n/2) y)
+ * Divergences of the Rust source corrected here: 1-based indices in a
+ * 0-based slice, the pre-pass twiddle wi1 never advanced (Cos_FT2.rs:25-35),
+ * recurrences adding wr1/wi1 where NR adds wr/wi (Cos_FT2.rs:56-57, 106-107),
+ * the inverse rotation's operands swapped (Cos_FT2.rs:118-119), realft left
+ * unimplemented.
+ * ------------------------------------------------------------------------ */
+void orc_cosft2(double *y, size_t n, int isign) {
+    const double theta = 0.5 * ORC_PI / (double)n;
+    double wr = 1.0, wi = 0.0, wr1 = cos(theta), wi1 = sin(theta);
+    const double wpr = -2.0 * wi1 * wi1, wpi = sin(2.0 * theta);
+    double wtemp;
+    if (isign == 1) {
+        for (size_t i = 0; i < n / 2; ++i) {
+            const double y1 = 0.5 * (y[i] + y[n - 1 - i]);
+            const double y2 = wi1 * (y[i] - y[n - 1 - i]);
+            y[i] = y1 + y2;
+            y[n - 1 - i] = y1 - y2;
+            wtemp = wr1;
+            wr1 = wtemp * wpr - wi1 * wpi + wr1;
+            wi1 = wi1 * wpr + wtemp * wpi + wi1;
+        }
+        orc_realft(y, n, 1);
+        for (size_t i = 2; i < n; i += 2) {
+            wtemp = wr;
+            wr = wtemp * wpr - wi * wpi + wr;
+            wi = wi * wpr + wtemp * wpi + wi;
+            const double y1 = y[i] * wr - y[i + 1] * wi;
+            const double y2 = y[i + 1] * wr + y[i] * wi;
+            y[i] = y1;
+            y[i + 1] = y2;
+        }
+        double sum = 0.5 * y[1];
+        for (size_t i = n; i >= 2; i -= 2) {
+            const double sum1 = sum;
+            sum += y[i - 1];
+            y[i - 1] = sum1;
+        }
+    } else {
+        const double ytemp = y[n - 1];
+        for (size_t i = n; i >= 4; i -= 2) y[i - 1] = y[i - 3] - y[i - 1];
+        y[1] = 2.0 * ytemp;
+        for (size_t i = 2; i < n; i += 2) {
+            wtemp = wr;
+            wr = wtemp * wpr - wi * wpi + wr;
+            wi = wi * wpr + wtemp * wpi + wi;
+            const double y1 = y[i] * wr + y[i + 1] * wi;
+            const double y2 = y[i + 1] * wr - y[i] * wi;
+            y[i] = y1;
+            y[i + 1] = y2;
+        }
+        orc_realft(y, n, -1);
+        for (size_t i = 0; i < n / 2; ++i) {
+            const double y1 = y[i] + y[n - 1 - i];
+            const double y2 = (0.5 / wi1) * (y[i] - y[n - 1 - i]);
+            y[i] = 0.5 * (y1 + y2);
+            y[n - 1 - i] = 0.5 * (y1 - y2);
+            wtemp = wr1;
+            wr1 = wtemp * wpr - wi1 * wpi + wr1;
+            wi1 = wi1 * wpr + wtemp * wpi + wi1;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------
+ * rlft3: Real_FT3.rs:8-141 -- NR rlft3, C order, 0-based.
+ * data: nn1 x nn2 x nn3 reals; speq: nn1 x 2*nn2 reals (the nn3/2 plane).
+ * isign = +1: data <- F[k1][k2][k3] for k3 < nn3/2 (complex, interleaved),
+ *             speq <- F[k1][k2][nn3/2],  F = sum x e^{+2 pi i (j.k / nn)}.
+ * isign = -1: the inverse; data <- (nn1 nn2 nn3 / 2) x.
+ * fourn over (nn1, nn2, nn3/2) runs with length-1 axes dropped (NR's 2-D use
+ * sets nn1 = 1).  Divergence of the Rust source corrected here: its forward
+ * fill of speq uses the mirrored column (Real_FT3.rs:44-49) where NR fills it
+ * in order, so the main loop's mirrored read (Real_FT3.rs:72-77) pairs the
+ * wrong bins.  Returns fourn's status (0 = ok).
+ * ------------------------------------------------------------------------ */
+int orc_rlft3(double *data, double *speq, size_t nn1, size_t nn2, size_t nn3, int isign) {
+    const double c1 = 0.5, c2 = -0.5 * isign;
+    const double theta = isign * (2.0 * ORC_PI / (double)nn3);
+    double wtemp = sin(0.5 * theta);
+    const double wpr = -2.0 * wtemp * wtemp, wpi = sin(theta);
+    const size_t h = nn3 >> 1;
+    size_t dims[3], nd = 0;
+    if (nn1 > 1) dims[nd++] = nn1;
+    if (nn2 > 1) dims[nd++] = nn2;
+    if (h > 1) dims[nd++] = h;
+#define RD(a, b, c) data[((a) * nn2 + (b)) * nn3 + (c)]
+#define RS(a, c) speq[(a) * 2 * nn2 + (c)]
+    if (isign == 1) {
+        if (nd) {
+            int rc = orc_fourn(data, dims, nd, 1);
+            if (rc) return rc;
+        }
+        for (size_t i1 = 0; i1 < nn1; ++i1)
+            for (size_t i2 = 0; i2 < nn2; ++i2) {
+                RS(i1, 2 * i2) = RD(i1, i2, 0);
+                RS(i1, 2 * i2 + 1) = RD(i1, i2, 1);
+            }
+    }
+    for (size_t i1 = 0; i1 < nn1; ++i1) {
+        const size_t j1 = (nn1 - i1) % nn1;
+        double wr = 1.0, wi = 0.0;
+        for (size_t k3 = 0; k3 <= (nn3 >> 2); ++k3) {
+            for (size_t i2 = 0; i2 < nn2; ++i2) {
+                const size_t j2 = (nn2 - i2) % nn2;
+                if (k3 == 0) {
+                    const double h1r = c1 * (RD(i1, i2, 0) + RS(j1, 2 * j2));
+                    const double h1i = c1 * (RD(i1, i2, 1) - RS(j1, 2 * j2 + 1));
+                    const double h2i = c2 * (RD(i1, i2, 0) - RS(j1, 2 * j2));
+                    const double h2r = -c2 * (RD(i1, i2, 1) + RS(j1, 2 * j2 + 1));
+                    RD(i1, i2, 0) = h1r + h2r;
+                    RD(i1, i2, 1) = h1i + h2i;
+                    RS(j1, 2 * j2) = h1r - h2r;
+                    RS(j1, 2 * j2 + 1) = h2i - h1i;
+                } else {
+                    const size_t a = 2 * k3, b = 2 * (h - k3);
+                    const double h1r = c1 * (RD(i1, i2, a) + RD(j1, j2, b));
+                    const double h1i = c1 * (RD(i1, i2, a + 1) - RD(j1, j2, b + 1));
+                    const double h2i = c2 * (RD(i1, i2, a) - RD(j1, j2, b));
+                    const double h2r = -c2 * (RD(i1, i2, a + 1) + RD(j1, j2, b + 1));
+                    RD(i1, i2, a) = h1r + wr * h2r - wi * h2i;
+                    RD(i1, i2, a + 1) = h1i + wr * h2i + wi * h2r;
+                    RD(j1, j2, b) = h1r - wr * h2r + wi * h2i;
+                    RD(j1, j2, b + 1) = -h1i + wr * h2i + wi * h2r;
+                }
+            }
+            wtemp = wr;
+            wr = wtemp * wpr - wi * wpi + wr;
+            wi = wi * wpr + wtemp * wpi + wi;
+        }
+    }
+#undef RD
+#undef RS
+    if (isign == -1 && nd) return orc_fourn(data, dims, nd, -1);
+    return 0;
+}
+
 int orc_four1_batch(double *data, size_t nn, size_t batch, int isign, int nthreads) {
     int used = 1;
 #ifdef _OPENMP

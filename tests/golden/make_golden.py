@@ -9,12 +9,18 @@ Conventions pinned (numrs, FFT_1.rs:5-44 / Real_FT.rs:4-21):
   convlv   = circular convolution with the Convolve.rs:41-63 placement
   correl   = circular correlation r[l] = sum_j d1[(j+l) mod n] d2[j] (n > 32)
              linear correlation at lags 0..n-1 (n <= 32, Correlation.rs:37-50)
+  twofft   = (n * ifft(d1), n * ifft(d2)), bins 0..n-1 (FFT_2.rs:3-17)
+  cosft1   = dct(y, type=1) / 2 on n + 1 points (Cos_FT.rs:7)
+  cosft2   = dct(y, type=2) / 2 (isign 1), dct(y, type=3) / 2 (isign -1) (Cos_FT2.rs:7)
+  rlft3    = prod(nn) * ifftn(x): data holds k3 < nn3/2, speq holds k3 = nn3/2
+             (Real_FT3.rs:8)
 
 Run: python tests/golden/make_golden.py
 """
 import os
 
 import numpy as np
+import scipy.fft as sfft
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -96,6 +102,30 @@ def main():
     d1, d2 = rng(23).normal(size=n), rng(24).normal(size=n)
     g["correl_16_d1"], g["correl_16_d2"] = d1, d2
     g["correl_16_out"] = np.array([np.dot(d1[lag:], d2[:n - lag]) for lag in range(n)])
+    n = 64
+    d1, d2 = rng(31).normal(size=n), rng(32).normal(size=n)
+    g["twofft_64_d1"], g["twofft_64_d2"] = d1, d2
+    for name, d in (("fft1", d1), ("fft2", d2)):
+        F = n * np.fft.ifft(d)
+        out = np.empty(2 * n)
+        out[0::2], out[1::2] = F.real, F.imag
+        g[f"twofft_64_{name}"] = out
+    y = rng(33).normal(size=n + 1)
+    g["cosft1_64_in"], g["cosft1_64_out"] = y, 0.5 * sfft.dct(y, type=1)
+    y = rng(34).normal(size=n)
+    g["cosft2_64_in"] = y
+    g["cosft2_64_1_out"] = 0.5 * sfft.dct(y, type=2)
+    g["cosft2_64_-1_out"] = 0.5 * sfft.dct(y, type=3)
+    shape = (4, 8, 16)
+    x = rng(35).normal(size=shape)
+    F = np.prod(shape) * np.fft.ifftn(x)
+    data = np.empty(x.size)
+    data[0::2] = F[:, :, :shape[2] // 2].reshape(-1).real
+    data[1::2] = F[:, :, :shape[2] // 2].reshape(-1).imag
+    speq = np.empty(2 * shape[0] * shape[1])
+    speq[0::2] = F[:, :, shape[2] // 2].reshape(-1).real
+    speq[1::2] = F[:, :, shape[2] // 2].reshape(-1).imag
+    g["rlft3_4x8x16_in"], g["rlft3_4x8x16_data"], g["rlft3_4x8x16_speq"] = x.reshape(-1), data, speq
     np.savez_compressed(os.path.join(HERE, "golden.npz"), **g)
     print(f"wrote {len(g)} arrays")
 

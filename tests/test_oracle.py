@@ -238,3 +238,106 @@ def test_generator_matches_oracle():
     assert u.min() >= -1.0 and u.max() < 1.0 and abs(u.mean()) < 0.01
     g = synth.normal(1, 100000)
     assert abs(g.mean()) < 0.02 and abs(g.std() - 1.0) < 0.02
+
+
+# ---- twofft (FFT_2.rs:3-132) -------------------------------------------------
+def test_twofft_golden(golden):
+    f1, f2 = oracle.twofft(golden["twofft_64_d1"], golden["twofft_64_d2"])
+    assert rel_l2(f1, golden["twofft_64_fft1"]) < 1e-14
+    assert rel_l2(f2, golden["twofft_64_fft2"]) < 1e-14
+
+
+@pytest.mark.parametrize("n", [2, 4, 16, 256, 2048])
+def test_twofft_vs_brute(n):
+    # each spectrum equals the brute-force DFT of its real input alone
+    d1, d2 = synth.normal(61, n), synth.normal(62, n)
+    f1, f2 = oracle.twofft(d1, d2)
+    for f, d in ((f1, d1), (f2, d2)):
+        z = np.zeros(2 * n)
+        z[0::2] = d
+        assert rel_l2(f, oracle.dft_brute(z, 1)) < 1e-13
+
+
+def test_twofft_closed_forms():
+    n = 32
+    delta = np.zeros(n)
+    delta[0] = 1.0
+    const = np.full(n, 2.0)
+    f1, f2 = oracle.twofft(delta, const)
+    assert np.allclose(cplx(f1), 1.0, atol=1e-15)        # delta -> all ones
+    exp2 = np.zeros(n, complex)
+    exp2[0] = 2.0 * n                                    # constant -> n c delta_0
+    assert np.allclose(cplx(f2), exp2, atol=1e-13)
+
+
+# ---- cosft1 (Cos_FT.rs:7-68) ---------------------------------------------------
+def _dct1_brute(y):
+    n = y.size - 1
+    j = np.arange(1, n)
+    return np.array([0.5 * (y[0] + (-1) ** k * y[n]) + np.sum(y[1:n] * np.cos(np.pi * j * k / n))
+                     for k in range(n + 1)])
+
+
+def test_cosft1_golden(golden):
+    assert rel_l2(oracle.cosft1(golden["cosft1_64_in"]), golden["cosft1_64_out"]) < 1e-14
+
+
+@pytest.mark.parametrize("n", [2, 4, 8, 64, 512])
+def test_cosft1_brute_and_inverse(n):
+    y = synth.normal(63, n + 1)
+    Y = oracle.cosft1(y)
+    assert rel_l2(Y, _dct1_brute(y)) < 1e-13
+    # cosft1 is its own inverse up to n/2
+    assert rel_l2(oracle.cosft1(Y), 0.5 * n * y) < 1e-13
+
+
+# ---- cosft2 (Cos_FT2.rs:7-173) -------------------------------------------------
+def _dct2_brute(y):
+    n = y.size
+    j = np.arange(n)
+    return np.array([np.sum(y * np.cos(np.pi * k * (j + 0.5) / n)) for k in range(n)])
+
+
+def _dct3_brute(Y):
+    n = Y.size
+    k = np.arange(1, n)
+    return np.array([0.5 * Y[0] + np.sum(Y[1:] * np.cos(np.pi * k * (j + 0.5) / n)) for j in range(n)])
+
+
+@pytest.mark.parametrize("isign", [1, -1])
+def test_cosft2_golden(golden, isign):
+    assert rel_l2(oracle.cosft2(golden["cosft2_64_in"], isign), golden[f"cosft2_64_{isign}_out"]) < 1e-14
+
+
+@pytest.mark.parametrize("n", [2, 4, 8, 64, 512])
+def test_cosft2_brute_and_roundtrip(n):
+    y = synth.normal(64, n)
+    Y = oracle.cosft2(y, 1)
+    assert rel_l2(Y, _dct2_brute(y)) < 1e-13
+    assert rel_l2(oracle.cosft2(y, -1), _dct3_brute(y)) < 1e-13
+    assert rel_l2(oracle.cosft2(Y, -1), 0.5 * n * y) < 1e-13
+
+
+# ---- rlft3 (Real_FT3.rs:8-141) -------------------------------------------------
+def test_rlft3_golden(golden):
+    x = golden["rlft3_4x8x16_in"]
+    d, s = oracle.rlft3(x, np.zeros(2 * 4 * 8), 4, 8, 16, 1)
+    assert rel_l2(d, golden["rlft3_4x8x16_data"]) < 1e-14
+    assert rel_l2(s, golden["rlft3_4x8x16_speq"]) < 1e-14
+
+
+@pytest.mark.parametrize("shape", [(4, 8, 16), (1, 8, 16), (2, 2, 4), (1, 1, 8), (4, 4, 2), (8, 1, 32)])
+def test_rlft3_vs_fourn_and_roundtrip(shape):
+    # forward equals the complex fourn of the zero-imaginary data, cut at
+    # k3 <= nn3/2; the inverse returns (nn1 nn2 nn3 / 2) x
+    nn1, nn2, nn3 = shape
+    x = synth.normal(65, nn1 * nn2 * nn3)
+    d, s = oracle.rlft3(x, np.zeros(2 * nn1 * nn2), nn1, nn2, nn3, 1)
+    z = np.zeros(2 * x.size)
+    z[0::2] = x
+    dims = [v for v in shape if v > 1]
+    F = cplx(oracle.fourn(z, dims, 1) if dims else z).reshape(shape)
+    assert rel_l2(cplx(d).reshape(nn1, nn2, nn3 // 2), F[:, :, :nn3 // 2]) < 1e-13
+    assert rel_l2(cplx(s).reshape(nn1, nn2), F[:, :, nn3 // 2]) < 1e-13
+    xi, _ = oracle.rlft3(d, s, nn1, nn2, nn3, -1)
+    assert rel_l2(xi, 0.5 * nn1 * nn2 * nn3 * x) < 1e-13
