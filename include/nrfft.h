@@ -139,6 +139,47 @@ int nrfft_execute_correl(nrfft_plan plan, const void* d_data1, const void* d_dat
 int nrfft_execute_c2c_launches(nrfft_plan plan, const void* d_in, void* d_out, int isign, void* stream, int first,
                                int count);
 
+/* ---- twofft / cosine transforms / 3-D real FFT ------------------------------
+ * Numerical Recipes semantics with 0-based storage (the Rust sources are 1-based
+ * NR translations; the divergences are listed in the oracle).  Host-pointer
+ * calls stage through HBM like the ones above. */
+/* FFT_2.rs:3 twofft(data1, data2, fft1, fft2): the isign = +1 spectra of two
+ * real sequences of n points from one complex FFT.  fft1 / fft2 hold 2n + 2
+ * values (FFT_2.rs:6-7): bins 0..n-1 interleaved in the first 2n, then 0, 0. */
+int nrfft_twofft_f64(const double* data1, const double* data2, size_t n, double* fft1, double* fft2);
+int nrfft_twofft_f32(const float* data1, const float* data2, size_t n, float* fft1, float* fft2);
+/* Cos_FT.rs:7 cosft1(y, n): DCT-I in place on y[0..n] (n + 1 values):
+ * Y_k = (y_0 + (-1)^k y_n)/2 + sum_{j=1}^{n-1} y_j cos(pi j k / n); its own
+ * inverse up to n/2. */
+int nrfft_cosft1_f64(double* y, size_t n);
+int nrfft_cosft1_f32(float* y, size_t n);
+/* Cos_FT2.rs:7 cosft2(y, n, isign): in place on y[0..n-1]; isign = 1:
+ * Y_k = sum_j y_j cos(pi k (j + 1/2) / n); isign = -1: the inverse
+ * y_j = Y_0/2 + sum_{k>=1} Y_k cos(pi k (j + 1/2) / n) (= (n/2) x the input of
+ * the forward call). */
+int nrfft_cosft2_f64(double* y, size_t n, int isign);
+int nrfft_cosft2_f32(float* y, size_t n, int isign);
+/* Real_FT3.rs:8 rlft3(data, speq, nn1, nn2, nn3, isign): 3-D real FFT, C order.
+ * isign = 1: data (nn1 x nn2 x nn3 reals) becomes the complex half spectrum
+ * F[k1][k2][k3], k3 < nn3/2, and speq (nn1 x 2 nn2 reals) F[k1][k2][nn3/2];
+ * isign = -1 inverts (data <- (nn1 nn2 nn3 / 2) x).  nn1 = 1 gives the 2-D
+ * transform. */
+int nrfft_rlft3_f64(double* data, double* speq, size_t nn1, size_t nn2, size_t nn3, int isign);
+int nrfft_rlft3_f32(float* data, float* speq, size_t nn1, size_t nn2, size_t nn3, int isign);
+/* Device plans of the same transforms (batch back to back; twofft outputs
+ * are 2n values per transform, cosft1 n + 1, cosft2 n, rlft3 speq
+ * 2 nn1 nn2).  Outputs must not overlap inputs except in/out of cosft1/2
+ * (in place allowed) and rlft3 (in place by definition). */
+int nrfft_plan_twofft(nrfft_plan* plan, int dtype, size_t n, size_t batch);
+int nrfft_execute_twofft(nrfft_plan plan, const void* d_data1, const void* d_data2, void* d_fft1, void* d_fft2,
+                         void* stream);
+int nrfft_plan_cosft1(nrfft_plan* plan, int dtype, size_t n, size_t batch);
+int nrfft_execute_cosft1(nrfft_plan plan, const void* d_in, void* d_out, void* stream);
+int nrfft_plan_cosft2(nrfft_plan* plan, int dtype, size_t n, size_t batch);
+int nrfft_execute_cosft2(nrfft_plan plan, const void* d_in, void* d_out, int isign, void* stream);
+int nrfft_plan_rlft3(nrfft_plan* plan, int dtype, size_t nn1, size_t nn2, size_t nn3, size_t batch);
+int nrfft_execute_rlft3(nrfft_plan plan, void* d_data, void* d_speq, int isign, void* stream);
+
 /* ---- uneven batches -------------------------------------------------------
  * Many transforms of different lengths and dimensionality in one persistent
  * work-queue kernel (two dispatches, one per pass phase, replayed as a

@@ -1,4 +1,5 @@
-"""numrs_amd -- the numrs FFT path (four1, fourn, realft, convlv, correl) as
+"""numrs_amd -- the numrs FFT path (four1, fourn, realft, convlv, correl, and
+twofft, cosft1, cosft2, rlft3 on top of them) as
 hand-written HIP kernels for MI355X (gfx950), behind the C ABI of
 include/nrfft.h.
 
@@ -14,8 +15,10 @@ from .api import (  # noqa: F401
     FournError,
     NrfftError,
     RealFTProcessor,
+    TwoFFTProcessor,
     autocorrel,
     autocorrel_fast,
+    combine_real_imag,
     complex_to_real,
     convlv,
     convlv_batch,
@@ -24,6 +27,11 @@ from .api import (  # noqa: F401
     correl_batch,
     correl_normalized,
     correl_normalized_fast,
+    cosft1,
+    cosft1_optimized,
+    cosft2,
+    cosft2_simd,
+    extract_real_imag,
     four1,
     four1_optimized,
     fourn,
@@ -32,12 +40,21 @@ from .api import (  # noqa: F401
     real_to_complex,
     realft,
     realft_optimized,
+    rlft3,
+    rlft3_optimized,
+    twofft,
+    twofft_batch,
+    twofft_optimized,
 )
 from .device import (  # noqa: F401
     C2CPlan,
     ConvlvPlan,
     CorrelPlan,
+    Cosft1Plan,
+    Cosft2Plan,
     RealFTPlan,
+    Rlft3Plan,
+    TwofftPlan,
     UnevenPlan,
     autotune_c2c,
     gen_uniform,

@@ -65,6 +65,22 @@ _SIGS = {
     "nrfft_execute_realft": (_i, [_vp, _vp, _i, _vp]),
     "nrfft_plan_convlv": (_i, [ctypes.POINTER(_vp), _i, _sz, _sz, _sz]),
     "nrfft_execute_convlv": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
+    "nrfft_twofft_f64": (_i, [_dp, _dp, _sz, _dp, _dp]),
+    "nrfft_twofft_f32": (_i, [_fp, _fp, _sz, _fp, _fp]),
+    "nrfft_cosft1_f64": (_i, [_dp, _sz]),
+    "nrfft_cosft1_f32": (_i, [_fp, _sz]),
+    "nrfft_cosft2_f64": (_i, [_dp, _sz, _i]),
+    "nrfft_cosft2_f32": (_i, [_fp, _sz, _i]),
+    "nrfft_rlft3_f64": (_i, [_dp, _dp, _sz, _sz, _sz, _i]),
+    "nrfft_rlft3_f32": (_i, [_fp, _fp, _sz, _sz, _sz, _i]),
+    "nrfft_plan_twofft": (_i, [ctypes.POINTER(_vp), _i, _sz, _sz]),
+    "nrfft_execute_twofft": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "nrfft_plan_cosft1": (_i, [ctypes.POINTER(_vp), _i, _sz, _sz]),
+    "nrfft_execute_cosft1": (_i, [_vp, _vp, _vp, _vp]),
+    "nrfft_plan_cosft2": (_i, [ctypes.POINTER(_vp), _i, _sz, _sz]),
+    "nrfft_execute_cosft2": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "nrfft_plan_rlft3": (_i, [ctypes.POINTER(_vp), _i, _sz, _sz, _sz, _sz]),
+    "nrfft_execute_rlft3": (_i, [_vp, _vp, _vp, _i, _vp]),
     "nrfft_plan_correl": (_i, [ctypes.POINTER(_vp), _i, _sz, _sz]),
     "nrfft_execute_correl": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "nrfft_execute_c2c_launches": (_i, [_vp, _vp, _vp, _i, _vp, _i, _i]),
@@ -106,6 +122,15 @@ def lib() -> ctypes.CDLL:
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build the HIP extension first "
                 "(python -c 'import __graft_entry__ as g; g.build()')")
+        # torch ships its own libamdhip64 / libhsa-runtime64.  Loaded first,
+        # they satisfy this library's libamdhip64.so.7 dependency and the
+        # process has one HIP runtime; loaded after this library has started
+        # the system runtime, torch's second copy finds no GPU.  So torch (when
+        # installed) is imported before the library is opened.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             f = getattr(L, name)

@@ -429,3 +429,137 @@ def autocorrel_fast(data) -> np.ndarray:
     if a.size == 0:
         raise CorrelError("EmptyInput")
     return correl(a, a)
+
+
+# ---------------------------------------------------------------------------
+# twofft (FFT_2.rs), cosft1 (Cos_FT.rs), cosft2 (Cos_FT2.rs), rlft3 (Real_FT3.rs)
+# NR semantics, 0-based (the Rust sources are 1-based NR translations).
+# ---------------------------------------------------------------------------
+
+def twofft(data1, data2, fft1: np.ndarray, fft2: np.ndarray) -> None:
+    """FFT_2.rs:3 ``twofft(data1, data2, fft1, fft2)``: the isign = +1 spectra
+    of two real sequences of n points from one complex FFT.  fft1 / fft2 are
+    outputs of 2n + 2 values (asserted as FFT_2.rs:5-7): bins 0..n-1
+    interleaved, then 0, 0."""
+    d1 = np.asarray(data1)
+    n = d1.size
+    assert np.asarray(data2).size == n, "data2 length must equal data1 length"   # FFT_2.rs:5
+    assert fft1.size == 2 * n + 2, "fft1 must have length 2*n + 2"               # FFT_2.rs:6
+    assert fft2.size == 2 * n + 2, "fft2 must have length 2*n + 2"               # FFT_2.rs:7
+    dt = fft1.dtype if fft1.dtype in (np.float32, np.float64) else np.float64
+    a, b = _as_input(d1, dt), _as_input(data2, dt)
+    for f in (fft1, fft2):
+        _inplace_view(f, 2 * n + 2)
+        if f.dtype != dt:
+            raise TypeError("fft1 and fft2 must have the same float dtype")
+    p1, sfx = _buf(a)
+    p2, _ = _buf(b)
+    q1, _ = _buf(fft1)
+    q2, _ = _buf(fft2)
+    _check(getattr(_lib.lib(), f"nrfft_twofft_{sfx}")(p1, p2, n, q1, q2))
+
+
+twofft_optimized = twofft  # FFT_2.rs:134
+
+
+def cosft1(y: np.ndarray, n: int) -> None:
+    """Cos_FT.rs:7 ``cosft1(y, n)``: DCT-I in place on y[0..n] (n + 1 values),
+    Y_k = (y_0 + (-1)^k y_n)/2 + sum_{j=1}^{n-1} y_j cos(pi j k / n)."""
+    v = _inplace_view(y, n + 1)
+    p, sfx = _buf(v)
+    _check(getattr(_lib.lib(), f"nrfft_cosft1_{sfx}")(p, n))
+
+
+cosft1_optimized = cosft1  # Cos_FT.rs:78
+
+
+def cosft2(y: np.ndarray, n: int, isign: int) -> None:
+    """Cos_FT2.rs:7 ``cosft2(y, n, isign)``: in place on y[0..n-1]; isign = 1
+    the DCT-II Y_k = sum_j y_j cos(pi k (j + 1/2) / n), isign = -1 its
+    inverse scaled by n/2.  Any other isign panics in the reference
+    (Cos_FT2.rs:11); here it raises ``ValueError``."""
+    if isign not in (1, -1):
+        raise ValueError(f"Invalid isign value: {isign}. Must be 1 or -1")
+    v = _inplace_view(y, n)
+    p, sfx = _buf(v)
+    _check(getattr(_lib.lib(), f"nrfft_cosft2_{sfx}")(p, n, isign))
+
+
+def rlft3(data: np.ndarray, speq: np.ndarray, nn1: int, nn2: int, nn3: int, isign: int) -> None:
+    """Real_FT3.rs:8 ``rlft3(data, speq, nn1, nn2, nn3, isign)`` with the
+    asserts of Real_FT3.rs:16-19.  data: (nn1, nn2, nn3) reals, in place --
+    forward leaves the half spectrum F[k1, k2, k3 < nn3/2] as interleaved
+    complex values; speq: (nn1, 2 nn2) reals, the k3 = nn3/2 plane."""
+    assert isign in (1, -1), "isign must be 1 or -1"
+    assert tuple(data.shape) == (nn1, nn2, nn3), "data dimensions mismatch"
+    assert tuple(speq.shape) == (nn1, 2 * nn2), "speq dimensions mismatch"
+    for a in (data, speq):
+        if not a.flags.c_contiguous or not a.flags.writeable:
+            raise ValueError("data and speq must be writable C-contiguous arrays")
+    if data.dtype != speq.dtype:
+        raise TypeError("data and speq must have the same dtype")
+    p, sfx = _buf(data.reshape(-1))
+    q, _ = _buf(speq.reshape(-1))
+    _check(getattr(_lib.lib(), f"nrfft_rlft3_{sfx}")(p, q, nn1, nn2, nn3, isign))
+
+
+def twofft_batch(pairs: Sequence[tuple]) -> None:
+    """Each (data1, data2, fft1, fft2) through ``twofft``."""
+    for d1, d2, f1, f2 in pairs:
+        twofft(d1, d2, f1, f2)
+
+
+class TwoFFTProcessor:
+    """FFT_2.rs:221-264: ``process`` / ``process_batch`` over (data1, data2,
+    fft1, fft2) tuples; the size threshold only picks between CPU variants in
+    the reference, every size runs on the GPU here."""
+
+    def __init__(self) -> None:
+        self.use_optimized = True
+        self.parallel_threshold = 1024
+
+    @classmethod
+    def new(cls) -> "TwoFFTProcessor":
+        return cls()
+
+    def with_optimized(self, use_optimized: bool) -> "TwoFFTProcessor":
+        self.use_optimized = use_optimized
+        return self
+
+    def with_threshold(self, threshold: int) -> "TwoFFTProcessor":
+        self.parallel_threshold = threshold
+        return self
+
+    def process(self, data1, data2, fft1: np.ndarray, fft2: np.ndarray) -> None:
+        twofft(data1, data2, fft1, fft2)
+
+    def process_batch(self, batches: Sequence[tuple]) -> None:
+        twofft_batch(batches)
+
+
+def extract_real_imag(fft) -> tuple:
+    """FFT_2.rs:361-373."""
+    f = np.asarray(fft, dtype=np.float64)
+    n = f.size // 2
+    return f[0:2 * n:2].copy(), f[1:2 * n:2].copy()
+
+
+def combine_real_imag(real, imag) -> np.ndarray:
+    """FFT_2.rs:375-384."""
+    r, i = np.asarray(real, dtype=np.float64), np.asarray(imag, dtype=np.float64)
+    assert r.size == i.size, "Real and imaginary parts must have same length"
+    out = np.empty(2 * r.size)
+    out[0::2], out[1::2] = r, i
+    return out
+
+
+cosft2_simd = cosft2  # Cos_FT2.rs:202
+
+
+def rlft3_optimized(data: np.ndarray, speq: np.ndarray, nn1: int, nn2: int, nn3: int, isign: int) -> None:
+    """Real_FT3.rs:145: rlft3 on flat slices (asserts of Real_FT3.rs:153-154)."""
+    assert data.size >= nn1 * nn2 * nn3, "data array too small"
+    assert speq.size >= nn1 * 2 * nn2, "speq array too small"
+    d = _inplace_view(data, nn1 * nn2 * nn3).reshape(nn1, nn2, nn3)
+    s = _inplace_view(speq, nn1 * 2 * nn2).reshape(nn1, 2 * nn2)
+    rlft3(d, s, nn1, nn2, nn3, isign)

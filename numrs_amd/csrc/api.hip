@@ -328,13 +328,13 @@ int plan_c2c_nd(Plan& p, const std::vector<size_t>& nn, int64_t batch) {
 // Execution.
 // ---------------------------------------------------------------------------
 int exec_plan(const Plan& p, const void* in, void* out, void* tmp, const void* aux, int isign, hipStream_t stream,
-              int first, int count) {
+              int first, int count, void* aux2) {
     DeviceCtx* ctx = nullptr;
     int rc = get_ctx(&ctx);
     if (rc) return rc;
     int idx = -1;
     void* tmp2 = tmp ? static_cast<char*>(tmp) + p.tmp2_offset * csize(p.dtype) : nullptr;
-    void* bufs[B_COUNT] = {const_cast<void*>(in), out, tmp, const_cast<void*>(aux), tmp2};
+    void* bufs[B_COUNT] = {const_cast<void*>(in), out, tmp, const_cast<void*>(aux), tmp2, aux2};
     for (const Launch& l : p.launches) {
         ++idx;
         if (idx < first || (count >= 0 && idx >= first + count)) continue;
@@ -365,7 +365,7 @@ int exec_plan(const Plan& p, const void* in, void* out, void* tmp, const void* a
             }
             continue;
         }
-        rc = exec_special(p, l, bufs, eff, stream, ctx);
+        rc = l.kind == LK_SPEC ? exec_spectral(p, l, bufs, eff, stream, ctx) : exec_special(p, l, bufs, eff, stream, ctx);
         if (rc) return rc;
     }
     return NRFFT_OK;
@@ -408,6 +408,11 @@ int build_plan(Plan& p) {
     switch (p.kind) {
         case PK_C2C:
             return plan_c2c_nd(p, p.nn, (int64_t)p.batch);
+        case PK_TWOFFT:
+        case PK_COSFT1:
+        case PK_COSFT2:
+        case PK_RLFT3:
+            return build_plan_spectral(p);
         default:
             return build_plan_real(p);
     }

@@ -40,11 +40,21 @@ inline Twiddles<T> ctx_twiddles(const DeviceCtx* c) {
 int get_ctx(DeviceCtx** out);
 
 // Buffers a launch reads or writes.
-enum BufId { B_IN = 0, B_OUT = 1, B_TMP = 2, B_AUX = 3, B_TMP2 = 4, B_COUNT = 5 };
+// B_AUX2: a second caller buffer (twofft's second spectrum, rlft3's speq).
+enum BufId { B_IN = 0, B_OUT = 1, B_TMP = 2, B_AUX = 3, B_TMP2 = 4, B_AUX2 = 5, B_COUNT = 6 };
 // Plan kinds.
-enum PlanKind { PK_C2C = 0, PK_REALFT = 1, PK_CONVLV = 2, PK_CORREL = 3 };
+enum PlanKind {
+    PK_C2C = 0,
+    PK_REALFT = 1,
+    PK_CONVLV = 2,
+    PK_CORREL = 3,
+    PK_TWOFFT = 4,
+    PK_COSFT1 = 5,
+    PK_COSFT2 = 6,
+    PK_RLFT3 = 7
+};
 // Launch kinds.
-enum LaunchKind { LK_C2C = 0, LK_COPY = 1, LK_REAL = 2 };
+enum LaunchKind { LK_C2C = 0, LK_COPY = 1, LK_REAL = 2, LK_SPEC = 3 };
 
 // Tunables chosen per (kind, N, batch, dtype); -1 = default.
 struct Tune {
@@ -104,12 +114,15 @@ size_t csize(int dtype);
 int build_plan(Plan& p);
 int build_plan_real(Plan& p);
 int exec_special(const Plan& p, const Launch& l, void* const* bufs, int isign, hipStream_t stream, DeviceCtx* ctx);
+int build_plan_spectral(Plan& p);
+int exec_spectral(const Plan& p, const Launch& l, void* const* bufs, int isign, hipStream_t stream, DeviceCtx* ctx);
+int plan_c2c_nd(Plan& p, const std::vector<size_t>& nn, int64_t batch);
 void tune_lookup(Plan& p);
 Tune default_tune(const Plan& p);
 int plan_c2c_1d(Plan& p, int logn, int64_t batch, int src, int dst, size_t* tmp_elems);
 int plan_add_c2c_pass(Plan& p, int mode, int logl, int tilog, const PassDesc& d, int src, int dst);
 int exec_plan(const Plan& p, const void* in, void* out, void* tmp, const void* aux, int isign, hipStream_t stream,
-              int first = 0, int count = -1);
+              int first = 0, int count = -1, void* aux2 = nullptr);
 int cached_plan(int kind, int dtype, const std::vector<size_t>& nn, size_t batch, size_t extra,
                 std::shared_ptr<Plan>* out);
 

@@ -1,0 +1,372 @@
+// spectral.hpp -- twofft, cosft1, cosft2 and rlft3 kernels on the Stockham
+// engine and the real-FFT untangle (numrs FFT_2.rs, Cos_FT.rs, Cos_FT2.rs,
+// Real_FT3.rs; Numerical Recipes semantics, 0-based).
+//
+// twofft: two real sequences d1, d2 of n points are one complex sequence
+//   z = d1 + i d2; after an n-point FFT (isign +1)
+//     F1[k] = (Z_k + conj Z_{n-k}) / 2,   F2[k] = (Z_k - conj Z_{n-k}) / (2i).
+// cosft1 (DCT-I, n + 1 points): the even extension x of y (2n reals,
+//   x_j = y_j for j <= n, x_{2n-j} = y_j) has a real spectrum, and
+//   Y_k = F_k(x) / 2; x is read as n complex values and the 2n-point real
+//   spectrum is untangled from the n-point FFT.
+// cosft2 (n points; DCT-II forward, DCT-III inverse): v_i = y_{2i},
+//   v_{n-1-i} = y_{2i+1} (i < n/2); with V the isign +1 real spectrum of v
+//   and u_k = exp(i pi k / 2n) V_k:  Y_k = Re u_k,  Y_{n-k} = Im u_k.  The
+//   inverse rebuilds V_k = exp(-i pi k / 2n) (Y_k + i Y_{n-k}) (Y_n = 0) and
+//   returns unpermute(realft^-1(V)), which is NR's (n/2)-scaled inverse.
+// rlft3: fourn over (nn1, nn2, nn3/2) of the data read as complex values,
+//   then the realft untangle between bin (k1, k2, k3) and its mirror
+//   (-k1, -k2, nn3/2 - k3); the k3 = 0 column pairs with the speq plane
+//   (k3 = nn3/2).  The middle column k3 = nn3/4 pairs with itself through a
+//   twiddle of +-i, for which the untangle is the identity, so it is skipped.
+#pragma once
+#include "real_fft.hpp"
+
+namespace nrfft {
+
+template <typename T>
+__device__ __forceinline__ T cos_quarter_pi() {
+    return (T)0.70710678118654752440;
+}
+
+// cosft2 storage position of v_i (n = 2L values).
+__device__ __forceinline__ int64_t cf2_pos(int64_t i, int64_t L) { return i < L ? 2 * i : 2 * (2 * L - 1 - i) + 1; }
+
+// cosft1 even-extension value x_j of y (n = L).
+template <typename T>
+__device__ __forceinline__ T cf1_ext(const T* ys, int64_t j, int64_t L) {
+    return ys[j <= L ? j : 2 * L - j];
+}
+
+// ---------------------------------------------------------------------------
+// Per-pair finishing steps shared by the fused and the multi-pass kernels.
+// ---------------------------------------------------------------------------
+// twofft: bins k and n-k of both spectra from Z_k, Z_{n-k}.
+template <typename T>
+__device__ __forceinline__ void twofft_pair(cpx<T> zk, cpx<T> zr, cpx<T>& g1, cpx<T>& g2) {
+    const T half = (T)0.5;
+    g1 = cpx<T>{half * (zk.x + zr.x), half * (zk.y - zr.y)};
+    g2 = cpx<T>{half * (zk.y + zr.y), -half * (zk.x - zr.x)};
+}
+
+// cosft2 forward: Y_k and Y_{n-k} from V_k (LOGN = log2 n).
+template <typename T>
+__device__ __forceinline__ void cf2_fwd_out(const Twiddles<T>& tw, cpx<T> vk, int64_t k, int logn, T* os, int64_t n) {
+    const cpx<T> u = cmul(twiddle_big<1>(tw, (uint32_t)k, logn + 2), vk);
+    os[k] = u.x;
+    os[n - k] = u.y;
+}
+
+// cosft2 inverse: V_k from Y_k, Y_{n-k}.
+template <typename T>
+__device__ __forceinline__ cpx<T> cf2_inv_in(const Twiddles<T>& tw, const T* ys, int64_t k, int logn, int64_t n) {
+    return cmul(twiddle_big<-1>(tw, (uint32_t)k, logn + 2), cpx<T>{ys[k], ys[n - k]});
+}
+
+// ---------------------------------------------------------------------------
+// Fused kernels: TI transforms per group, the FFT on the engine, the
+// finishing step on the result left in LDS (one thread per bin pair).
+// ---------------------------------------------------------------------------
+template <typename T, int L, int TI, int NT>
+__global__ __launch_bounds__(NT) void k_twofft(const T* d1, const T* d2, cpx<T>* f1, cpx<T>* f2, int64_t nsig,
+                                               Twiddles<T> tw) {
+    using E = Engine<T, 1, L, TI, NT, MAP_J, MAP_J>;
+    __shared__ cpx<T> lds[E::LDS_ELEMS];
+    const int tid = threadIdx.x;
+    const int64_t s0 = int64_t(blockIdx.x) * TI;
+    const int nvalid = (nsig - s0) < TI ? int(nsig - s0) : TI;
+    const T* a = d1 + s0 * L;
+    const T* b = d2 + s0 * L;
+    E::run_to_lds(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
+        return (TI == 1 || t < nvalid) ? cpx<T>{a[t * L + i], b[t * L + i]} : cpx<T>{(T)0, (T)0};
+    });
+    constexpr int NP = L / 2 + 1;
+    for (int q = tid; q < TI * NP; q += NT) {
+        const int t = q / NP, k = q % NP;
+        if (TI > 1 && t >= nvalid) continue;
+        const int r = (L - k) & (L - 1);
+        cpx<T> g1, g2;
+        twofft_pair(lds[E::lidx(t, k)], lds[E::lidx(t, r)], g1, g2);
+        cpx<T>* o1 = f1 + (s0 + t) * L;
+        cpx<T>* o2 = f2 + (s0 + t) * L;
+        o1[k] = g1;
+        o2[k] = g2;
+        if (r != k) {
+            o1[r] = conj(g1);
+            o2[r] = conj(g2);
+        }
+    }
+}
+
+// cosft1, n = L: n + 1 reals per transform (stride L + 1).
+template <typename T, int L, int TI, int NT>
+__global__ __launch_bounds__(NT) void k_cosft1(const T* y, T* out, int64_t nsig, Twiddles<T> tw) {
+    using E = Engine<T, 1, L, TI, NT, MAP_J, MAP_J>;
+    constexpr int64_t S = L + 1;
+    constexpr int LOG2N2 = ilog2c(L) + 1;  // log2 of the 2n-point real transform
+    __shared__ cpx<T> lds[E::LDS_ELEMS];
+    const int tid = threadIdx.x;
+    const int64_t s0 = int64_t(blockIdx.x) * TI;
+    const int nvalid = (nsig - s0) < TI ? int(nsig - s0) : TI;
+    const T* yb = y + s0 * S;
+    E::run_to_lds(tid, lds, tw.stage, [&](int t, int m) -> cpx<T> {
+        if (!(TI == 1 || t < nvalid)) return cpx<T>{(T)0, (T)0};
+        const T* ys = yb + t * S;
+        return cpx<T>{cf1_ext(ys, 2 * m, L), cf1_ext(ys, 2 * m + 1, L)};
+    });
+    constexpr int NP = L / 2 + 1;
+    const T half = (T)0.5;
+    for (int q = tid; q < TI * NP; q += NT) {
+        const int t = q / NP, p = q % NP;
+        if (TI > 1 && t >= nvalid) continue;
+        T* os = out + (s0 + t) * S;
+        if (p == 0) {
+            const cpx<T> z0 = lds[E::lidx(t, 0)];
+            os[0] = half * (z0.x + z0.y);
+            os[L] = half * (z0.x - z0.y);
+        } else {
+            cpx<T> fa, fb;
+            untangle_pair(lds[E::lidx(t, p)], lds[E::lidx(t, L - p)], twiddle_big<1>(tw, (uint32_t)p, LOG2N2),
+                          (T)-0.5, fa, fb);
+            os[p] = half * fa.x;
+            if (L - p != p) os[L - p] = half * fb.x;
+        }
+    }
+}
+
+// cosft2, n = 2L reals per transform; DIR = +1 forward, -1 inverse.
+template <typename T, int L, int TI, int NT, int DIR>
+__global__ __launch_bounds__(NT) void k_cosft2(const T* y, T* out, int64_t nsig, Twiddles<T> tw) {
+    using E = Engine<T, DIR, L, TI, NT, MAP_J, MAP_J>;
+    constexpr int64_t N = 2 * L;
+    constexpr int LOGN = ilog2c(L) + 1;
+    __shared__ cpx<T> lds[E::LDS_ELEMS];
+    const int tid = threadIdx.x;
+    const int64_t s0 = int64_t(blockIdx.x) * TI;
+    const int nvalid = (nsig - s0) < TI ? int(nsig - s0) : TI;
+    const T* yb = y + s0 * N;
+    constexpr int NP = L / 2 + 1;
+    if constexpr (DIR > 0) {
+        E::run_to_lds(tid, lds, tw.stage, [&](int t, int m) -> cpx<T> {
+            if (!(TI == 1 || t < nvalid)) return cpx<T>{(T)0, (T)0};
+            const T* ys = yb + t * N;
+            return cpx<T>{ys[cf2_pos(2 * m, L)], ys[cf2_pos(2 * m + 1, L)]};
+        });
+        for (int q = tid; q < TI * NP; q += NT) {
+            const int t = q / NP, p = q % NP;
+            if (TI > 1 && t >= nvalid) continue;
+            T* os = out + (s0 + t) * N;
+            if (p == 0) {
+                const cpx<T> z0 = lds[E::lidx(t, 0)];
+                os[0] = z0.x + z0.y;
+                os[L] = (z0.x - z0.y) * cos_quarter_pi<T>();
+            } else {
+                cpx<T> va, vb;
+                untangle_pair(lds[E::lidx(t, p)], lds[E::lidx(t, L - p)], twiddle_big<1>(tw, (uint32_t)p, LOGN),
+                              (T)-0.5, va, vb);
+                cf2_fwd_out(tw, va, p, LOGN, os, N);
+                if (L - p != p) cf2_fwd_out(tw, vb, L - p, LOGN, os, N);
+            }
+        }
+    } else {
+        for (int q = tid; q < TI * NP; q += NT) {
+            const int t = q / NP, p = q % NP;
+            const bool ok = TI == 1 || t < nvalid;
+            const T* ys = yb + t * N;
+            if (p == 0) {
+                const T sqrt2 = (T)1.41421356237309504880;
+                lds[E::lidx(t, 0)] = ok ? dc_inverse(cpx<T>{ys[0], sqrt2 * ys[L]}) : cpx<T>{(T)0, (T)0};
+            } else {
+                cpx<T> za{(T)0, (T)0}, zb{(T)0, (T)0};
+                if (ok)
+                    untangle_pair(cf2_inv_in(tw, ys, p, LOGN, N), cf2_inv_in(tw, ys, L - p, LOGN, N),
+                                  twiddle_big<-1>(tw, (uint32_t)p, LOGN), (T)0.5, za, zb);
+                lds[E::lidx(t, p)] = za;
+                lds[E::lidx(t, L - p)] = zb;
+            }
+        }
+        __syncthreads();
+        E::run_from_lds(tid, lds, tw.stage, [&](int t, int m, cpx<T> v) {
+            if (TI == 1 || t < nvalid) {
+                T* os = out + (s0 + t) * N;
+                os[cf2_pos(2 * m, L)] = v.x;
+                os[cf2_pos(2 * m + 1, L)] = v.y;
+            }
+        });
+    }
+}
+
+// cosft2 with n = 2 (no FFT): Y0 = y0 + y1, Y1 = (y0 - y1) cos(pi/4); the
+// inverse y_j = Y0 / 2 +- Y1 cos(pi/4).
+template <typename T, int DIR>
+__global__ void k_cosft2_n2(const T* y, T* out, int64_t nsig) {
+    const int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (s >= nsig) return;
+    const T a = y[2 * s], b = y[2 * s + 1];
+    const T c = cos_quarter_pi<T>();
+    if (DIR > 0) {
+        out[2 * s] = a + b;
+        out[2 * s + 1] = (a - b) * c;
+    } else {
+        out[2 * s] = (T)0.5 * a + b * c;
+        out[2 * s + 1] = (T)0.5 * a - b * c;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Multi-pass kernels (transforms longer than one tile): pack -> C2C plan ->
+// finish, one thread per element or bin pair.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_twofft_pack(const T* d1, const T* d2, cpx<T>* z, int64_t total) {
+    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < total; q += int64_t(gridDim.x) * blockDim.x)
+        z[q] = cpx<T>{d1[q], d2[q]};
+}
+
+// In place on f1 (which holds Z), second spectrum to f2.
+template <typename T>
+__global__ void k_twofft_split(cpx<T>* f1, cpx<T>* f2, int64_t n, int64_t nsig) {
+    const int64_t np = n / 2 + 1;
+    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < np * nsig;
+         q += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t s = q / np, k = q % np;
+        const int64_t r = (n - k) & (n - 1);
+        cpx<T>* o1 = f1 + s * n;
+        cpx<T>* o2 = f2 + s * n;
+        cpx<T> g1, g2;
+        twofft_pair(o1[k], o1[r], g1, g2);
+        o1[k] = g1;
+        o2[k] = g2;
+        if (r != k) {
+            o1[r] = conj(g1);
+            o2[r] = conj(g2);
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_cosft1_pack(const T* y, cpx<T>* z, int64_t L, int64_t nsig) {
+    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < L * nsig;
+         q += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t s = q / L, m = q % L;
+        const T* ys = y + s * (L + 1);
+        z[q] = cpx<T>{cf1_ext(ys, 2 * m, L), cf1_ext(ys, 2 * m + 1, L)};
+    }
+}
+
+template <typename T>
+__global__ void k_cosft1_post(const cpx<T>* z, T* out, int64_t L, int64_t nsig, int log2n2, Twiddles<T> tw) {
+    const int64_t np = L / 2 + 1;
+    const T half = (T)0.5;
+    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < np * nsig;
+         q += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t s = q / np, p = q % np;
+        const cpx<T>* zs = z + s * L;
+        T* os = out + s * (L + 1);
+        if (p == 0) {
+            os[0] = half * (zs[0].x + zs[0].y);
+            os[L] = half * (zs[0].x - zs[0].y);
+        } else {
+            cpx<T> fa, fb;
+            untangle_pair(zs[p], zs[L - p], twiddle_big<1>(tw, (uint32_t)p, log2n2), (T)-0.5, fa, fb);
+            os[p] = half * fa.x;
+            if (L - p != p) os[L - p] = half * fb.x;
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_cosft2_fpack(const T* y, cpx<T>* z, int64_t L, int64_t nsig) {
+    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < L * nsig;
+         q += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t s = q / L, m = q % L;
+        const T* ys = y + s * 2 * L;
+        z[q] = cpx<T>{ys[cf2_pos(2 * m, L)], ys[cf2_pos(2 * m + 1, L)]};
+    }
+}
+
+template <typename T>
+__global__ void k_cosft2_fpost(const cpx<T>* z, T* out, int64_t L, int64_t nsig, int logn, Twiddles<T> tw) {
+    const int64_t np = L / 2 + 1, n = 2 * L;
+    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < np * nsig;
+         q += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t s = q / np, p = q % np;
+        const cpx<T>* zs = z + s * L;
+        T* os = out + s * n;
+        if (p == 0) {
+            os[0] = zs[0].x + zs[0].y;
+            os[L] = (zs[0].x - zs[0].y) * cos_quarter_pi<T>();
+        } else {
+            cpx<T> va, vb;
+            untangle_pair(zs[p], zs[L - p], twiddle_big<1>(tw, (uint32_t)p, logn), (T)-0.5, va, vb);
+            cf2_fwd_out(tw, va, p, logn, os, n);
+            if (L - p != p) cf2_fwd_out(tw, vb, L - p, logn, os, n);
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_cosft2_ipre(const T* y, cpx<T>* z, int64_t L, int64_t nsig, int logn, Twiddles<T> tw) {
+    const int64_t np = L / 2 + 1, n = 2 * L;
+    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < np * nsig;
+         q += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t s = q / np, p = q % np;
+        const T* ys = y + s * n;
+        cpx<T>* zs = z + s * L;
+        if (p == 0) {
+            zs[0] = dc_inverse(cpx<T>{ys[0], (T)1.41421356237309504880 * ys[L]});
+        } else {
+            cpx<T> za, zb;
+            untangle_pair(cf2_inv_in(tw, ys, p, logn, n), cf2_inv_in(tw, ys, L - p, logn, n),
+                          twiddle_big<-1>(tw, (uint32_t)p, logn), (T)0.5, za, zb);
+            zs[p] = za;
+            zs[L - p] = zb;
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_cosft2_iunpack(const cpx<T>* z, T* out, int64_t L, int64_t nsig) {
+    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < L * nsig;
+         q += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t s = q / L, m = q % L;
+        T* os = out + s * 2 * L;
+        os[cf2_pos(2 * m, L)] = z[q].x;
+        os[cf2_pos(2 * m + 1, L)] = z[q].y;
+    }
+}
+
+// rlft3: speq[s][i1][i2] = data[s][i1][i2][0] (complex; forward, after fourn).
+template <typename T>
+__global__ void k_rlft3_copycol(const cpx<T>* data, cpx<T>* speq, int64_t plane, int64_t h, int64_t nsig) {
+    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < plane * nsig;
+         q += int64_t(gridDim.x) * blockDim.x)
+        speq[q] = data[q * h];
+}
+
+// rlft3 untangle, one thread per (s, i1, i2, k3), k3 < max(1, h/2).
+template <typename T, int DIR>
+__global__ void k_rlft3_untangle(cpx<T>* data, cpx<T>* speq, int64_t nn1, int64_t nn2, int64_t h, int64_t nsig,
+                                 int lognn3, Twiddles<T> tw) {
+    const int64_t nk = h > 1 ? h / 2 : 1;
+    const int64_t plane = nn1 * nn2;
+    const int64_t total = plane * nk * nsig;
+    const T c2 = (T)(-0.5 * DIR);
+    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < total; q += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t k3 = q % nk;
+        const int64_t r = q / nk;
+        const int64_t i2 = r % nn2, i1 = (r / nn2) % nn1, s = r / plane;
+        const int64_t j1 = (nn1 - i1) & (nn1 - 1), j2 = (nn2 - i2) & (nn2 - 1);
+        cpx<T>* ds = data + s * plane * h;
+        cpx<T>* ss = speq + s * plane;
+        cpx<T>& a = ds[(i1 * nn2 + i2) * h + k3];
+        cpx<T>& b = k3 == 0 ? ss[j1 * nn2 + j2] : ds[(j1 * nn2 + j2) * h + (h - k3)];
+        const cpx<T> w = k3 == 0 ? cpx<T>{(T)1, (T)0} : twiddle_big<DIR>(tw, (uint32_t)k3, lognn3);
+        cpx<T> oa, ob;
+        untangle_pair(a, b, w, c2, oa, ob);
+        a = oa;
+        b = ob;
+    }
+}
+
+}  // namespace nrfft

@@ -1,0 +1,467 @@
+// spectral_api.hip -- twofft / cosft1 / cosft2 / rlft3: planning, execution
+// and the extern "C" entry points of include/nrfft.h.
+#include <string>
+#include <vector>
+
+#include "../../include/nrfft.h"
+#include "internal.hpp"
+#include "spectral_kern.hpp"
+
+namespace nrfft {
+
+#define HIPCHK(x)                                                                                      \
+    do {                                                                                               \
+        hipError_t e_ = (x);                                                                           \
+        if (e_ != hipSuccess) return fail(NRFFT_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int COSFT_MAX_LOGN = 24;   // cosft2's exp(i pi k / 2n) comes from the 2^26-point table
+constexpr int TWOFFT_MAX_LOGN = 26;  // the C2C plans' limit
+
+enum SpecSpecial {
+    SP_TW_FUSED = 1,  // args: nsig, n
+    SP_TW_PACK,       // args: nsig, n
+    SP_TW_SPLIT,      // args: nsig, n
+    SP_CF1_FUSED,     // args: nsig, n
+    SP_CF1_PACK,      // args: nsig, n
+    SP_CF1_POST,      // args: nsig, n
+    SP_CF2_FUSED,     // args: nsig, L
+    SP_CF2_N2,        // args: nsig
+    SP_CF2_FPACK,     // args: nsig, L
+    SP_CF2_FPOST,     // args: nsig, L
+    SP_CF2_IPRE,      // args: nsig, L
+    SP_CF2_IUNPACK,   // args: nsig, L
+    SP_R3_COPY,       // args: nsig, plane, h
+    SP_R3_UNTANGLE,   // args: nsig, nn1, nn2, h, log2 nn3
+};
+
+static Launch spec_launch(int special, int src, int dst, std::initializer_list<int64_t> args) {
+    Launch l;
+    l.kind = LK_SPEC;
+    l.special = special;
+    l.src = src;
+    l.dst = dst;
+    int i = 0;
+    for (int64_t a : args) l.args[i++] = a;
+    return l;
+}
+
+static const SpecRef& spec_lookup(int dtype, int logl) {
+    return dtype == NRFFT_F32 ? spec_lookup_f32(logl) : spec_lookup_f64(logl);
+}
+
+// C2C launches appended from index `from` run with isign `force` and only for
+// calls of direction `only` (0 = both).
+static void tag_launches(Plan& p, size_t from, int force, int only) {
+    for (size_t i = from; i < p.launches.size(); ++i) {
+        if (force) p.launches[i].args[6] = force;
+        if (only) p.launches[i].args[7] = only;
+    }
+}
+
+int build_plan_spectral(Plan& p) {
+    const int64_t batch = (int64_t)p.batch;
+    size_t tmp = 0;
+    int rc = NRFFT_OK;
+    switch (p.kind) {
+        case PK_TWOFFT: {
+            const int64_t n = (int64_t)p.nn.at(0);
+            const int logn = ilog2((size_t)n);
+            if (logn <= SPEC_MAX_LOGL) {
+                p.launches.push_back(spec_launch(SP_TW_FUSED, B_IN, B_OUT, {batch, n}));
+                return NRFFT_OK;
+            }
+            p.launches.push_back(spec_launch(SP_TW_PACK, B_IN, B_OUT, {batch, n}));
+            const size_t from = p.launches.size();
+            if ((rc = plan_c2c_1d(p, logn, batch, B_OUT, B_OUT, &tmp))) return rc;
+            tag_launches(p, from, 1, 0);
+            p.launches.push_back(spec_launch(SP_TW_SPLIT, B_OUT, B_AUX2, {batch, n}));
+            p.tmp_elems = tmp;
+            return NRFFT_OK;
+        }
+        case PK_COSFT1: {
+            const int64_t n = (int64_t)p.nn.at(0);
+            const int logn = ilog2((size_t)n);
+            if (logn <= SPEC_MAX_LOGL) {
+                p.launches.push_back(spec_launch(SP_CF1_FUSED, B_IN, B_OUT, {batch, n}));
+                return NRFFT_OK;
+            }
+            p.launches.push_back(spec_launch(SP_CF1_PACK, B_IN, B_TMP2, {batch, n}));
+            const size_t from = p.launches.size();
+            if ((rc = plan_c2c_1d(p, logn, batch, B_TMP2, B_TMP2, &tmp))) return rc;
+            tag_launches(p, from, 1, 0);
+            p.launches.push_back(spec_launch(SP_CF1_POST, B_TMP2, B_OUT, {batch, n}));
+            p.tmp2_offset = tmp;
+            p.tmp_elems = tmp + size_t(batch) * size_t(n);
+            return NRFFT_OK;
+        }
+        case PK_COSFT2: {
+            const int64_t n = (int64_t)p.nn.at(0), L = n / 2;
+            if (n == 2) {
+                p.launches.push_back(spec_launch(SP_CF2_N2, B_IN, B_OUT, {batch}));
+                return NRFFT_OK;
+            }
+            const int logl = ilog2((size_t)L);
+            if (logl <= SPEC_MAX_LOGL) {
+                p.launches.push_back(spec_launch(SP_CF2_FUSED, B_IN, B_OUT, {batch, L}));
+                return NRFFT_OK;
+            }
+            // forward: permute-pack -> C2C(+1) -> untangle and rotate
+            Launch a = spec_launch(SP_CF2_FPACK, B_IN, B_TMP2, {batch, L});
+            a.args[7] = 1;
+            p.launches.push_back(a);
+            size_t from = p.launches.size();
+            if ((rc = plan_c2c_1d(p, logl, batch, B_TMP2, B_TMP2, &tmp))) return rc;
+            tag_launches(p, from, 1, 1);
+            Launch b = spec_launch(SP_CF2_FPOST, B_TMP2, B_OUT, {batch, L});
+            b.args[7] = 1;
+            p.launches.push_back(b);
+            // inverse: rotate and untangle -> C2C(-1) -> unpermute
+            Launch c = spec_launch(SP_CF2_IPRE, B_IN, B_TMP2, {batch, L});
+            c.args[7] = -1;
+            p.launches.push_back(c);
+            from = p.launches.size();
+            if ((rc = plan_c2c_1d(p, logl, batch, B_TMP2, B_TMP2, &tmp))) return rc;
+            tag_launches(p, from, -1, -1);
+            Launch d = spec_launch(SP_CF2_IUNPACK, B_TMP2, B_OUT, {batch, L});
+            d.args[7] = -1;
+            p.launches.push_back(d);
+            p.tmp2_offset = tmp;
+            p.tmp_elems = tmp + size_t(batch) * size_t(L);
+            return NRFFT_OK;
+        }
+        case PK_RLFT3: {
+            const int64_t nn1 = (int64_t)p.nn.at(0), nn2 = (int64_t)p.nn.at(1), nn3 = (int64_t)p.nn.at(2);
+            const int64_t h = nn3 / 2;
+            std::vector<size_t> dims;
+            for (int64_t v : {nn1, nn2, h})
+                if (v > 1) dims.push_back((size_t)v);
+            // forward: fourn(+1) on the data (in place) -> speq <- k3 = 0 column -> untangle
+            size_t from = p.launches.size();
+            if (!dims.empty() && (rc = plan_c2c_nd(p, dims, batch))) return rc;
+            tag_launches(p, from, 1, 1);
+            Launch a = spec_launch(SP_R3_COPY, B_OUT, B_AUX2, {batch, nn1 * nn2, h});
+            a.args[7] = 1;
+            p.launches.push_back(a);
+            Launch b = spec_launch(SP_R3_UNTANGLE, B_OUT, B_OUT, {batch, nn1, nn2, h, ilog2((size_t)nn3)});
+            b.args[7] = 1;
+            p.launches.push_back(b);
+            // inverse: untangle -> fourn(-1)
+            Launch c = spec_launch(SP_R3_UNTANGLE, B_OUT, B_OUT, {batch, nn1, nn2, h, ilog2((size_t)nn3)});
+            c.args[7] = -1;
+            p.launches.push_back(c);
+            from = p.launches.size();
+            if (!dims.empty() && (rc = plan_c2c_nd(p, dims, batch))) return rc;
+            tag_launches(p, from, -1, -1);
+            return NRFFT_OK;
+        }
+        default:
+            return fail(NRFFT_E_INVALID, "not a spectral plan");
+    }
+}
+
+static unsigned grid_of(int64_t work, int nt) {
+    int64_t g = (work + nt - 1) / nt;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
+template <typename T>
+static int exec_spec_t(const Plan& p, const Launch& l, void* const* bufs, int isign, hipStream_t s, DeviceCtx* ctx) {
+    Twiddles<T> tw = ctx_twiddles<T>(ctx);
+    void* src = bufs[l.src];
+    void* dst = bufs[l.dst];
+    const int64_t* a = l.args;
+    const int64_t nsig = a[0];
+    switch (l.special) {
+        case SP_TW_FUSED: {
+            const SpecRef& r = spec_lookup(p.dtype, ilog2((size_t)a[1]));
+            void* d2 = bufs[B_AUX];
+            void* f2 = bufs[B_AUX2];
+            int64_t ns = nsig;
+            void* args[] = {&src, &d2, &dst, &f2, &ns, &tw};
+            HIPCHK(hipLaunchKernel(r.twofft, dim3((unsigned)((nsig + r.ti - 1) / r.ti)), dim3(r.nt), args, 0, s));
+            return NRFFT_OK;
+        }
+        case SP_TW_PACK:
+            hipLaunchKernelGGL((k_twofft_pack<T>), dim3(grid_of(nsig * a[1], 256)), dim3(256), 0, s, (const T*)src,
+                               (const T*)bufs[B_AUX], (cpx<T>*)dst, nsig * a[1]);
+            break;
+        case SP_TW_SPLIT:
+            hipLaunchKernelGGL((k_twofft_split<T>), dim3(grid_of(nsig * (a[1] / 2 + 1), 256)), dim3(256), 0, s,
+                               (cpx<T>*)src, (cpx<T>*)dst, a[1], nsig);
+            break;
+        case SP_CF1_FUSED: {
+            const SpecRef& r = spec_lookup(p.dtype, ilog2((size_t)a[1]));
+            int64_t ns = nsig;
+            void* args[] = {&src, &dst, &ns, &tw};
+            HIPCHK(hipLaunchKernel(r.cosft1, dim3((unsigned)((nsig + r.ti - 1) / r.ti)), dim3(r.nt), args, 0, s));
+            return NRFFT_OK;
+        }
+        case SP_CF1_PACK:
+            hipLaunchKernelGGL((k_cosft1_pack<T>), dim3(grid_of(nsig * a[1], 256)), dim3(256), 0, s, (const T*)src,
+                               (cpx<T>*)dst, a[1], nsig);
+            break;
+        case SP_CF1_POST:
+            hipLaunchKernelGGL((k_cosft1_post<T>), dim3(grid_of(nsig * (a[1] / 2 + 1), 256)), dim3(256), 0, s,
+                               (const cpx<T>*)src, (T*)dst, a[1], nsig, ilog2((size_t)a[1]) + 1, tw);
+            break;
+        case SP_CF2_FUSED: {
+            const SpecRef& r = spec_lookup(p.dtype, ilog2((size_t)a[1]));
+            int64_t ns = nsig;
+            void* args[] = {&src, &dst, &ns, &tw};
+            HIPCHK(hipLaunchKernel(r.cosft2[isign > 0 ? 0 : 1], dim3((unsigned)((nsig + r.ti - 1) / r.ti)),
+                                   dim3(r.nt), args, 0, s));
+            return NRFFT_OK;
+        }
+        case SP_CF2_N2:
+            if (isign > 0)
+                hipLaunchKernelGGL((k_cosft2_n2<T, 1>), dim3(grid_of(nsig, 256)), dim3(256), 0, s, (const T*)src,
+                                   (T*)dst, nsig);
+            else
+                hipLaunchKernelGGL((k_cosft2_n2<T, -1>), dim3(grid_of(nsig, 256)), dim3(256), 0, s, (const T*)src,
+                                   (T*)dst, nsig);
+            break;
+        case SP_CF2_FPACK:
+            hipLaunchKernelGGL((k_cosft2_fpack<T>), dim3(grid_of(nsig * a[1], 256)), dim3(256), 0, s, (const T*)src,
+                               (cpx<T>*)dst, a[1], nsig);
+            break;
+        case SP_CF2_FPOST:
+            hipLaunchKernelGGL((k_cosft2_fpost<T>), dim3(grid_of(nsig * (a[1] / 2 + 1), 256)), dim3(256), 0, s,
+                               (const cpx<T>*)src, (T*)dst, a[1], nsig, ilog2((size_t)a[1]) + 1, tw);
+            break;
+        case SP_CF2_IPRE:
+            hipLaunchKernelGGL((k_cosft2_ipre<T>), dim3(grid_of(nsig * (a[1] / 2 + 1), 256)), dim3(256), 0, s,
+                               (const T*)src, (cpx<T>*)dst, a[1], nsig, ilog2((size_t)a[1]) + 1, tw);
+            break;
+        case SP_CF2_IUNPACK:
+            hipLaunchKernelGGL((k_cosft2_iunpack<T>), dim3(grid_of(nsig * a[1], 256)), dim3(256), 0, s,
+                               (const cpx<T>*)src, (T*)dst, a[1], nsig);
+            break;
+        case SP_R3_COPY:
+            hipLaunchKernelGGL((k_rlft3_copycol<T>), dim3(grid_of(nsig * a[1], 256)), dim3(256), 0, s,
+                               (const cpx<T>*)src, (cpx<T>*)dst, a[1], a[2], nsig);
+            break;
+        case SP_R3_UNTANGLE: {
+            const int64_t nk = a[3] > 1 ? a[3] / 2 : 1;
+            const unsigned g = grid_of(nsig * a[1] * a[2] * nk, 256);
+            cpx<T>* speq = (cpx<T>*)bufs[B_AUX2];
+            if (isign > 0)
+                hipLaunchKernelGGL((k_rlft3_untangle<T, 1>), dim3(g), dim3(256), 0, s, (cpx<T>*)dst, speq, a[1], a[2],
+                                   a[3], nsig, (int)a[4], tw);
+            else
+                hipLaunchKernelGGL((k_rlft3_untangle<T, -1>), dim3(g), dim3(256), 0, s, (cpx<T>*)dst, speq, a[1],
+                                   a[2], a[3], nsig, (int)a[4], tw);
+            break;
+        }
+        default:
+            return fail(NRFFT_E_INVALID, "unknown spectral launch");
+    }
+    HIPCHK(hipGetLastError());
+    return NRFFT_OK;
+}
+
+int exec_spectral(const Plan& p, const Launch& l, void* const* bufs, int isign, hipStream_t stream, DeviceCtx* ctx) {
+    if (p.dtype == NRFFT_F32) return exec_spec_t<float>(p, l, bufs, isign, stream, ctx);
+    return exec_spec_t<double>(p, l, bufs, isign, stream, ctx);
+}
+
+// ---------------------------------------------------------------------------
+// Validation (reference order first, then this build's power-of-two limits).
+// ---------------------------------------------------------------------------
+static int check_twofft(size_t n) {
+    if (n == 0) return fail(NRFFT_E_EMPTY, "Input arrays cannot be empty");
+    if (!is_pow2(n) || n < 2) return fail(NRFFT_E_NOT_POW2, "n must be a power of two >= 2");
+    if (ilog2(n) > TWOFFT_MAX_LOGN) return fail(NRFFT_E_UNSUPPORTED, "n > 2^26");
+    return NRFFT_OK;
+}
+
+static int check_cosft(size_t n) {
+    if (n == 0) return fail(NRFFT_E_EMPTY, "n == 0");
+    if (!is_pow2(n) || n < 2) return fail(NRFFT_E_NOT_POW2, "n must be a power of two >= 2");
+    if (ilog2(n) > COSFT_MAX_LOGN) return fail(NRFFT_E_UNSUPPORTED, "n > 2^24");
+    return NRFFT_OK;
+}
+
+// Real_FT3.rs:16-19 (isign, then the shapes); dims powers of two, nn3 >= 2.
+static int check_rlft3(size_t nn1, size_t nn2, size_t nn3, int isign) {
+    if (isign != 1 && isign != -1) return fail(NRFFT_E_ISIGN, "isign must be 1 or -1");
+    if (nn1 == 0 || nn2 == 0 || nn3 == 0) return fail(NRFFT_E_EMPTY, "empty dimension");
+    if (!is_pow2(nn1) || !is_pow2(nn2) || !is_pow2(nn3) || nn3 < 2)
+        return fail(NRFFT_E_NOT_POW2, "dimensions must be powers of two, nn3 >= 2");
+    return NRFFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Host-pointer calls: stage through HBM on the per-thread stream.
+// ---------------------------------------------------------------------------
+struct HostBuf {
+    const void* src;  // copied in (nullptr: not an input)
+    void* dst;        // copied out (nullptr: not an output)
+    size_t bytes;
+};
+
+static int host_spectral(int kind, int dtype, std::vector<size_t> nn, HostBuf in, HostBuf aux, HostBuf out,
+                         HostBuf aux2, int isign) {
+    std::shared_ptr<Plan> p;
+    int rc = cached_plan(kind, dtype, nn, 1, 0, &p);
+    if (rc) return rc;
+    hipStream_t s = hipStreamPerThread;
+    DevBuf dbin, daux, dout, daux2, dtmp;
+    // `out` may be the same host buffer as `in` (in-place calls): then one
+    // device buffer serves both.
+    const bool same = out.dst != nullptr && out.dst == in.src;
+    if ((rc = dev_alloc(dbin, in.bytes, s))) return rc;
+    if (!same && out.bytes && (rc = dev_alloc(dout, out.bytes, s))) return rc;
+    if (aux.bytes && (rc = dev_alloc(daux, aux.bytes, s))) return rc;
+    if (aux2.bytes && (rc = dev_alloc(daux2, aux2.bytes, s))) return rc;
+    if ((rc = dev_alloc(dtmp, p->tmp_elems * csize(dtype), s))) return rc;
+    void* dout_p = same ? dbin.p : dout.p;
+    HIPCHK(hipMemcpyAsync(dbin.p, in.src, in.bytes, hipMemcpyHostToDevice, s));
+    if (aux.src) HIPCHK(hipMemcpyAsync(daux.p, aux.src, aux.bytes, hipMemcpyHostToDevice, s));
+    if (aux2.src) HIPCHK(hipMemcpyAsync(daux2.p, aux2.src, aux2.bytes, hipMemcpyHostToDevice, s));
+    if ((rc = exec_plan(*p, dbin.p, dout_p, dtmp.p, daux.p, isign, s, 0, -1, daux2.p))) return rc;
+    if (out.dst) HIPCHK(hipMemcpyAsync(out.dst, dout_p, out.bytes, hipMemcpyDeviceToHost, s));
+    if (aux2.dst) HIPCHK(hipMemcpyAsync(aux2.dst, daux2.p, aux2.bytes, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return NRFFT_OK;
+}
+
+template <typename T>
+static int host_twofft(const T* d1, const T* d2, size_t n, T* fft1, T* fft2, int dtype) {
+    if (!d1 || !d2 || !fft1 || !fft2) return fail(NRFFT_E_INVALID, "null pointer");
+    int rc = check_twofft(n);
+    if (rc) return rc;
+    const size_t ib = n * sizeof(T), ob = 2 * n * sizeof(T);
+    rc = host_spectral(PK_TWOFFT, dtype, {n}, HostBuf{d1, nullptr, ib}, HostBuf{d2, nullptr, ib},
+                       HostBuf{nullptr, fft1, ob}, HostBuf{nullptr, fft2, ob}, 1);
+    if (rc) return rc;
+    // FFT_2.rs:6-7 buffers hold 2n + 2 values; bins 0..n-1 fill the first 2n.
+    fft1[2 * n] = fft1[2 * n + 1] = (T)0;
+    fft2[2 * n] = fft2[2 * n + 1] = (T)0;
+    return NRFFT_OK;
+}
+
+template <typename T>
+static int host_cosft1(T* y, size_t n, int dtype) {
+    if (!y) return fail(NRFFT_E_INVALID, "null pointer");
+    int rc = check_cosft(n);
+    if (rc) return rc;
+    const size_t b = (n + 1) * sizeof(T);
+    return host_spectral(PK_COSFT1, dtype, {n}, HostBuf{y, nullptr, b}, HostBuf{nullptr, nullptr, 0},
+                         HostBuf{nullptr, y, b}, HostBuf{nullptr, nullptr, 0}, 1);
+}
+
+template <typename T>
+static int host_cosft2(T* y, size_t n, int isign, int dtype) {
+    if (isign != 1 && isign != -1) return fail(NRFFT_E_ISIGN, "Invalid isign value: must be 1 or -1");
+    if (!y) return fail(NRFFT_E_INVALID, "null pointer");
+    int rc = check_cosft(n);
+    if (rc) return rc;
+    const size_t b = n * sizeof(T);
+    return host_spectral(PK_COSFT2, dtype, {n}, HostBuf{y, nullptr, b}, HostBuf{nullptr, nullptr, 0},
+                         HostBuf{nullptr, y, b}, HostBuf{nullptr, nullptr, 0}, isign);
+}
+
+template <typename T>
+static int host_rlft3(T* data, T* speq, size_t nn1, size_t nn2, size_t nn3, int isign, int dtype) {
+    int rc = check_rlft3(nn1, nn2, nn3, isign);
+    if (rc) return rc;
+    if (!data || !speq) return fail(NRFFT_E_INVALID, "null pointer");
+    const size_t db = nn1 * nn2 * nn3 * sizeof(T), sb = nn1 * 2 * nn2 * sizeof(T);
+    return host_spectral(PK_RLFT3, dtype, {nn1, nn2, nn3}, HostBuf{data, nullptr, db}, HostBuf{nullptr, nullptr, 0},
+                         HostBuf{nullptr, data, db}, HostBuf{isign < 0 ? speq : nullptr, speq, sb}, isign);
+}
+
+}  // namespace nrfft
+
+using namespace nrfft;
+
+extern "C" {
+
+int nrfft_twofft_f64(const double* data1, const double* data2, size_t n, double* fft1, double* fft2) {
+    g_err.clear();
+    return host_twofft<double>(data1, data2, n, fft1, fft2, NRFFT_F64);
+}
+int nrfft_twofft_f32(const float* data1, const float* data2, size_t n, float* fft1, float* fft2) {
+    g_err.clear();
+    return host_twofft<float>(data1, data2, n, fft1, fft2, NRFFT_F32);
+}
+int nrfft_cosft1_f64(double* y, size_t n) {
+    g_err.clear();
+    return host_cosft1<double>(y, n, NRFFT_F64);
+}
+int nrfft_cosft1_f32(float* y, size_t n) {
+    g_err.clear();
+    return host_cosft1<float>(y, n, NRFFT_F32);
+}
+int nrfft_cosft2_f64(double* y, size_t n, int isign) {
+    g_err.clear();
+    return host_cosft2<double>(y, n, isign, NRFFT_F64);
+}
+int nrfft_cosft2_f32(float* y, size_t n, int isign) {
+    g_err.clear();
+    return host_cosft2<float>(y, n, isign, NRFFT_F32);
+}
+int nrfft_rlft3_f64(double* data, double* speq, size_t nn1, size_t nn2, size_t nn3, int isign) {
+    g_err.clear();
+    return host_rlft3<double>(data, speq, nn1, nn2, nn3, isign, NRFFT_F64);
+}
+int nrfft_rlft3_f32(float* data, float* speq, size_t nn1, size_t nn2, size_t nn3, int isign) {
+    g_err.clear();
+    return host_rlft3<float>(data, speq, nn1, nn2, nn3, isign, NRFFT_F32);
+}
+
+int nrfft_plan_twofft(nrfft_plan* plan, int dtype, size_t n, size_t batch) {
+    g_err.clear();
+    int rc = check_twofft(n);
+    return rc ? rc : make_plan(plan, PK_TWOFFT, dtype, {n}, batch, 0);
+}
+int nrfft_execute_twofft(nrfft_plan plan, const void* d_data1, const void* d_data2, void* d_fft1, void* d_fft2,
+                         void* stream) {
+    g_err.clear();
+    if (!plan || plan->plan.kind != PK_TWOFFT) return fail(NRFFT_E_INVALID, "not a twofft plan");
+    if (!d_data1 || !d_data2 || !d_fft1 || !d_fft2) return fail(NRFFT_E_INVALID, "null buffer");
+    return exec_plan(plan->plan, d_data1, d_fft1, plan->tmp, d_data2, 1, (hipStream_t)stream, 0, -1, d_fft2);
+}
+
+int nrfft_plan_cosft1(nrfft_plan* plan, int dtype, size_t n, size_t batch) {
+    g_err.clear();
+    int rc = check_cosft(n);
+    return rc ? rc : make_plan(plan, PK_COSFT1, dtype, {n}, batch, 0);
+}
+int nrfft_execute_cosft1(nrfft_plan plan, const void* d_in, void* d_out, void* stream) {
+    g_err.clear();
+    if (!plan || plan->plan.kind != PK_COSFT1) return fail(NRFFT_E_INVALID, "not a cosft1 plan");
+    if (!d_in || !d_out) return fail(NRFFT_E_INVALID, "null buffer");
+    return exec_plan(plan->plan, d_in, d_out, plan->tmp, nullptr, 1, (hipStream_t)stream);
+}
+
+int nrfft_plan_cosft2(nrfft_plan* plan, int dtype, size_t n, size_t batch) {
+    g_err.clear();
+    int rc = check_cosft(n);
+    return rc ? rc : make_plan(plan, PK_COSFT2, dtype, {n}, batch, 0);
+}
+int nrfft_execute_cosft2(nrfft_plan plan, const void* d_in, void* d_out, int isign, void* stream) {
+    g_err.clear();
+    if (!plan || plan->plan.kind != PK_COSFT2) return fail(NRFFT_E_INVALID, "not a cosft2 plan");
+    if (isign != 1 && isign != -1) return fail(NRFFT_E_ISIGN, "Invalid isign value: must be 1 or -1");
+    if (!d_in || !d_out) return fail(NRFFT_E_INVALID, "null buffer");
+    return exec_plan(plan->plan, d_in, d_out, plan->tmp, nullptr, isign, (hipStream_t)stream);
+}
+
+int nrfft_plan_rlft3(nrfft_plan* plan, int dtype, size_t nn1, size_t nn2, size_t nn3, size_t batch) {
+    g_err.clear();
+    int rc = check_rlft3(nn1, nn2, nn3, 1);
+    return rc ? rc : make_plan(plan, PK_RLFT3, dtype, {nn1, nn2, nn3}, batch, 0);
+}
+int nrfft_execute_rlft3(nrfft_plan plan, void* d_data, void* d_speq, int isign, void* stream) {
+    g_err.clear();
+    if (!plan || plan->plan.kind != PK_RLFT3) return fail(NRFFT_E_INVALID, "not an rlft3 plan");
+    if (isign != 1 && isign != -1) return fail(NRFFT_E_ISIGN, "isign must be 1 or -1");
+    if (!d_data || !d_speq) return fail(NRFFT_E_INVALID, "null buffer");
+    return exec_plan(plan->plan, d_data, d_data, plan->tmp, nullptr, isign, (hipStream_t)stream, 0, -1, d_speq);
+}
+
+}  // extern "C"

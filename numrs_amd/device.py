@@ -163,6 +163,56 @@ class CorrelPlan(_Plan):
         _check(_lib.lib().nrfft_execute_correl(self._h, _ptr(d1), _ptr(d2), _ptr(ans), _stream(stream)))
 
 
+class TwofftPlan(_Plan):
+    """Batched twofft: batch pairs of real n-point sequences -> two spectra of
+    n complex values each (2n reals per transform)."""
+
+    def __init__(self, dtype, n: int, batch: int = 1):
+        super().__init__()
+        _check(_lib.lib().nrfft_plan_twofft(ctypes.byref(self._h), _dtype_code(dtype), n, batch))
+        self.n, self.batch = n, batch
+
+    def execute(self, d1, d2, fft1, fft2, stream=None) -> None:
+        _check(_lib.lib().nrfft_execute_twofft(self._h, _ptr(d1), _ptr(d2), _ptr(fft1), _ptr(fft2),
+                                               _stream(stream)))
+
+
+class Cosft1Plan(_Plan):
+    """Batched cosft1 (DCT-I): n + 1 reals per transform."""
+
+    def __init__(self, dtype, n: int, batch: int = 1):
+        super().__init__()
+        _check(_lib.lib().nrfft_plan_cosft1(ctypes.byref(self._h), _dtype_code(dtype), n, batch))
+        self.n, self.batch = n, batch
+
+    def execute(self, x, y, stream=None) -> None:
+        _check(_lib.lib().nrfft_execute_cosft1(self._h, _ptr(x), _ptr(y), _stream(stream)))
+
+
+class Cosft2Plan(_Plan):
+    """Batched cosft2 (DCT-II forward, scaled DCT-III inverse): n reals per transform."""
+
+    def __init__(self, dtype, n: int, batch: int = 1):
+        super().__init__()
+        _check(_lib.lib().nrfft_plan_cosft2(ctypes.byref(self._h), _dtype_code(dtype), n, batch))
+        self.n, self.batch = n, batch
+
+    def execute(self, x, y, isign: int, stream=None) -> None:
+        _check(_lib.lib().nrfft_execute_cosft2(self._h, _ptr(x), _ptr(y), isign, _stream(stream)))
+
+
+class Rlft3Plan(_Plan):
+    """Batched rlft3: data nn1 x nn2 x nn3 reals (in place), speq 2 nn1 nn2 reals per transform."""
+
+    def __init__(self, dtype, nn1: int, nn2: int, nn3: int, batch: int = 1):
+        super().__init__()
+        _check(_lib.lib().nrfft_plan_rlft3(ctypes.byref(self._h), _dtype_code(dtype), nn1, nn2, nn3, batch))
+        self.shape, self.batch = (nn1, nn2, nn3), batch
+
+    def execute(self, data, speq, isign: int, stream=None) -> None:
+        _check(_lib.lib().nrfft_execute_rlft3(self._h, _ptr(data), _ptr(speq), isign, _stream(stream)))
+
+
 class UnevenPlan:
     """Uneven batch: items of different lengths / dimensionality in one
     persistent work-queue launch (replayed as a captured hipGraph).

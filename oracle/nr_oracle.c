@@ -596,6 +596,10 @@ void orc_twofft(const double *d1, const double *d2, size_t n, double *fft1, doub
  * slots.  Divergences of the Rust source corrected here: 1-based indices in a
  * 0-based slice (Cos_FT.rs:16-17), realft left unimplemented (Cos_FT.rs:71-75),
  * and the missing y[2] = sum step of NR after realft (Cos_FT.rs:58-64).
+ * Accuracy: the twiddles come from NR's trigonometric recurrence, which drifts
+ * by ~n eps: 3e-14 of the output at n = 2^14, 7e-13 at 2^17 and 2e-12 at 2^20
+ * (relative L2 against scipy's DCT).  cosft2 below drifts the same way
+ * (1e-12 at 2^17).
  * ------------------------------------------------------------------------ */
 void orc_cosft1(double *y, size_t n) {
     const double theta = ORC_PI / (double)n;

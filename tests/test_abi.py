@@ -120,3 +120,36 @@ def test_host_helpers():
     z = np.array([3.0, 4.0, 0.0, 1.0])
     assert np.allclose(numrs_amd.magnitude_spectrum(z), [5.0, 1.0])
     assert np.allclose(numrs_amd.power_spectrum(z), [25.0, 1.0])
+
+
+def test_spectral_validation():
+    # twofft / cosft1 / cosft2 / rlft3: argument checks that run before any
+    # device work (FFT_2.rs:5-7, Cos_FT2.rs:8-12, Real_FT3.rs:16-19).
+    L = _lib.lib()
+    d, _ = _d(8)
+    p = d.ctypes.data_as(_lib._dp)
+    assert L.nrfft_twofft_f64(p, p, 3, p, p) == _lib.E_NOT_POW2
+    assert L.nrfft_twofft_f64(p, p, 0, p, p) == _lib.E_EMPTY
+    assert L.nrfft_twofft_f64(None, p, 4, p, p) == _lib.E_INVALID
+    assert L.nrfft_cosft1_f64(p, 6) == _lib.E_NOT_POW2
+    assert L.nrfft_cosft1_f64(p, 1) == _lib.E_NOT_POW2
+    assert L.nrfft_cosft2_f64(p, 4, 0) == _lib.E_ISIGN
+    assert L.nrfft_cosft2_f64(p, 6, 1) == _lib.E_NOT_POW2
+    assert L.nrfft_rlft3_f64(p, p, 2, 2, 2, 2) == _lib.E_ISIGN
+    assert L.nrfft_rlft3_f64(p, p, 2, 3, 2, 1) == _lib.E_NOT_POW2
+    assert L.nrfft_rlft3_f64(p, p, 2, 2, 1, 1) == _lib.E_NOT_POW2
+    h = ctypes.c_void_p()
+    assert L.nrfft_plan_cosft2(ctypes.byref(h), 1, 12, 1) == _lib.E_NOT_POW2
+    assert L.nrfft_execute_rlft3(None, None, None, 1, None) == _lib.E_INVALID
+    with pytest.raises(AssertionError):
+        numrs_amd.twofft(np.zeros(8), np.zeros(4), np.zeros(18), np.zeros(18))   # FFT_2.rs:5
+    with pytest.raises(AssertionError):
+        numrs_amd.twofft(np.zeros(8), np.zeros(8), np.zeros(16), np.zeros(18))   # FFT_2.rs:6
+    with pytest.raises(ValueError, match="Invalid isign value"):
+        numrs_amd.cosft2(np.zeros(8), 8, 2)                                        # Cos_FT2.rs:11
+    with pytest.raises(AssertionError):
+        numrs_amd.rlft3(np.zeros((2, 2, 4)), np.zeros((2, 4)), 2, 2, 4, 0)         # Real_FT3.rs:16
+    with pytest.raises(AssertionError):
+        numrs_amd.rlft3(np.zeros((2, 2, 4)), np.zeros((2, 2)), 2, 2, 4, 1)         # Real_FT3.rs:18
+    assert numrs_amd.extract_real_imag(numrs_amd.combine_real_imag([1.0, 2.0, 3.0], [4.0, 5.0, 6.0]))[1].tolist() \
+        == [4.0, 5.0, 6.0]                                                         # FFT_2.rs:470-479
