@@ -14,7 +14,8 @@ The same line carries
                  cores on a bounded sample of the same workload;
   extra:         the other configurations (four1 2^16 FP64, fourn 256^3,
                  batched convlv 4096 x 2^14, the uneven batch of 10,000
-                 mixed 1-D/2-D transforms), each with its own GB/s figure.
+                 mixed 1-D/2-D transforms) and twofft / cosft1 / cosft2 /
+                 rlft3, each with its own GB/s figure.
 """
 from __future__ import annotations
 
@@ -263,6 +264,8 @@ def extra_configs(args, only: str = "all") -> dict:
         res.update(_extra_convlv())
     if only in ("all", "uneven"):
         res.update(_extra_uneven())
+    if only in ("all", "spectral"):
+        res.update(_extra_spectral())
     return res
 
 
@@ -368,6 +371,69 @@ def _extra_convlv() -> dict:
     return res
 
 
+def _extra_spectral() -> dict:
+    """twofft, cosft1, cosft2 and rlft3 (FP32, batched, device resident).
+    Bytes are compulsory: every input read once, every output written once."""
+    import torch
+
+    import numrs_amd
+
+    res = {}
+
+    def entry(ms, byt, flops=None):
+        e = {"ms": round(ms, 4), "GBps": round(byt / (ms * 1e-3) / 1e9, 1),
+             "frac_hbm": round(byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        if flops:
+            e["GFLOPs"] = round(flops / (ms * 1e-3) / 1e9, 1)
+        return e
+
+    f32 = torch.float32
+    for b, n in ((4096, 1 << 14), (64, 1 << 20)):
+        d1 = torch.empty(b * n, dtype=f32, device="cuda")
+        d2 = torch.empty_like(d1)
+        numrs_amd.gen_uniform(d1, 40)
+        numrs_amd.gen_uniform(d2, 41)
+        f1 = torch.empty(2 * b * n, dtype=f32, device="cuda")
+        f2 = torch.empty_like(f1)
+        p = numrs_amd.TwofftPlan("float32", n, b)
+        p.execute(d1, d2, f1, f2)
+        ms = time_events(lambda: p.execute(d1, d2, f1, f2), 10)
+        res[f"twofft_f32_{b}x2p{n.bit_length() - 1}"] = entry(
+            ms, (2 * n + 4 * n) * 4.0 * b, 5.0 * n * math.log2(n) * b)
+        del p, d1, d2, f1, f2
+    for b, n in ((4096, 1 << 14), (64, 1 << 20)):
+        x = torch.empty(b * (n + 1), dtype=f32, device="cuda")
+        numrs_amd.gen_uniform(x, 42)
+        y = torch.empty_like(x)
+        p = numrs_amd.Cosft1Plan("float32", n, b)
+        p.execute(x, y)
+        ms = time_events(lambda: p.execute(x, y), 10)
+        res[f"cosft1_f32_{b}x2p{n.bit_length() - 1}"] = entry(ms, 2.0 * (n + 1) * 4 * b)
+        del p, x, y
+    for b, n in ((4096, 1 << 14), (64, 1 << 20)):
+        x = torch.empty(b * n, dtype=f32, device="cuda")
+        numrs_amd.gen_uniform(x, 43)
+        y = torch.empty_like(x)
+        p = numrs_amd.Cosft2Plan("float32", n, b)
+        for isign, tag in ((1, "fwd"), (-1, "inv")):
+            p.execute(x, y, isign)
+            ms = time_events(lambda: p.execute(x, y, isign), 10)
+            res[f"cosft2_{tag}_f32_{b}x2p{n.bit_length() - 1}"] = entry(ms, 2.0 * n * 4 * b)
+        del p, x, y
+    shape = (256, 256, 256)
+    vol = 256 ** 3
+    x = torch.empty(vol, dtype=f32, device="cuda")
+    numrs_amd.gen_uniform(x, 44)
+    sp = torch.empty(2 * 256 * 256, dtype=f32, device="cuda")
+    p = numrs_amd.Rlft3Plan("float32", *shape, 1)
+    p.execute(x, sp, 1)
+    ms = time_events(lambda: p.execute(x, sp, 1), 10)
+    res["rlft3_f32_256cubed_fwd"] = entry(ms, 2.0 * vol * 4 + sp.numel() * 4, 2.5 * vol * 24)
+    del p, x, sp
+    torch.cuda.empty_cache()
+    return res
+
+
 def _segment_sums(v, lengths):
     """Sums of consecutive segments of v (lengths in elements), on the GPU,
     from differences of an FP64 prefix sum."""
@@ -448,7 +514,7 @@ def main() -> int:
     ap.add_argument("--cpu-reps", type=int, default=5, help="timed runs of the CPU sample (median kept)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--only", choices=["headline", "four1", "fourn", "convlv", "uneven"], default=None,
+    ap.add_argument("--only", choices=["headline", "four1", "fourn", "convlv", "uneven", "spectral"], default=None,
                     help="run one workload only (profiling runs); the JSON line then holds just that workload")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 512 MiB copy kernel (counter calibration for the traffic figure)")
