@@ -7,8 +7,9 @@ execution over the whole batch.  GFLOP/s uses 5 N log2 N per transform.
 
 The same line carries
   roofline:      the dominant kernel (slower four-step pass), algorithmic bytes
-                 per launch (one read + one write of the batch, 16 B per
-                 point) / its average duration from HIP events on its stream;
+                 per launch (one read + one write of its chunk, 16 B per
+                 point) / its average duration between HIP events recorded
+                 after every launch of the plan, in sequence, on its stream;
   cpu_baseline:  the oracle's restatement of FFT_1::four1 (FP64, OpenMP over
                  transforms like FFTProcessor::fft_batch) timed on the host
                  cores on a bounded sample of the same workload;
@@ -112,6 +113,24 @@ def _time_plan(plan, x, y, steps: int, warmup: int) -> float:
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
+def _time_launches_in_sequence(plan, x, y, steps: int) -> list:
+    """Average ms of each launch of the plan, measured between consecutive
+    HIP events in `steps` back-to-back executions."""
+    import torch
+
+    nl = plan.num_launches
+    s = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps * nl + 1)]
+    torch.cuda.synchronize()
+    ev[0].record(s)
+    for st in range(steps):
+        for i in range(nl):
+            plan.execute_launches(x, y, 1, i, 1)
+            ev[st * nl + i + 1].record(s)
+    torch.cuda.synchronize()
+    return [sum(ev[st * nl + i].elapsed_time(ev[st * nl + i + 1]) for st in range(steps)) / steps for i in range(nl)]
+
+
 def headline(args) -> dict:
     import torch
 
@@ -131,11 +150,14 @@ def headline(args) -> dict:
     flops = 5.0 * n * logn * batch
     gflops = flops / (ms * 1e-3) / 1e9
     compulsory = 2.0 * pts * 8  # one read + one write, 8 B per complex
-    # Per-launch durations (HIP events on the stream the library launches on).
-    # Four-step plans alternate pass A / pass B over batch chunks; each launch
-    # reads and writes every point of its chunk once.
-    reps = max(5, args.steps)
-    per = [time_events(lambda i=i: plan.execute_launches(x, y, 1, i, 1), reps) for i in range(nl)]
+    # Per-launch durations: the plan's launches issued one by one in their
+    # normal order for `steps` more steps, with a HIP event recorded after
+    # each on the stream the library launches on (torch's current stream), so
+    # every launch is timed inside the real sequence (the chunk written by
+    # pass A is what pass B reads).  Four-step plans alternate pass A / pass B
+    # over batch chunks; each launch reads and writes every point of its
+    # chunk once.
+    per = _time_launches_in_sequence(plan, x, y, args.steps)
     kinds = ["A", "B"] if nl % 2 == 0 and nl > 1 else ["pass"]
     groups = {k: [per[i] for i in range(nl) if kinds[i % len(kinds)] == k] for k in kinds}
     dom = max(groups, key=lambda k: sum(groups[k]))

@@ -8,19 +8,22 @@
 
 namespace nrfft {
 
-// Operand tables of F/4 (F_mk = exp(S 2 pi i mk / 16)) in the A layout of
-// v_mfma_f32_16x16x32: lane l, element e -> row m = l % 16, column
-// k = 8 (l / 16) + e of [Fr -Fi] (r) and [Fi Fr] (i); values computed in FP64
-// and split into FP16 hi/lo and BF16 hi/mid/lo parts.
+// Operand tables of F/4 (F_mp = exp(S 2 pi i mp / 16)) in the A layout of
+// v_mfma_f32_16x16x32, one table per output half h: lane l, element e -> row
+// q = l % 16 (output m = 8 h + q / 2, its real part for even q, imaginary for
+// odd q), column k = 8 (l / 16) + e = 2 p + c (point p, input part c):
+//   real rows [Re F_mp, -Im F_mp],  imaginary rows [Im F_mp, Re F_mp];
+// values computed in FP64 and split into FP16 hi/lo and BF16 hi/mid/lo parts.
 __global__ void k_mfma_tables(uint16_t* f16tab, uint16_t* bf16tab) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // (sign, ri, lane, e)
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // (sign, half, lane, e)
     if (idx >= 2 * 2 * 64 * 8) return;
     const int e = idx % 8, l = (idx / 8) % 64, ri = (idx / 512) % 2, si = idx / 1024;
     const double S = si ? -1.0 : 1.0;
-    const int m = l % 16, k = 8 * (l / 16) + e;
+    const int q = l % 16, m = 8 * ri + q / 2, k = 8 * (l / 16) + e, p = k / 2, c = k % 2;
+    const double fr = cospi(m * p / 8.0), fi = S * sinpi(m * p / 8.0);
     double v;
-    if (k < 16) v = ri == 0 ? cospi(m * k / 8.0) : S * sinpi(m * k / 8.0);
-    else v = ri == 0 ? -S * sinpi(m * (k - 16) / 8.0) : cospi(m * (k - 16) / 8.0);
+    if (q % 2 == 0) v = c == 0 ? fr : -fi;
+    else v = c == 0 ? fi : fr;
     v *= 0.25;
     // FP16 hi / lo
     const _Float16 h = (_Float16)v;
@@ -57,7 +60,9 @@ KernelRef mfma_ref_one() {
     if constexpr (MODE == MODE_TJ) {
         return KernelRef{nullptr, 0, 0, 0};
     } else {
-        return KernelRef{reinterpret_cast<const void*>(&k_c2c_mfma<PREC, S, L, L, ML, MS>), L, 16, 16 * L * 8};
+        using E = Engine<float, S, L, 16, L, ML, MS>;
+        return KernelRef{reinterpret_cast<const void*>(&k_c2c_mfma<PREC, S, L, L, ML, MS>), L, 16,
+                         E::LDS_ELEMS * (int)sizeof(c32)};
     }
 }
 

@@ -1,27 +1,34 @@
 // mfma_fft.hpp -- split-precision MFMA form of the FP32 Stockham pass.
 //
-// Every radix-16 stage is a 16x16 complex DFT-matrix product on the matrix
-// cores.  For 16 vectors x_n (16 complex points each) of one group:
-//   [Re Y; Im Y] = [Fr -Fi; Fi Fr] [Re X; Im X],  F_mk = exp(S 2 pi i mk/16)/4
-// as two v_mfma_f32_16x16x32 products (rows of Re Y, rows of Im Y; K = 32
-// covers Re X and Im X).  Operands are split into 16-bit parts and the
-// products accumulate in FP32:
+// Stage 0 (the 2/4/8 radix, or the first radix 16 when log2 L is a multiple
+// of 4) runs on the VALU through the stage engine; every later radix-16 stage
+// is a 16x16 complex DFT-matrix product on the matrix cores.  For 16 vectors
+// x_n (16 complex points each) of one group, with F_mp = exp(S 2 pi i mp/16)/4,
+// two v_mfma_f32_16x16x32 products (halves h = 0, 1) give outputs
+// m = 8h .. 8h + 7 as interleaved (Re Y_m, Im Y_m) rows:
+//   A_h[2r][2p] = Re F_mp,  A_h[2r][2p+1] = -Im F_mp,
+//   A_h[2r+1][2p] = Im F_mp, A_h[2r+1][2p+1] = Re F_mp,   m = 8h + r,
+// applied to B = [x.re_0, x.im_0, x.re_1, x.im_1, ...].  With real and
+// imaginary parts interleaved along K and along the output rows, a lane's
+// eight B values are four consecutive complex points and its accumulators
+// hold whole complex outputs, so every global and LDS access moves complex
+// values.  Operands are split into 16-bit parts and the products accumulate
+// in FP32:
 //   F16x2 : x = hi + lo (FP16),  F X ~ Fhi Xhi + Fhi Xlo + Flo Xhi    (3 MFMA)
 //   BF16x3: x = hi + mid + lo,   six products of combined order <= 2 (6 MFMA)
 // F16 needs range control: each tile is scaled by a power of two so that its
 // largest magnitude is < 1, and the 1/4 in F keeps a stage's outputs within
 // 4x of its inputs; the exact inverse scale is applied at the final store.
-// Twiddles stay FP32 (tables generated in FP64) and are applied by the stage
-// that produces a value, for the stage that consumes it.  The last radix-2/4/8
-// stage (when log2 L is not a multiple of 4) runs on the VALU.
+// Twiddles stay FP32 (tables generated in FP64): the consuming stage reads its
+// four contiguous per-stage table entries, as the VALU engine does.
 //
 // Operand lane layout of v_mfma_f32_16x16x32_{f16,bf16} (checked on MI355X):
 //   A: lane l holds A[l % 16][8 (l / 16) + e],  e = 0..7
 //   B: lane l holds B[8 (l / 16) + e][l % 16]
 //   D: lane l holds D[4 (l / 16) + e][l % 16],  e = 0..3
-// So in a group, lane (n = l % 16, kb = l / 16) supplies 8 consecutive input
-// points of vector n (kb & 1 selects points 0-7 / 8-15, kb >> 1 re / im) and
-// receives outputs 4 kb .. 4 kb + 3 of vector n.
+// So in a group, lane (n = l % 16, kb = l / 16) supplies points 4 kb .. 4 kb + 3
+// of vector n and receives outputs 2 kb, 2 kb + 1 (half 0) and 8 + 2 kb,
+// 9 + 2 kb (half 1) of vector n.
 #pragma once
 #include "pass_c2c.hpp"
 
@@ -102,29 +109,124 @@ __device__ __forceinline__ f4v split_product(const typename MfmaTraits<PREC>::V*
 }
 
 template <int PREC, int L, int NT>
-constexpr int kmfma_waves = waves_per_eu_target<2 * 17 * L * 4, NT, true>();
+constexpr int kmfma_waves = waves_per_eu_target<(16 * L + L + 1) * 8, NT, false>();
 
-// Tile: 16 transforms (TI = 16) of length L, LDS planes re/im in the padded
-// MAP_T layout of the VALU engine: position p of transform t at p * 17 + t,
-// so every access below is a per-lane base plus compile-time offsets.
+// Radix-16 MFMA stages s .. NST-1 of one tile whose stage s - 1 output is in
+// LDS (a barrier has published it).  Groups of 16 vectors: 16 transforms
+// with the same butterfly (column layout) or 16 consecutive butterflies of one
+// transform (row layout).
+template <int PREC, int S, int L, int NT, int ML, int MS, int s>
+__device__ __forceinline__ void mfma_stages(cpx<float>* lds, unsigned lane, unsigned wave, int nvalid,
+                                            const Twiddles<float>& tw, const typename MfmaTraits<PREC>::V* a0,
+                                            const typename MfmaTraits<PREC>::V* a1, c32* ob, int so0, int eso,
+                                            const PassDesc& d, int64_t b0base, float out_scale) {
+    using E = Engine<float, S, L, 16, NT, ML, MS>;
+    using P = RadixPlan<L>;
+    using Tr = MfmaTraits<PREC>;
+    using V = typename Tr::V;
+    constexpr unsigned NST = P::NST, NB = L / 16, NS = P::ns(s), W = NT / 64, GPW = (L / 16) / W;
+    constexpr bool LAST = s == NST - 1;
+    constexpr bool COLG = E::LAYOUT == MAP_T;
+    static_assert(P::radix(s) == 16, "MFMA stages are radix 16");
+    static_assert(COLG || NB % 16 == 0, "row groups need 16 | L/16");
+    const unsigned n = lane & 15u, kb = lane >> 4;
+    auto vec = [&](unsigned g, unsigned& t, unsigned& j) {
+        const unsigned gid = wave * GPW + g;
+        if constexpr (COLG) {
+            t = n;
+            j = gid;
+        } else {
+            t = gid / (NB / 16);
+            j = (gid % (NB / 16)) * 16 + n;
+        }
+    };
+    // inputs: points 4 kb + e of butterfly j, twiddled by row j mod NS of the stage table
+    const cpx<float>* stab = tw.stage + (2 * L + StageTable<L>::off(s)) + 4 * kb;
+    cpx<float> x[GPW][4];
+#pragma unroll
+    for (unsigned g = 0; g < GPW; ++g) {
+        unsigned t, j;
+        vec(g, t, j);
+        const cpx<float>* row = stab + (j & (NS - 1)) * 16;
+        if constexpr (E::template affine<NB>) {
+            const cpx<float>* src = lds + E::lidx(t, j + 4 * kb * NB);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) x[g][e] = cmul(src[E::template loff<NB>(e)], dirw<S>(row[e]));
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) x[g][e] = cmul(lds[E::lidx(t, j + (4 * kb + e) * NB)], dirw<S>(row[e]));
+        }
+    }
+    if constexpr (!LAST) __syncthreads();  // every read of this stage done before its writes
+#pragma unroll
+    for (unsigned g = 0; g < GPW; ++g) {
+        unsigned t, j;
+        vec(g, t, j);
+        V b[Tr::PARTS];
+        split_parts(reinterpret_cast<const float*>(x[g]), b);
+        const f4v y0 = split_product<PREC>(a0, b);  // outputs 2kb, 2kb + 1
+        const f4v y1 = split_product<PREC>(a1, b);  // outputs 8 + 2kb, 9 + 2kb
+        cpx<float> o[4] = {{y0[0], y0[1]}, {y0[2], y0[3]}, {y1[0], y1[1]}, {y1[2], y1[3]}};
+        if constexpr (LAST) {
+            // NS = NB: output m of butterfly j is natural index j + m NB
+            if ((int)t < nvalid) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) o[e] = cpx<float>{o[e].x * out_scale, o[e].y * out_scale};
+                if (d.tw_logn) {
+                    const uint32_t n2 = uint32_t((b0base + t) >> d.tw_b0_shift);
+                    const uint32_t mask = (1u << d.tw_logn) - 1u;
+                    const cpx<float> step = twiddle_big<S>(tw, (NB * n2) & mask, d.tw_logn);
+                    const cpx<float> w0 = twiddle_big<S>(tw, ((j + 2 * kb * NB) * n2) & mask, d.tw_logn);
+                    const cpx<float> w2 = twiddle_big<S>(tw, ((j + (8 + 2 * kb) * NB) * n2) & mask, d.tw_logn);
+                    o[0] = cmul(o[0], w0);
+                    o[1] = cmul(o[1], cmul(w0, step));
+                    o[2] = cmul(o[2], w2);
+                    o[3] = cmul(o[3], cmul(w2, step));
+                }
+                c32* obt = ob + (int64_t)t * so0 + (int64_t)(j + 2 * kb * NB) * eso;
+                const int64_t st = (int64_t)NB * eso;
+                obt[0] = o[0];
+                obt[st] = o[1];
+                obt[8 * st] = o[2];
+                obt[9 * st] = o[3];
+            }
+        } else {
+            const unsigned i0 = (j / NS) * NS * 16 + (j & (NS - 1)) + 2 * kb * NS;
+            if constexpr (E::template affine<NS>) {
+                cpx<float>* dst = lds + E::lidx(t, i0);
+                dst[E::template loff<NS>(0)] = o[0];
+                dst[E::template loff<NS>(1)] = o[1];
+                dst[E::template loff<NS>(8)] = o[2];
+                dst[E::template loff<NS>(9)] = o[3];
+            } else {
+                lds[E::lidx(t, i0)] = o[0];
+                lds[E::lidx(t, i0 + NS)] = o[1];
+                lds[E::lidx(t, i0 + 8 * NS)] = o[2];
+                lds[E::lidx(t, i0 + 9 * NS)] = o[3];
+            }
+        }
+    }
+    if constexpr (!LAST) {
+        __syncthreads();
+        mfma_stages<PREC, S, L, NT, ML, MS, s + 1>(lds, lane, wave, nvalid, tw, a0, a1, ob, so0, eso, d, b0base,
+                                                    out_scale);
+    }
+}
+
+// Tile: TI = 16 transforms of length L (256..1024), NT = L threads; persistent
+// over tiles like k_c2c, with the same PassDesc addressing.
 template <int PREC, int S, int L, int NT, int ML, int MS>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, NT), amdgpu_waves_per_eu(kmfma_waves<PREC, L, NT>))) void
 k_c2c_mfma(const c32* in, c32* out, PassDesc d, Twiddles<float> tw, const void* amat) {
     using Tr = MfmaTraits<PREC>;
     using V = typename Tr::V;
-    constexpr int TI = 16;
-    constexpr int LOGL = ilog2c(L);
-    constexpr int A = LOGL / 4;             // MFMA stages
-    constexpr int REM = 1 << (LOGL % 4);    // trailing VALU radix (1 = none)
-    constexpr int NB = L / 16;              // vectors per transform in an MFMA stage
-    constexpr int G = TI * L / 256;         // groups per MFMA stage
-    constexpr int W = NT / 64;              // waves
-    constexpr int GPW = G / W;              // groups per wave
-    static_assert(A >= 1 && G % W == 0 && GPW >= 1, "tile must split into whole groups per wave");
-    static_assert(NB % 16 == 0 || ML == MAP_T, "row groups need 16 | L/16");
+    using E = Engine<float, S, L, 16, NT, ML, MS>;
+    constexpr int TI = 16, W = NT / 64;
+    constexpr int A = RadixPlan<L>::NST - 1;  // MFMA stages
+    static_assert(A >= 1 && E::EPT == 16, "tile shape");
+    static_assert(((L / 16) % W) == 0, "groups must split evenly over waves");
 
-    __shared__ float pre[17 * L];
-    __shared__ float pim[17 * L];
+    __shared__ cpx<float> lds[E::LDS_ELEMS];
     __shared__ float red[W];
 
     const int tid0 = threadIdx.x;
@@ -136,7 +238,7 @@ k_c2c_mfma(const c32* in, c32* out, PassDesc d, Twiddles<float> tw, const void* 
     if (d.xcd_remap && (Gd % 8) == 0) tile = (tile % 8) * (Gd / 8) + tile / 8;
     if (tile >= ntiles) return;
 
-    // DFT-matrix operands for this lane.
+    // DFT-matrix operands for this lane (halves 0 and 1).
     V ar[Tr::PARTS], ai[Tr::PARTS];
     {
         const uint16_t* base = static_cast<const uint16_t*>(amat) + (S > 0 ? 0 : Tr::PARTS * 2 * 64 * 8);
@@ -153,13 +255,10 @@ k_c2c_mfma(const c32* in, c32* out, PassDesc d, Twiddles<float> tw, const void* 
         }
     }
 
-    auto lpos = [](int t, int p) -> int { return p * 17 + t; };
-
     bool first = true;
     for (;;) {
         const int tid = fresh_tid(tid0);
         const int lane = tid & 63, wave = tid >> 6;
-        const int n = lane & 15, kb = lane >> 4;
         const int64_t t0 = tile % tiles0;
         const int64_t rest = tile / tiles0;
         const int64_t b1 = rest % d.nb1, b2 = rest / d.nb1;
@@ -169,38 +268,21 @@ k_c2c_mfma(const c32* in, c32* out, PassDesc d, Twiddles<float> tw, const void* 
         const c32* ib = in + b0base * si0 + b1 * d.si1 + b2 * d.si2;
         c32* ob = out + b0base * so0 + b1 * d.so1 + b2 * d.so2;
 
-        // ---- stage 0 operands straight from HBM ------------------------
-        float x[GPW][8];
-#pragma unroll
-        for (int g = 0; g < GPW; ++g) {
-            const int gid = wave * GPW + g;
-            int t, j;
-            if constexpr (ML == MAP_T) {
-                t = n;
-                j = gid;
-            } else {
-                t = gid / (NB / 16);
-                j = (gid % (NB / 16)) * 16 + n;
-            }
-            const bool ok = t < nvalid;
-            const float* src = reinterpret_cast<const float*>(ib + t * si0) + (kb >> 1);
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int r = 8 * (kb & 1) + e;
-                x[g][e] = ok ? src[2 * (int64_t)(j + r * NB) * esi] : 0.f;
-            }
-        }
-        // Tile scale: power of two putting the largest magnitude below 1.
+        // ---- stage 0 on the VALU, operands straight from HBM ----------------
+        cpx<float> v[16];
+        E::load0(v, tid, [&](int t, int i) -> c32 {
+            if (t < nvalid) return ib[t * si0 + i * esi];
+            return c32{0.f, 0.f};
+        });
         float inv_scale = 1.f;
         if constexpr (PREC == MP_F16X2) {
+            // tile scale: a power of two putting the largest magnitude below 1
             float m = 0.f;
 #pragma unroll
-            for (int g = 0; g < GPW; ++g)
-#pragma unroll
-                for (int e = 0; e < 8; ++e) m = fmaxf(m, fabsf(x[g][e]));
+            for (int e = 0; e < 16; ++e) m = fmaxf(m, fmaxf(fabsf(v[e].x), fabsf(v[e].y)));
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-            if (!first) __syncthreads();  // red[] and planes read by the previous tile
+            if (!first) __syncthreads();  // red[] and LDS read by the previous tile
             if (lane == 0) red[wave] = m;
             __syncthreads();
             m = 0.f;
@@ -211,135 +293,17 @@ k_c2c_mfma(const c32* in, c32* out, PassDesc d, Twiddles<float> tw, const void* 
             const float sc = ldexpf(1.f, -ex);
             inv_scale = ldexpf(1.f, ex);
 #pragma unroll
-            for (int g = 0; g < GPW; ++g)
-#pragma unroll
-                for (int e = 0; e < 8; ++e) x[g][e] *= sc;
+            for (int e = 0; e < 16; ++e) v[e] = cscale(v[e], sc);
         } else {
             if (!first) __syncthreads();
         }
+        E::template compute<0>(v, tid, tw.stage);
+        E::template store_lds<0>(v, tid, lds);
+        __syncthreads();
 
-        // ---- MFMA stages ------------------------------------------------
-        int ns = 1;
-#pragma unroll
-        for (int s = 0; s < A; ++s) {
-            const bool last_mfma = (s == A - 1);
-            const bool to_global = last_mfma && REM == 1;
-            // Row groups for the first stage of row-loaded tiles and for a
-            // final store of row-stored tiles; column groups otherwise.
-            const bool rowgrp = (s == 0 && ML == MAP_J) || (to_global && MS == MAP_J);
-            if (s > 0) {
-                __syncthreads();  // previous stage's LDS writes visible
-#pragma unroll
-                for (int g = 0; g < GPW; ++g) {
-                    const int gid = wave * GPW + g;
-                    int t, j;
-                    if (rowgrp) {
-                        t = gid / (NB / 16);
-                        j = (gid % (NB / 16)) * 16 + n;
-                    } else {
-                        t = n;
-                        j = gid;
-                    }
-                    const float* src = ((kb >> 1) ? pim : pre) + lpos(t, j + 8 * (kb & 1) * NB);
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) x[g][e] = src[e * NB * 17];
-                }
-                __syncthreads();  // all reads done before this stage's writes
-            }
-            f4v yr[GPW], yi[GPW];
-#pragma unroll
-            for (int g = 0; g < GPW; ++g) {
-                V b[Tr::PARTS];
-                split_parts(x[g], b);
-                yr[g] = split_product<PREC>(ar, b);
-                yi[g] = split_product<PREC>(ai, b);
-            }
-            // outputs: lane (n, kb) holds Y_m of its vector, m = 4 kb + e
-            const int ns_next = ns * 16;
-#pragma unroll
-            for (int g = 0; g < GPW; ++g) {
-                const int gid = wave * GPW + g;
-                int t, j;
-                if (rowgrp) {
-                    t = gid / (NB / 16);
-                    j = (gid % (NB / 16)) * 16 + n;
-                } else {
-                    t = n;
-                    j = gid;
-                }
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int m = 4 * kb + e;
-                    const int p = (j / ns) * ns * 16 + (j % ns) + m * ns;
-                    c32 v{yr[g][e], yi[g][e]};
-                    if (to_global) {
-                        if (t < nvalid) {
-                            // p = j + m * NB: natural output index
-                            v = cscale(v, inv_scale * (float)(1 << (2 * A)));
-                            if (d.tw_logn) {
-                                const uint32_t n2 = uint32_t((b0base + t) >> d.tw_b0_shift);
-                                const uint32_t mask = (1u << d.tw_logn) - 1u;
-                                v = cmul(v, twiddle_big<S>(tw, (uint32_t(p) * n2) & mask, d.tw_logn));
-                            }
-                            ob[t * so0 + p * eso] = v;
-                        }
-                    } else {
-                        // twiddle of the consuming stage (radix R', NS' = ns_next)
-                        const int Rn = (s + 1 < A) ? 16 : REM;
-                        const int NBn = L / Rn;
-                        const int jn = p % NBn, rn = p / NBn;
-                        const int kn = jn % ns_next;
-                        const int idx = (rn * kn) * (TW_COARSE / (ns_next * Rn));
-                        v = cmul(v, dirw<S>(tw.coarse[idx]));
-                        const int a = lpos(t, p);
-                        pre[a] = v.x;
-                        pim[a] = v.y;
-                    }
-                }
-            }
-            ns = ns_next;
-        }
-
-        // ---- trailing VALU radix-REM stage --------------------------------
-        if constexpr (REM > 1) {
-            __syncthreads();
-            constexpr int NBR = L / REM;
-            constexpr int BPT = TI * NBR / NT;  // butterflies per thread
-            static_assert(TI * NBR % NT == 0, "butterflies must split evenly");
-            const float out_scale = inv_scale * (float)(1 << (2 * A));
-#pragma unroll
-            for (int b = 0; b < BPT; ++b) {
-                const int q = tid + b * NT;
-                int t, j;
-                if constexpr (MS == MAP_T) {
-                    t = q % TI;
-                    j = q / TI;
-                } else {
-                    t = q / NBR;
-                    j = q % NBR;
-                }
-                c32 v[REM];
-#pragma unroll
-                for (int r = 0; r < REM; ++r) {
-                    const int a = lpos(t, j + r * NBR);
-                    v[r] = c32{pre[a], pim[a]};
-                }
-                dft<REM, S>(v);
-                if (t < nvalid) {
-                    if (d.tw_logn) {
-                        const uint32_t n2 = uint32_t((b0base + t) >> d.tw_b0_shift);
-                        const uint32_t mask = (1u << d.tw_logn) - 1u;
-                        c32 f[REM];
-                        twiddle_run<REM, S>(tw, (uint32_t(j) * n2) & mask, (uint32_t(NBR) * n2) & mask, d.tw_logn, f);
-#pragma unroll
-                        for (int r = 0; r < REM; ++r) ob[t * so0 + (j + r * NBR) * eso] = cmul(cscale(v[r], out_scale), f[r]);
-                    } else {
-#pragma unroll
-                        for (int r = 0; r < REM; ++r) ob[t * so0 + (j + r * NBR) * eso] = cscale(v[r], out_scale);
-                    }
-                }
-            }
-        }
+        // ---- radix-16 stages on the matrix cores ----------------------------
+        mfma_stages<PREC, S, L, NT, ML, MS, 1>(lds, (unsigned)lane, (unsigned)wave, nvalid, tw, ar, ai, ob, so0, eso,
+                                               d, b0base, inv_scale * (float)(1 << (2 * A)));
 
         const int64_t next = tile + Gd;
         if (next >= ntiles) break;
