@@ -477,6 +477,9 @@ def _extra_uneven() -> dict:
     # 10,000 transforms, 80% 1-D (log2 N on 8..20), 20% 2-D (sides 2^6..2^11),
     # values uniform[-1,1) from seed 4, FP32, one graph-replayed queue launch.
     items = synth.uneven_shapes(4, 10000)
+    # the autotuner picks the four-step split of every 1-D size class that
+    # takes two passes (2^14 .. 2^20) before the plan is built
+    tuned = numrs_amd.autotune_uneven("float32", 14, 20, iters=3)
     plan = numrs_amd.UnevenPlan("float32", items)
     total = plan.total
     x = torch.empty(2 * total, dtype=torch.float32, device="cuda")
@@ -523,6 +526,7 @@ def _extra_uneven() -> dict:
         "GBps": round(byt / (ms * 1e-3) / 1e9, 1), "frac_hbm": round(byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "max_rel_l2_vs_fp64": worst, "rel_l2_all_vs_fp64": math.sqrt(acc_e / acc_r),
         "tasks": [info["tasks_phase1"], info["tasks_phase2"]], "groups": info["groups"],
+        "autotuned_split_log2_n1": {str(k): v[0] for k, v in tuned.items()},
         "launch": "graph replay: counter-reset fill kernel + the persistent queue kernel, one dispatch per pass phase"}
     return res
 

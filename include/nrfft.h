@@ -200,6 +200,14 @@ int nrfft_uneven_check(nrfft_uneven plan, void* stream);
 int nrfft_uneven_info(nrfft_uneven plan, size_t* tasks_phase1, size_t* tasks_phase2, size_t* workspace_bytes,
                       unsigned* groups);
 int nrfft_uneven_destroy(nrfft_uneven plan);
+/* Autotuner of the uneven-batch size classes: for every 1-D length 2^k,
+ * log2n_min <= k <= log2n_max, that takes the four-step path, times each
+ * legal split N = 2^l1 x 2^(k - l1) on a batch of that class (iters
+ * executions) and caches the fastest per (dtype, N) with the C2C tunings
+ * (nrfft_tune_cache_save/load persist both).  Plans created afterwards use it.
+ * best_l1[k - log2n_min] / best_ms[...] receive the choice and its time (0 for
+ * single-pass classes); either may be NULL. */
+int nrfft_autotune_uneven(int dtype, int log2n_min, int log2n_max, int iters, int* best_l1, float* best_ms);
 
 /* FFTProcessor::fft_batch (FFT_1.rs:185-189): `count` host buffers, buffer i
  * holding 2*nn[i] interleaved values, transformed in place in one launch. */

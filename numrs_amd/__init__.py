@@ -57,6 +57,7 @@ from .device import (  # noqa: F401
     TwofftPlan,
     UnevenPlan,
     autotune_c2c,
+    autotune_uneven,
     gen_uniform,
     tune_cache_load,
     tune_cache_save,

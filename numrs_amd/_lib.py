@@ -94,6 +94,7 @@ _SIGS = {
     "nrfft_uneven_check": (_i, [_vp, _vp]),
     "nrfft_uneven_info": (_i, [_vp, _szp, _szp, _szp, ctypes.POINTER(ctypes.c_uint)]),
     "nrfft_uneven_destroy": (_i, [_vp]),
+    "nrfft_autotune_uneven": (_i, [_i, _i, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_float)]),
     "nrfft_fft_batch_f64": (_i, [ctypes.POINTER(_dp), _szp, _sz, _i]),
     "nrfft_fft_batch_f32": (_i, [ctypes.POINTER(_fp), _szp, _sz, _i]),
     "nrfft_gen_uniform_f32": (_i, [_vp, _sz, ctypes.c_uint64, ctypes.c_uint64, _vp]),

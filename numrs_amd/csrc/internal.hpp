@@ -51,7 +51,8 @@ enum PlanKind {
     PK_TWOFFT = 4,
     PK_COSFT1 = 5,
     PK_COSFT2 = 6,
-    PK_RLFT3 = 7
+    PK_RLFT3 = 7,
+    PK_UNEVEN = 8  // tune-cache key of an uneven-batch size class (nn = {N}, batch 0)
 };
 // Launch kinds.
 enum LaunchKind { LK_C2C = 0, LK_COPY = 1, LK_REAL = 2, LK_SPEC = 3 };
@@ -118,6 +119,8 @@ int build_plan_spectral(Plan& p);
 int exec_spectral(const Plan& p, const Launch& l, void* const* bufs, int isign, hipStream_t stream, DeviceCtx* ctx);
 int plan_c2c_nd(Plan& p, const std::vector<size_t>& nn, int64_t batch);
 void tune_lookup(Plan& p);
+bool tune_find(int kind, int dtype, const std::vector<size_t>& nn, size_t batch, Tune* out);
+void tune_store(int kind, int dtype, const std::vector<size_t>& nn, size_t batch, const Tune& t);
 Tune default_tune(const Plan& p);
 int plan_c2c_1d(Plan& p, int logn, int64_t batch, int src, int dst, size_t* tmp_elems);
 int plan_add_c2c_pass(Plan& p, int mode, int logl, int tilog, const PassDesc& d, int src, int dst);

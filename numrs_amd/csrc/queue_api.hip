@@ -86,7 +86,17 @@ static void add_pass(std::vector<QTask>& out, int dtype, int logl, int mode, int
     }
 }
 
-static int build_uneven(UnevenState& u) {
+// Four-step split of a 1-D item of 2^logn points: the autotuned choice for
+// its size class when the tune cache holds one, else N1 = 2^floor(logn / 2).
+static int uneven_split(int dtype, int logn, int lo, int hi, const std::vector<int>* force) {
+    if (force && (size_t)logn < force->size() && (*force)[logn] > 0) return (*force)[logn];
+    Tune t;
+    if (tune_find(PK_UNEVEN, dtype, {size_t(1) << logn}, 0, &t) && t.split_log1 >= lo && t.split_log1 <= hi)
+        return t.split_log1;
+    return std::min(hi, std::max(lo, logn / 2));
+}
+
+static int build_uneven(UnevenState& u, const std::vector<int>* force_split = nullptr) {
     const bool f32 = u.dtype == NRFFT_F32;
     const int qlog = ilog2(f32 ? qpts<float>() : qpts<double>());
     const int maxl = f32 ? q_max_logl<float>() : q_max_logl<double>();
@@ -102,11 +112,10 @@ static int build_uneven(UnevenState& u) {
                 if (logn < Q_MIN_LOGL) return fail(NRFFT_E_UNSUPPORTED, "uneven items need N >= 64");
                 add_pass(a, u.dtype, logn, 0, 1, B_IN, B_OUT, it.off, it.n0, 1, it.off, it.n0, 1, 0);
             } else {
-                int l1 = logn / 2;
-                if (l1 < Q_MIN_LOGL) l1 = Q_MIN_LOGL;
+                const int lo = std::max(Q_MIN_LOGL, logn - maxl_col), hi = std::min(maxl_col, logn - Q_MIN_LOGL);
+                if (lo > hi) return fail(NRFFT_E_UNSUPPORTED, "uneven 1-D item longer than the queue supports");
+                const int l1 = uneven_split(u.dtype, logn, lo, hi, force_split);
                 const int l2 = logn - l1;
-                if (l2 > maxl_col || l1 > maxl_col)
-                    return fail(NRFFT_E_UNSUPPORTED, "uneven 1-D item longer than the queue supports");
                 const int64_t N1 = int64_t(1) << l1, N2 = int64_t(1) << l2;
                 // pass A: columns n2 (stride N2), length N1, twiddle -> workspace [k1][n2]
                 add_pass(a, u.dtype, l1, 1, N2, B_IN, B_TMP, (int64_t)it.off, 1, (int32_t)N2, (int64_t)ws, 1,
@@ -223,6 +232,88 @@ static int uneven_execute(UnevenState& u, const void* in, void* out, int isign, 
         u.g_isign = isign;
     }
     HIPCHK(hipGraphLaunch(u.gexec, stream));
+    return NRFFT_OK;
+}
+
+// Average ms of `iters` executions of an uneven plan (plain launches).
+template <typename T>
+static int time_uneven(UnevenState& u, void* in, void* out, int iters, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                       float* ms) {
+    int rc = uneven_enqueue<T>(u, in, out, 1, s);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(e0, s));
+    for (int i = 0; i < iters; ++i)
+        if ((rc = uneven_enqueue<T>(u, in, out, 1, s))) return rc;
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipEventElapsedTime(ms, e0, e1));
+    *ms /= (float)iters;
+    return NRFFT_OK;
+}
+
+// Every four-step split of the 1-D size classes 2^lmin .. 2^lmax, timed on a
+// batch of that class (2^25 FP32 / 2^24 FP64 points); the fastest goes to the
+// tune cache under (PK_UNEVEN, dtype, {N}).
+static int autotune_uneven(int dtype, int lmin, int lmax, int iters, int* best_l1, float* best_ms) {
+    const bool f32 = dtype == NRFFT_F32;
+    const int qlog = ilog2(f32 ? qpts<float>() : qpts<double>());
+    const int maxl = f32 ? q_max_logl<float>() : q_max_logl<double>();
+    const int maxl_col = std::min(maxl, qlog - 2);
+    const size_t pts = size_t(1) << (f32 ? 25 : 24);
+    struct Res {
+        hipStream_t s = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        void* in = nullptr;
+        void* out = nullptr;
+        ~Res() {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (in) (void)hipFree(in);
+            if (out) (void)hipFree(out);
+            if (s) (void)hipStreamDestroy(s);
+        }
+    } r;
+    HIPCHK(hipStreamCreateWithFlags(&r.s, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&r.e0));
+    HIPCHK(hipEventCreate(&r.e1));
+    if (hipMalloc(&r.in, pts * csize(dtype)) != hipSuccess || hipMalloc(&r.out, pts * csize(dtype)) != hipSuccess)
+        return fail(NRFFT_E_NOMEM, "autotune buffers");
+    HIPCHK(hipMemsetAsync(r.in, 0, pts * csize(dtype), r.s));
+    for (int logn = lmin; logn <= lmax; ++logn) {
+        const int slot = logn - lmin;
+        if (best_l1) best_l1[slot] = 0;
+        if (best_ms) best_ms[slot] = 0.f;
+        const int lo = std::max(Q_MIN_LOGL, logn - maxl_col), hi = std::min(maxl_col, logn - Q_MIN_LOGL);
+        if (logn <= maxl || lo > hi) continue;  // single pass: nothing to choose
+        const size_t n = size_t(1) << logn;
+        const size_t count = std::max<size_t>(1, pts / n);
+        float best = 0.f;
+        int arg = 0;
+        for (int l1 = lo; l1 <= hi; ++l1) {
+            UnevenState u;
+            u.dtype = dtype;
+            for (size_t k = 0; k < count; ++k) u.items.push_back(UnevenItem{1, n, 1, k * n});
+            u.total_elems = count * n;
+            std::vector<int> force(size_t(logn) + 1, 0);
+            force[logn] = l1;
+            int rc = build_uneven(u, &force);
+            if (!rc) rc = uneven_setup_device(u);
+            float ms = 0.f;
+            if (!rc) rc = f32 ? time_uneven<float>(u, r.in, r.out, iters, r.s, r.e0, r.e1, &ms)
+                              : time_uneven<double>(u, r.in, r.out, iters, r.s, r.e0, r.e1, &ms);
+            if (rc) return rc;
+            if (arg == 0 || ms < best) {
+                best = ms;
+                arg = l1;
+            }
+        }
+        Tune t;
+        t.split_log1 = arg;
+        tune_store(PK_UNEVEN, dtype, {n}, 0, t);
+        if (best_l1) best_l1[slot] = arg;
+        if (best_ms) best_ms[slot] = best;
+    }
+    HIPCHK(hipStreamSynchronize(r.s));
     return NRFFT_OK;
 }
 
@@ -365,6 +456,15 @@ int nrfft_uneven_info(nrfft_uneven plan, size_t* tasks_phase1, size_t* tasks_pha
 int nrfft_uneven_destroy(nrfft_uneven plan) {
     delete plan;
     return NRFFT_OK;
+}
+
+int nrfft_autotune_uneven(int dtype, int log2n_min, int log2n_max, int iters, int* best_l1, float* best_ms) {
+    g_err.clear();
+    if (dtype != NRFFT_F32 && dtype != NRFFT_F64) return fail(NRFFT_E_INVALID, "dtype");
+    if (log2n_min < Q_MIN_LOGL || log2n_max < log2n_min || log2n_max > 26)
+        return fail(NRFFT_E_INVALID, "size-class range");
+    if (iters < 1) iters = 1;
+    return autotune_uneven(dtype, log2n_min, log2n_max, iters, best_l1, best_ms);
 }
 
 int nrfft_fft_batch_f64(double* const* data, const size_t* nn, size_t count, int isign) {

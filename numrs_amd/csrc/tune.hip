@@ -96,6 +96,23 @@ void tune_lookup(Plan& p) {
 }
 
 
+bool tune_find(int kind, int dtype, const std::vector<size_t>& nn, size_t batch, Tune* out) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (!g_tune_env_loaded) {
+        g_tune_env_loaded = true;
+        if (const char* path = std::getenv("NRFFT_TUNE_CACHE")) (void)load_file_locked(path);
+    }
+    auto it = g_tune.find(TuneKey(kind, dtype, nn, batch));
+    if (it == g_tune.end()) return false;
+    *out = it->second;
+    return true;
+}
+
+void tune_store(int kind, int dtype, const std::vector<size_t>& nn, size_t batch, const Tune& t) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[TuneKey(kind, dtype, nn, batch)] = t;
+}
+
 static std::vector<Tune> candidates(int dtype, const std::vector<size_t>& nn, size_t batch) {
     std::vector<Tune> out;
     const int maxt = dtype == NRFFT_F32 ? 5 : 4;

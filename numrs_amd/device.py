@@ -260,6 +260,17 @@ class UnevenPlan:
         return {"tasks_phase1": a.value, "tasks_phase2": b.value, "workspace_bytes": w.value, "groups": g.value}
 
 
+def autotune_uneven(dtype, log2n_min: int = 14, log2n_max: int = 20, iters: int = 5) -> dict:
+    """Autotune the four-step split of the uneven-batch 1-D size classes
+    2^log2n_min .. 2^log2n_max; returns {log2 N: (log2 N1, ms)} for the
+    classes that take the four-step path.  Cached like the C2C tunings."""
+    k = log2n_max - log2n_min + 1
+    l1 = (ctypes.c_int * k)()
+    ms = (ctypes.c_float * k)()
+    _check(_lib.lib().nrfft_autotune_uneven(_dtype_code(dtype), log2n_min, log2n_max, iters, l1, ms))
+    return {log2n_min + i: (l1[i], ms[i]) for i in range(k) if l1[i] > 0}
+
+
 def gen_uniform(x, seed: int, start: int = 0, stream=None) -> None:
     """Fill tensor x (float32/float64, contiguous, on the GPU) with the seeded
     uniform[-1,1) stream of numrs_amd.synth, generated in HBM."""

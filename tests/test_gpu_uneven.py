@@ -166,3 +166,26 @@ def test_uneven_size_limits(dtype, items, tol):
 def test_uneven_rejects_out_of_range(dtype, item):
     with pytest.raises(numrs_amd.NrfftError):
         numrs_amd.UnevenPlan(dtype, [item])
+
+
+def test_uneven_autotune_keeps_results():
+    # The autotuner's per-class four-step split is used by later plans, whose
+    # results stay those of the oracle.
+    import torch
+
+    choice = numrs_amd.autotune_uneven("float64", 14, 17, iters=2)
+    assert set(choice) == {14, 15, 16, 17}
+    for logn, (l1, ms) in choice.items():
+        assert 6 <= l1 <= logn - 6 and ms > 0
+    items = [(1 << 14,), (1 << 15,), (1 << 16,), (1 << 17,), (128, 64)]
+    plan = numrs_amd.UnevenPlan("float64", items)
+    x = torch.empty(2 * plan.total, dtype=torch.float64, device="cuda")
+    numrs_amd.gen_uniform(x, 12)
+    y = torch.empty_like(x)
+    plan.execute(x, y, 1)
+    plan.check()
+    xh, yh = x.cpu().numpy(), y.cpu().numpy()
+    for it, off in zip(items, plan.offsets):
+        n = int(np.prod(it))
+        seg = slice(2 * off, 2 * (off + n))
+        assert rel_l2(yh[seg], _ref(xh[seg], it, 1)) < 1e-12, it
