@@ -183,6 +183,10 @@ __host__ __device__ constexpr int ilog2c(long long v) { return v <= 1 ? 0 : 1 + 
 #ifndef NRFFT_LDS_SWIZZLE
 #define NRFFT_LDS_SWIZZLE 0
 #endif
+// One pad slot per NRFFT_LDS_PAD complex values of a padded LDS tile.
+#ifndef NRFFT_LDS_PAD
+#define NRFFT_LDS_PAD 16
+#endif
 
 __host__ __device__ constexpr int plan_radix(int L, int LOG, int NST, int s) {
     return (L < 16) ? L
@@ -285,23 +289,24 @@ struct Engine {
     // column with a hash of the row instead (no affine offsets).
     static constexpr int LSI = (LAYOUT == MAP_T) ? TI : 1;  // raw distance between i and i + 1
     static constexpr bool SWZ = NRFFT_LDS_SWIZZLE && LAYOUT == MAP_T && TI > 1;
-    static constexpr int LDS_ELEMS = SWZ ? TI * L : TI * L + (TI * L) / 16 + 1;
+    static constexpr int PAD = NRFFT_LDS_PAD;
+    static constexpr int LDS_ELEMS = SWZ ? TI * L : TI * L + (TI * L) / PAD + 1;
     static __device__ __forceinline__ int lidx(int t, int i) {
         if constexpr (SWZ) {
             const unsigned h = unsigned(i) + (unsigned(i) >> 4) + (unsigned(i) >> 8);
             return i * TI + (t ^ int(h & unsigned(TI - 1)));
         } else {
             const int raw = (LAYOUT == MAP_T) ? i * TI + t : t * L + i;
-            return raw + (raw >> 4);
+            return raw + raw / PAD;
         }
     }
     // lidx(t, i0 + r * STRIDE) = lidx(t, i0) + loff<STRIDE>(r) whenever the raw
     // step STRIDE * LSI is a multiple of 16 (then the pad term is affine too).
     template <int STRIDE>
-    static constexpr bool affine = !SWZ && ((STRIDE * LSI) % 16) == 0;
+    static constexpr bool affine = !SWZ && ((STRIDE * LSI) % PAD) == 0;
     template <int STRIDE>
     static constexpr int loff(int r) {
-        return r * (STRIDE * LSI + (STRIDE * LSI) / 16);
+        return r * (STRIDE * LSI + (STRIDE * LSI) / PAD);
     }
 
     template <int MAP, int NB>

@@ -109,7 +109,7 @@ __device__ __forceinline__ f4v split_product(const typename MfmaTraits<PREC>::V*
 }
 
 template <int PREC, int L, int NT>
-constexpr int kmfma_waves = waves_per_eu_target<(16 * L + L + 1) * 8, NT, false>();
+constexpr int kmfma_waves = waves_per_eu_target<(16 * L + 16 * L / NRFFT_LDS_PAD + 1) * 8, NT, false>();
 
 // Radix-16 MFMA stages s .. NST-1 of one tile whose stage s - 1 output is in
 // LDS (a barrier has published it).  Groups of 16 vectors: 16 transforms

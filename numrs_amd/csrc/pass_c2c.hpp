@@ -48,7 +48,7 @@ constexpr int waves_per_eu_target() {
 }
 template <typename T, int L, int TI, int NT>
 constexpr int kc2c_waves =
-    waves_per_eu_target<(RadixPlan<L>::NST > 1 ? (TI * L + (NRFFT_LDS_SWIZZLE ? 0 : TI * L / 16 + 1)) *
+    waves_per_eu_target<(RadixPlan<L>::NST > 1 ? (TI * L + (NRFFT_LDS_SWIZZLE ? 0 : TI * L / NRFFT_LDS_PAD + 1)) *
                                                      (int)sizeof(cpx<T>)
                                                : 0),
                         NT, NRFFT_PREFETCH != 0>();

@@ -50,7 +50,7 @@ constexpr int q_nclasses() {
 
 template <typename T>
 constexpr int q_lds_elems() {
-    return qpts<T>() + qpts<T>() / 16 + 1;
+    return qpts<T>() + qpts<T>() / NRFFT_LDS_PAD + 1;
 }
 
 struct QTask {
