@@ -288,6 +288,8 @@ def extra_configs(args, only: str = "all") -> dict:
         res.update(_extra_uneven())
     if only in ("all", "spectral"):
         res.update(_extra_spectral())
+    if only in ("all", "fourfs"):
+        res.update(_extra_fourfs())
     return res
 
 
@@ -456,6 +458,28 @@ def _extra_spectral() -> dict:
     return res
 
 
+def _extra_fourfs() -> dict:
+    """Out-of-core fourn (Four_FS.rs): a host array larger than the device
+    budget streamed through HBM (PCIe-bound by construction)."""
+    import numrs_amd
+
+    res = {}
+    for name, shape, budget in (("fourfs_f32_1d_2p25_budget64MiB", [1 << 25], 64 << 20),
+                                ("fourfs_f32_512x512x256_budget64MiB", [512, 512, 256], 64 << 20)):
+        total = int(np.prod(shape))
+        x = np.tile(np.linspace(-1.0, 1.0, 2 * 4096, dtype=np.float32), total // 4096)
+        g = x.copy()
+        numrs_amd.fourfs(g, shape, len(shape), 1, device_bytes=budget)  # warm-up (plans, twiddles)
+        g[:] = x
+        t0 = time.perf_counter()
+        numrs_amd.fourfs(g, shape, len(shape), 1, device_bytes=budget)
+        s = time.perf_counter() - t0
+        res[name] = {"s": round(s, 3), "data_MiB": total * 8 >> 20, "device_budget_MiB": budget >> 20,
+                     "GBps_data": round(total * 8 / s / 1e9, 2),
+                     "note": "host array in pageable memory; every axis pass moves the data over PCIe twice"}
+    return res
+
+
 def _segment_sums(v, lengths):
     """Sums of consecutive segments of v (lengths in elements), on the GPU,
     from differences of an FP64 prefix sum."""
@@ -540,7 +564,7 @@ def main() -> int:
     ap.add_argument("--cpu-reps", type=int, default=5, help="timed runs of the CPU sample (median kept)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--only", choices=["headline", "four1", "fourn", "convlv", "uneven", "spectral"], default=None,
+    ap.add_argument("--only", choices=["headline", "four1", "fourn", "convlv", "uneven", "spectral", "fourfs"], default=None,
                     help="run one workload only (profiling runs); the JSON line then holds just that workload")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 512 MiB copy kernel (counter calibration for the traffic figure)")

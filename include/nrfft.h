@@ -166,6 +166,16 @@ int nrfft_cosft2_f32(float* y, size_t n, int isign);
  * transform. */
 int nrfft_rlft3_f64(double* data, double* speq, size_t nn1, size_t nn2, size_t nn3, int isign);
 int nrfft_rlft3_f32(float* data, float* speq, size_t nn1, size_t nn2, size_t nn3, int isign);
+/* Four_FS.rs:55 FourFS::fourfs(nn, ndim, isign) -- the out-of-core n-D FFT:
+ * in place on a host array of prod(nn) interleaved complex values (C order;
+ * an mmap-ed file works), streamed through HBM with at most device_bytes of
+ * device memory (0: 1 GiB).  Results equal fourn's.  Strided axes up to 8192
+ * points; a 1-D transform larger than half the budget runs a four-step over
+ * the host array and needs `scratch` (same size as data), else scratch may
+ * be NULL.  Errors follow Four_FS.rs:328-339 (NRFFT_E_DIMS "Invalid
+ * dimensions" / "Invalid dimension size", NRFFT_E_ISIGN). */
+int nrfft_fourfs_f64(double* data, double* scratch, const size_t* nn, size_t ndim, int isign, size_t device_bytes);
+int nrfft_fourfs_f32(float* data, float* scratch, const size_t* nn, size_t ndim, int isign, size_t device_bytes);
 /* Device plans of the same transforms (batch back to back; twofft outputs
  * are 2n values per transform, cosft1 n + 1, cosft2 n, rlft3 speq
  * 2 nn1 nn2).  Outputs must not overlap inputs except in/out of cosft1/2

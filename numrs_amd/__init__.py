@@ -32,8 +32,10 @@ from .api import (  # noqa: F401
     cosft2,
     cosft2_simd,
     extract_real_imag,
+    FourFS,
     four1,
     four1_optimized,
+    fourfs,
     fourn,
     magnitude_spectrum,
     power_spectrum,

@@ -73,6 +73,8 @@ _SIGS = {
     "nrfft_cosft2_f32": (_i, [_fp, _sz, _i]),
     "nrfft_rlft3_f64": (_i, [_dp, _dp, _sz, _sz, _sz, _i]),
     "nrfft_rlft3_f32": (_i, [_fp, _fp, _sz, _sz, _sz, _i]),
+    "nrfft_fourfs_f64": (_i, [_dp, _dp, _szp, _sz, _i, _sz]),
+    "nrfft_fourfs_f32": (_i, [_fp, _fp, _szp, _sz, _i, _sz]),
     "nrfft_plan_twofft": (_i, [ctypes.POINTER(_vp), _i, _sz, _sz]),
     "nrfft_execute_twofft": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "nrfft_plan_cosft1": (_i, [ctypes.POINTER(_vp), _i, _sz, _sz]),

@@ -62,9 +62,11 @@ class CorrelError(NrfftError):
 class FournError(ValueError):
     """``io::Error`` of kind InvalidInput with the messages of Fourn.rs:367-378."""
 
-    def __init__(self, code: int):
+    def __init__(self, code: int, message: str | None = None):
         self.code = code
-        super().__init__(_lib.status_string(code) if code != _lib.E_DIMS else (_lib.last_error() or "Invalid dimensions"))
+        if message is None:
+            message = _lib.status_string(code) if code != _lib.E_DIMS else (_lib.last_error() or "Invalid dimensions")
+        super().__init__(message)
 
 
 def _check(code: int) -> None:
@@ -563,3 +565,94 @@ def rlft3_optimized(data: np.ndarray, speq: np.ndarray, nn1: int, nn2: int, nn3:
     d = _inplace_view(data, nn1 * nn2 * nn3).reshape(nn1, nn2, nn3)
     s = _inplace_view(speq, nn1 * 2 * nn2).reshape(nn1, 2 * nn2)
     rlft3(d, s, nn1, nn2, nn3, isign)
+
+
+# ---------------------------------------------------------------------------
+# Out-of-core fourn (Four_FS.rs)
+# ---------------------------------------------------------------------------
+
+def _fourfs_call(data: np.ndarray, scratch, nn: Sequence[int], ndim: int, isign: int, device_bytes: int) -> None:
+    nn = [int(v) for v in nn]
+    if ndim == 0 or ndim > len(nn):                       # Four_FS.rs:329-331
+        raise FournError(_lib.E_DIMS, "Invalid dimensions")
+    arr = (ctypes.c_size_t * len(nn))(*nn)
+    p, sfx = _buf(data)
+    q = None
+    if scratch is not None:
+        if scratch.dtype != data.dtype or scratch.size < data.size:
+            raise ValueError("scratch must match data's dtype and size")
+        q, _ = _buf(scratch)
+    code = getattr(_lib.lib(), f"nrfft_fourfs_{sfx}")(p, q, arr, ndim, isign, int(device_bytes))
+    if code != _lib.OK:
+        if code in (_lib.E_DIMS, _lib.E_ISIGN, _lib.E_NOT_POW2):
+            raise FournError(code)
+        raise NrfftError(code)
+
+
+def fourfs(data: np.ndarray, nn: Sequence[int], ndim: int, isign: int, device_bytes: int = 0,
+           scratch: np.ndarray | None = None) -> None:
+    """The out-of-core n-D FFT of Four_FS.rs:55 on a host array (a numpy
+    memmap of a file works): in place, C order, fourn's results, streamed
+    through HBM with at most ``device_bytes`` of device memory (0: 1 GiB).
+    A 1-D transform larger than half the budget needs ``scratch`` (same size;
+    allocated here when not given)."""
+    total = int(np.prod([int(v) for v in nn][:max(ndim, 0)])) if ndim > 0 else 0
+    v = _inplace_view(data, 2 * total) if total > 0 else data
+    if scratch is None and ndim == 1 and total > 0 and 2 * total * v.itemsize * 2 > (device_bytes or (1 << 30)):
+        scratch = np.empty_like(v)
+    _fourfs_call(v, scratch, nn, ndim, isign, device_bytes)
+
+
+class FourFS:
+    """Four_FS.rs:13-75.  ``new(temp_dir)`` creates four temporary files
+    (fft_1 .. fft_4, Four_FS.rs:20-40).  The data to transform live in fft_1
+    as interleaved complex float64 values in C order (``write_data`` /
+    ``read_data``); ``fourfs(nn, ndim, isign)`` transforms them in place,
+    memory-mapped and streamed through HBM; fft_2 is the scratch of 1-D
+    transforms larger than the device budget."""
+
+    def __init__(self, temp_dir, device_bytes: int = 0) -> None:
+        import os
+        import tempfile
+
+        self.paths = []
+        for k in range(1, 5):
+            fd, path = tempfile.mkstemp(prefix=f"fft_{k}", dir=str(temp_dir))
+            os.close(fd)
+            self.paths.append(path)
+        self.device_bytes = device_bytes
+        self.memory_mapped = False
+
+    @classmethod
+    def new(cls, temp_dir) -> "FourFS":
+        return cls(temp_dir)
+
+    def with_memory_mapping(self) -> "FourFS":
+        """Four_FS.rs:42-53 (the transform always memory-maps the files)."""
+        self.memory_mapped = True
+        return self
+
+    def write_data(self, data) -> None:
+        np.ascontiguousarray(data, dtype=np.float64).tofile(self.paths[0])
+
+    def read_data(self) -> np.ndarray:
+        return np.fromfile(self.paths[0], dtype=np.float64)
+
+    def fourfs(self, nn: Sequence[int], ndim: int, isign: int) -> None:
+        nn = [int(v) for v in nn]
+        if ndim == 0 or ndim > len(nn):                          # Four_FS.rs:329-331
+            raise FournError(_lib.E_DIMS, "Invalid dimensions")
+        if isign not in (1, -1):                                   # Four_FS.rs:332-334
+            raise FournError(_lib.E_ISIGN, "isign must be 1 or -1")
+        if any(v <= 1 for v in nn[:ndim]):                         # Four_FS.rs:335-337
+            raise FournError(_lib.E_DIMS, "Invalid dimension size")
+        total = int(np.prod(nn[:ndim]))
+        data = np.memmap(self.paths[0], dtype=np.float64, mode="r+", shape=(2 * total,))
+        scratch = None
+        if ndim == 1:
+            scratch = np.memmap(self.paths[1], dtype=np.float64, mode="w+", shape=(2 * total,))
+        try:
+            _fourfs_call(data, scratch, nn, ndim, isign, self.device_bytes)
+            data.flush()
+        finally:
+            del data, scratch

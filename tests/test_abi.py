@@ -153,3 +153,26 @@ def test_spectral_validation():
         numrs_amd.rlft3(np.zeros((2, 2, 4)), np.zeros((2, 2)), 2, 2, 4, 1)         # Real_FT3.rs:18
     assert numrs_amd.extract_real_imag(numrs_amd.combine_real_imag([1.0, 2.0, 3.0], [4.0, 5.0, 6.0]))[1].tolist() \
         == [4.0, 5.0, 6.0]                                                         # FFT_2.rs:470-479
+
+
+def test_fourfs_validation(tmp_path):
+    # Four_FS.rs:328-339 order (checked before any device work) and the
+    # reference's test_invalid_inputs (Four_FS.rs:427-437).
+    L = _lib.lib()
+    d, _ = _d(64)
+    p = d.ctypes.data_as(_lib._dp)
+    nn = (ctypes.c_size_t * 3)(8, 8, 1)
+    assert L.nrfft_fourfs_f64(p, None, nn, 0, 1, 0) == _lib.E_DIMS
+    assert L.nrfft_fourfs_f64(p, None, nn, 2, 0, 0) == _lib.E_ISIGN
+    assert L.nrfft_fourfs_f64(p, None, nn, 3, 1, 0) == _lib.E_DIMS
+    bad = (ctypes.c_size_t * 1)(12)
+    assert L.nrfft_fourfs_f64(p, None, bad, 1, 1, 0) == _lib.E_NOT_POW2
+    assert L.nrfft_fourfs_f64(None, None, nn, 2, 1, 0) == _lib.E_INVALID
+    fs = numrs_amd.FourFS.new(tmp_path)
+    assert len(fs.paths) == 4 and fs.with_memory_mapping().memory_mapped
+    with pytest.raises(numrs_amd.FournError, match="Invalid dimension size"):
+        fs.fourfs([1], 1, 1)
+    with pytest.raises(numrs_amd.FournError, match="isign must be 1 or -1"):
+        fs.fourfs([8], 1, 0)
+    with pytest.raises(numrs_amd.FournError, match="Invalid dimensions"):
+        fs.fourfs([8], 2, 1)
