@@ -3,10 +3,11 @@
 // streamed through HBM with at most `device_bytes` of device memory.
 //
 //   last axis (contiguous rows)  chunks of whole rows -> batched C2C plan
-//   other axes (stride inner)    column bands of w columns gathered with
-//                                hipMemcpy2D into [L][w], an axis-0 pass
-//                                (column mapping, element stride w), scattered
-//                                back; whole slabs are batched when they fit
+//   other axes (stride inner)    column bands of w columns gathered by the
+//                                CPU into a pinned staging buffer, one DMA to
+//                                [L][w], an axis-0 pass (column mapping,
+//                                element stride w), scattered back the same
+//                                way; whole slabs are batched when they fit
 //   1-D longer than the budget   four-step N = N1 N2 over the host array:
 //                                column bands of [N1][N2] + the W_N^{k1 n2}
 //                                twiddle, then chunks of rows transformed and
@@ -81,7 +82,45 @@ struct FsCtx {
     size_t es;  // bytes per complex value
     hipStream_t s;
     DevBuf buf, ws;
+    char* pinned = nullptr;  // staging for strided gathers / scatters, allocated on first use
+    size_t pinned_bytes = 0;
+    ~FsCtx() {
+        if (pinned) (void)hipHostFree(pinned);
+    }
 };
+
+static int ensure_pinned(FsCtx& c, size_t bytes) {
+    if (bytes <= c.pinned_bytes) return NRFFT_OK;
+    if (c.pinned) (void)hipHostFree(c.pinned);
+    c.pinned = nullptr;
+    c.pinned_bytes = 0;
+    if (hipHostMalloc((void**)&c.pinned, bytes, hipHostMallocDefault) != hipSuccess) {
+        c.pinned = nullptr;
+        return fail(NRFFT_E_NOMEM, "pinned staging buffer");
+    }
+    c.pinned_bytes = bytes;
+    return NRFFT_OK;
+}
+
+// Strided band [rows][w] of a host array with row pitch `pitch` (complex
+// values) <-> the device buffer, through the pinned staging buffer: the CPU
+// gathers / scatters rows, one DMA moves the band.
+static int band_in(FsCtx& c, const char* src, int64_t pitch, int64_t w, int64_t rows) {
+    const size_t rb = size_t(w) * c.es;
+    if (int rc = ensure_pinned(c, size_t(rows) * rb)) return rc;
+    for (int64_t r = 0; r < rows; ++r) std::memcpy(c.pinned + size_t(r) * rb, src + size_t(r * pitch) * c.es, rb);
+    HIPCHK(hipMemcpyAsync(c.buf.p, c.pinned, size_t(rows) * rb, hipMemcpyHostToDevice, c.s));
+    HIPCHK(hipStreamSynchronize(c.s));  // staging free for the next band
+    return NRFFT_OK;
+}
+static int band_out(FsCtx& c, const void* dsrc, char* dst, int64_t pitch, int64_t w, int64_t rows) {
+    const size_t rb = size_t(w) * c.es;
+    if (int rc = ensure_pinned(c, size_t(rows) * rb)) return rc;
+    HIPCHK(hipMemcpyAsync(c.pinned, dsrc, size_t(rows) * rb, hipMemcpyDeviceToHost, c.s));
+    HIPCHK(hipStreamSynchronize(c.s));
+    for (int64_t r = 0; r < rows; ++r) std::memcpy(dst + size_t(r * pitch) * c.es, c.pinned + size_t(r) * rb, rb);
+    return NRFFT_OK;
+}
 
 static Plan fs_plan(int dtype) {
     Plan p;
@@ -140,8 +179,7 @@ static int fs_strided(FsCtx& c, char* host, int64_t L, int64_t inner, int64_t ou
             const int64_t wb = std::min(w, inner - c0);
             Plan p = fs_plan(c.dtype);
             if ((rc = plan_axis0(p, L, wb, 1))) return rc;
-            HIPCHK(hipMemcpy2DAsync(c.buf.p, size_t(wb) * c.es, sb + size_t(c0) * c.es, size_t(inner) * c.es,
-                                    size_t(wb) * c.es, (size_t)L, hipMemcpyHostToDevice, c.s));
+            if ((rc = band_in(c, sb + size_t(c0) * c.es, inner, wb, L))) return rc;
             if ((rc = exec_plan(p, c.buf.p, c.buf.p, nullptr, nullptr, isign, c.s))) return rc;
             if (tw_logn) {
                 const unsigned g = (unsigned)std::min<int64_t>(65536, (L * wb + 255) / 256);
@@ -164,8 +202,7 @@ static int fs_strided(FsCtx& c, char* host, int64_t L, int64_t inner, int64_t ou
                 }
                 HIPCHK(hipGetLastError());
             }
-            HIPCHK(hipMemcpy2DAsync(sb + size_t(c0) * c.es, size_t(inner) * c.es, c.buf.p, size_t(wb) * c.es,
-                                    size_t(wb) * c.es, (size_t)L, hipMemcpyDeviceToHost, c.s));
+            if ((rc = band_out(c, c.buf.p, sb + size_t(c0) * c.es, inner, wb, L))) return rc;
         }
     }
     return NRFFT_OK;
@@ -229,8 +266,7 @@ static int fourfs_host(int dtype, void* data, void* scratch, const size_t* nn, s
                                    (cpx<double>*)tbuf, b, N2);
             HIPCHK(hipGetLastError());
             // tbuf is [N2][b]: row k2 goes to scratch[k2 * N1 + r0 .. + b]
-            HIPCHK(hipMemcpy2DAsync(sc + size_t(r0) * c.es, size_t(N1) * c.es, tbuf, size_t(b) * c.es,
-                                    size_t(b) * c.es, (size_t)N2, hipMemcpyDeviceToHost, c.s));
+            if ((rc = band_out(c, tbuf, sc + size_t(r0) * c.es, N1, b, N2))) return rc;
         }
         HIPCHK(hipStreamSynchronize(c.s));
         std::memcpy(host, scratch, size_t(total) * c.es);
