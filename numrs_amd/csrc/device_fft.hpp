@@ -213,14 +213,14 @@ enum { MAP_J = 0, MAP_T = 1 };
 //   coarse[m] = exp(2 pi i m / 2^13),       m < 2^13
 //   fine[m]   = exp(2 pi i m / 2^26),       m < 2^13
 // exp(2 pi i M / 2^26) = coarse[M >> 13] * fine[M & 8191] for M < 2^26.
-//   stage: per-stage rows for every single-pass length L = 2^1..2^13; the
+//   stage: per-stage rows for every single-group length L = 2^1..2^14; the
 //   stage-s table of L starts at 2L + off_s and holds NS_s rows of R_s entries,
 //   row k = (exp(2 pi i r k / (NS_s R_s)), r = 0..R_s-1), so one thread's
 //   twiddles are contiguous (wide loads) and consecutive k are adjacent.
 constexpr int TW_LOG_COARSE = 13;
 constexpr int TW_COARSE = 1 << TW_LOG_COARSE;
 constexpr int TW_LOG_VIRT = 26;
-constexpr int TW_STAGE_MAX_LOGL = 13;
+constexpr int TW_STAGE_MAX_LOGL = 14;
 constexpr int TW_STAGE_ELEMS = 4 << TW_STAGE_MAX_LOGL;
 
 template <typename T>
@@ -272,7 +272,7 @@ template <typename T, int S, int L, int TI, int NT, int MAPL, int MAPS>
 struct Engine {
     using P = RadixPlan<L>;
     static constexpr int EPT = TI * L / NT;
-    static_assert(L <= (1 << TW_STAGE_MAX_LOGL), "per-stage twiddle tables cover L <= 2^13 only");
+    static_assert(L <= (1 << TW_STAGE_MAX_LOGL), "per-stage twiddle tables cover L <= 2^14 only");
     static_assert(TI * L % NT == 0, "tile must split evenly over threads");
     static constexpr int NST = P::NST;
     static_assert(EPT % P::EPT_MIN == 0 || L < 16, "EPT must cover the largest radix");

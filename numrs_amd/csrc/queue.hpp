@@ -35,11 +35,13 @@ constexpr int qpts() {
     return sizeof(T) == 4 ? NRFFT_QPTS_F32 : NRFFT_QPTS_F64;
 }
 constexpr int Q_MIN_LOGL = 6;
-// Longest pass of a tile: the per-stage twiddle tables' limit, and at most
-// the tile itself.
+// Longest pass of a tile: at most the tile itself and 8192 points (a single
+// 16384-point FP32 tile -- four stages, one transform -- measured slower on
+// the 10,000-item batch than the two-pass four-step it replaces).
+constexpr int Q_MAX_LOGL = 13;
 template <typename T>
 constexpr int q_max_logl() {
-    return ilog2c(qpts<T>()) < TW_STAGE_MAX_LOGL ? ilog2c(qpts<T>()) : TW_STAGE_MAX_LOGL;
+    return ilog2c(qpts<T>()) < Q_MAX_LOGL ? ilog2c(qpts<T>()) : Q_MAX_LOGL;
 }
 constexpr int Q_MODES = 3;  // 0: row (JJ), 1: column (TT), 2: row load / column store (JT)
 

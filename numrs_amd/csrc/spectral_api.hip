@@ -46,6 +46,8 @@ static Launch spec_launch(int special, int src, int dst, std::initializer_list<i
     return l;
 }
 
+static int spec_fused_max(int dtype) { return dtype == NRFFT_F32 ? spec_max_logl<float>() : spec_max_logl<double>(); }
+
 static const SpecRef& spec_lookup(int dtype, int logl) {
     return dtype == NRFFT_F32 ? spec_lookup_f32(logl) : spec_lookup_f64(logl);
 }
@@ -67,7 +69,7 @@ int build_plan_spectral(Plan& p) {
         case PK_TWOFFT: {
             const int64_t n = (int64_t)p.nn.at(0);
             const int logn = ilog2((size_t)n);
-            if (logn <= SPEC_MAX_LOGL) {
+            if (logn <= spec_fused_max(p.dtype)) {
                 p.launches.push_back(spec_launch(SP_TW_FUSED, B_IN, B_OUT, {batch, n}));
                 return NRFFT_OK;
             }
@@ -82,7 +84,7 @@ int build_plan_spectral(Plan& p) {
         case PK_COSFT1: {
             const int64_t n = (int64_t)p.nn.at(0);
             const int logn = ilog2((size_t)n);
-            if (logn <= SPEC_MAX_LOGL) {
+            if (logn <= spec_fused_max(p.dtype)) {
                 p.launches.push_back(spec_launch(SP_CF1_FUSED, B_IN, B_OUT, {batch, n}));
                 return NRFFT_OK;
             }
@@ -102,7 +104,7 @@ int build_plan_spectral(Plan& p) {
                 return NRFFT_OK;
             }
             const int logl = ilog2((size_t)L);
-            if (logl <= SPEC_MAX_LOGL) {
+            if (logl <= spec_fused_max(p.dtype)) {
                 p.launches.push_back(spec_launch(SP_CF2_FUSED, B_IN, B_OUT, {batch, L}));
                 return NRFFT_OK;
             }

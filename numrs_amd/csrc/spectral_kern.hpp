@@ -8,7 +8,13 @@
 
 namespace nrfft {
 
-constexpr int SPEC_MAX_LOGL = 13;  // fused: the FFT of a transform fits one group (L <= 8192)
+// Fused: the FFT of a transform fits one group -- L <= 16384 in FP32 (139 KB
+// of LDS), L <= 8192 in FP64.
+template <typename T>
+constexpr int spec_max_logl() {
+    return sizeof(T) == 4 ? 14 : 13;
+}
+constexpr int SPEC_MAX_LOGL = 14;
 
 struct SpecRef {
     const void* twofft;
@@ -40,8 +46,8 @@ template <typename T>
 struct SpecTable {
     SpecRef t[SPEC_MAX_LOGL + 1];
     SpecTable() {
-        t[0] = SpecRef{nullptr, nullptr, {nullptr, nullptr}, 0, 0};
-        spec_fill<T>(t, std::make_integer_sequence<int, SPEC_MAX_LOGL>{});
+        for (auto& e : t) e = SpecRef{nullptr, nullptr, {nullptr, nullptr}, 0, 0};
+        spec_fill<T>(t, std::make_integer_sequence<int, spec_max_logl<T>()>{});
     }
 };
 

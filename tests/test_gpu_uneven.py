@@ -139,7 +139,7 @@ def test_uneven_full_size_range():
 
 
 @pytest.mark.parametrize("dtype,items,tol", [
-    ("float32", [(1 << 24,), (8192, 64), (64, 8192), (1 << 13,), (1 << 14,)], 3e-6),
+    ("float32", [(1 << 24,), (8192, 64), (64, 8192), (1 << 13,), (1 << 14,), (1 << 15,)], 3e-6),
     ("float64", [(1 << 22,), (4096, 64), (64, 8192), (1 << 13,), (1 << 14,)], 1e-12),
 ])
 def test_uneven_size_limits(dtype, items, tol):
@@ -161,7 +161,7 @@ def test_uneven_size_limits(dtype, items, tol):
         assert rel_l2(yh[seg], _ref(xh[seg].astype(np.float64), it, -1)) < tol, it
 
 
-@pytest.mark.parametrize("dtype,item", [("float32", (1 << 25,)), ("float32", (16384, 64)),
+@pytest.mark.parametrize("dtype,item", [("float32", (1 << 25,)), ("float32", (16384, 64)), ("float32", (64, 16384)),
                                         ("float64", (1 << 23,)), ("float64", (8192, 64)), ("float32", (32,))])
 def test_uneven_rejects_out_of_range(dtype, item):
     with pytest.raises(numrs_amd.NrfftError):
