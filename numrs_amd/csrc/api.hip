@@ -164,6 +164,24 @@ int plan_add_c2c_pass(Plan& p, int mode, int logl, int tilog, const PassDesc& d,
     l.grid = (unsigned)grid;
     l.desc = d;
     l.desc.xcd_remap = p.tune.xcd_remap > 0 ? 1 : 0;
+    // DMA-staged column pass: full 16-wide tiles of 128-byte rows whose
+    // 16-byte pieces stay 16-byte aligned (checked for the pointer at launch).
+    if (!use_mfma && p.dtype == NRFFT_F32 && mode == MODE_TT && kp.ti == DMA_TI && d.si0 == 1 &&
+        d.nb0 % DMA_TI == 0 && d.esi % 2 == 0 && d.si1 % 2 == 0 && d.si2 % 2 == 0 && span_out < (1LL << 28) &&
+        NRFFT_PASS_DMA) {
+        const DmaRef& dp = dma_lookup_f32p(logl);
+        const DmaRef& dm = dma_lookup_f32m(logl);
+        if (dp.fn && dm.fn) {
+            int dcu = 0;
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&dcu, dp.fn, dp.nt, 0));
+            if (dcu >= 1) {
+                l.fn_dma[0] = dp.fn;
+                l.fn_dma[1] = dm.fn;
+                l.nt_dma = dp.nt;
+                l.grid_dma = (unsigned)std::min<int64_t>(ntiles, int64_t(dcu) * ctx->ncu);
+            }
+        }
+    }
     l.src = src;
     l.dst = dst;
     l.lds = kp.lds_bytes;
@@ -357,7 +375,10 @@ int exec_plan(const Plan& p, const void* in, void* out, void* tmp, const void* a
             } else if (p.dtype == NRFFT_F32) {
                 Twiddles<float> tw = ctx_twiddles<float>(ctx);
                 void* args[] = {&a, &b, &d, &tw};
-                HIPCHK(hipLaunchKernel(l.fn[si], dim3(l.grid), dim3(l.nt), args, 0, stream));
+                if (l.fn_dma[si] && (reinterpret_cast<uintptr_t>(a) & 15) == 0)
+                    HIPCHK(hipLaunchKernel(l.fn_dma[si], dim3(l.grid_dma), dim3(l.nt_dma), args, 0, stream));
+                else
+                    HIPCHK(hipLaunchKernel(l.fn[si], dim3(l.grid), dim3(l.nt), args, 0, stream));
             } else {
                 Twiddles<double> tw = ctx_twiddles<double>(ctx);
                 void* args[] = {&a, &b, &d, &tw};

@@ -25,6 +25,24 @@
 
 namespace nrfft {
 
+// Work-group barrier that orders LDS only: wait for this wave's LDS
+// operations, then s_barrier; the asm statement's memory clobber keeps the
+// compiler from moving memory accesses across it.  __syncthreads() is a
+// release / acquire fence on every address space, which on gfx950 waits for
+// all of the wave's outstanding global loads and stores (vmcnt(0)) first; a
+// fence on the local address space alone still waits for outstanding LDS-DMA
+// loads.  This form lets global stores drain and prefetches (register or
+// LDS-DMA) stay in flight across the barrier.
+// Valid wherever a barrier guards LDS exchanges and no thread of the group
+// reads global memory that another thread of the group writes in the same
+// kernel (every kernel here: groups own disjoint output elements, and a
+// group's loads of an element are consumed before the barrier that precedes
+// any store to it).  LDS-DMA data is waited for explicitly (vmcnt) before
+// the barrier that publishes it.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 template <typename T>
 struct alignas(2 * sizeof(T)) cpx {
     T x, y;
@@ -238,6 +256,19 @@ __device__ __forceinline__ cpx<T> twiddle_big(const Twiddles<T>& tw, uint32_t m,
     return dirw<S>(w);
 }
 
+// The two table values whose product twiddle_big forms (loaded now,
+// multiplied later with twiddle_join: same arithmetic, same result).
+template <typename T>
+__device__ __forceinline__ void twiddle_parts(const Twiddles<T>& tw, uint32_t m, int logn, cpx<T>& a, cpx<T>& b) {
+    uint32_t M = m << (TW_LOG_VIRT - logn);
+    a = tw.coarse[M >> TW_LOG_COARSE];
+    b = tw.fine[M & (TW_COARSE - 1)];
+}
+template <int S, typename T>
+__device__ __forceinline__ cpx<T> twiddle_join(cpx<T> a, cpx<T> b) {
+    return dirw<S>(cmul(a, b));
+}
+
 // out[r] = W^{S (m0 + r m1)} for r < R (W = exp(2 pi i / 2^logn)), from table
 // values of W^{m0} and W^{2^q m1}; out[r] = out[r - lowbit(r)] * W^{lowbit(r) m1},
 // so every value is at most popcount(r) + 1 products away from the tables.
@@ -355,7 +386,28 @@ struct Engine {
         }
     }
 
-    template <int s>
+    // LDS copy of the stage twiddles of stages 1..NST-1 (stage 0 has none):
+    // the rows of stage s start at twoff(s), R_s + 2 entries apart, so the
+    // 16-byte row reads of up to 8 consecutive rows hit distinct banks.
+    static constexpr int twrow(int s) { return P::radix(s) + 2; }
+    static constexpr int twoff(int s) {
+        int o = 0;
+        for (int q = 1; q < s; ++q) o += P::ns(q) * twrow(q);
+        return o;
+    }
+    static constexpr int TW_ELEMS = twoff(NST);
+    template <int s = 1>
+    static __device__ __forceinline__ void stage_tw_to_lds(cpx<T>* ltw, const cpx<T>* stage_tw, int tid) {
+        if constexpr (s < NST) {
+            constexpr int R = P::radix(s), NS = P::ns(s);
+            const cpx<T>* src = stage_tw + 2 * L + StageTable<L>::off(s);
+            for (int e = tid; e < NS * R; e += NT) ltw[twoff(s) + (e / R) * twrow(s) + (e % R)] = src[e];
+            stage_tw_to_lds<s + 1>(ltw, stage_tw, tid);
+        }
+    }
+
+    // LT: the table is the LDS copy above, else the global stage table.
+    template <int s, bool LT = false>
     static __device__ __forceinline__ void compute(cpx<T>* v, int tid, const cpx<T>* __restrict__ stage_tw) {
         constexpr int R = P::radix(s), NB = L / R, BPT = EPT / R, NS = P::ns(s);
 #pragma unroll
@@ -364,7 +416,8 @@ struct Engine {
                 int t, j;
                 bmap<map_of(s), NB>(tid + b * NT, t, j);
                 // Row k = j mod NS of this stage's table: W_{NS R}^{r k}, r < R.
-                const cpx<T>* row = stage_tw + 2 * L + StageTable<L>::off(s) + (j % NS) * R;
+                const cpx<T>* row = LT ? stage_tw + twoff(s) + (j % NS) * twrow(s)
+                                       : stage_tw + 2 * L + StageTable<L>::off(s) + (j % NS) * R;
 #pragma unroll
                 for (int r = 1; r < R; ++r) v[b * R + r] = cmul(v[b * R + r], dirw<S>(row[r]));
             }
@@ -412,16 +465,16 @@ struct Engine {
     // Stages 1..NST-1 (stage 0 has been computed on v).  The leading barrier
     // of each exchange protects the LDS buffer against the previous stage's
     // readers (WAR); the trailing one publishes the writes (RAW).
-    template <int s = 1>
+    template <int s = 1, bool LT = false>
     static __device__ __forceinline__ void middle(cpx<T>* v, int tid, cpx<T>* lds, const cpx<T>* stage_tw,
                                                   bool lead_barrier) {
         if constexpr (s < NST) {
-            if (lead_barrier) __syncthreads();
+            if (lead_barrier) lds_barrier();
             store_lds<s - 1>(v, tid, lds);
-            __syncthreads();
+            lds_barrier();
             load_lds<s>(v, tid, lds);
-            compute<s>(v, tid, stage_tw);
-            middle<s + 1>(v, tid, lds, stage_tw, true);
+            compute<s, LT>(v, tid, stage_tw);
+            middle<s + 1, LT>(v, tid, lds, stage_tw, true);
         }
     }
 
@@ -457,10 +510,10 @@ struct Engine {
         load0(v, tid, ld);
         compute<0>(v, tid, stage_tw);
         middle<1>(v, tid, lds, stage_tw, lead_barrier);
-        if constexpr (NST > 1) __syncthreads();
-        else if (lead_barrier) __syncthreads();
+        if constexpr (NST > 1) lds_barrier();
+        else if (lead_barrier) lds_barrier();
         store_lds<NST - 1>(v, tid, lds);
-        __syncthreads();
+        lds_barrier();
     }
 
     // Transform whose natural-order input is already in LDS (a barrier must

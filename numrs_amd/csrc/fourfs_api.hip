@@ -52,7 +52,7 @@ __global__ void k_fs_transpose(const cpx<T>* in, cpx<T>* out, int64_t rows, int6
         const int64_t r = tr + k, c = tc + tx;
         if (r < rows && c < cols) tile[k][tx] = in[r * cols + c];
     }
-    __syncthreads();
+    lds_barrier();
     for (int k = ty; k < 32; k += 8) {
         const int64_t c = tc + k, r = tr + tx;
         if (r < rows && c < cols) out[c * rows + r] = tile[tx][k];

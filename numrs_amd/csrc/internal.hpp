@@ -15,6 +15,8 @@ const KernelRef& c2c_lookup_f32m(int mode, int logl, int tilog);
 const KernelRef& c2c_lookup_f64p(int mode, int logl, int tilog);
 const KernelRef& c2c_lookup_f64m(int mode, int logl, int tilog);
 const KernelRef& c2c_lookup(int dtype, int sign, int mode, int logl, int tilog);
+const DmaRef& dma_lookup_f32p(int logl);
+const DmaRef& dma_lookup_f32m(int logl);
 int pick_tilog(int dtype, int mode, int logl, int64_t nb0, int want_log);
 
 // Per-device state: twiddle tables (FP64-generated; FP32 copies rounded).
@@ -84,6 +86,11 @@ struct Launch {
     int src = B_IN;
     int dst = B_OUT;
     int prec = 0;         // 0 = plain kernel, else split-precision MFMA kernel (MfmaPrec)
+    // DMA-staged variant of an FP32 MODE_TT pass (k_c2c_dma), used when the
+    // input pointer is 16-byte aligned; nullptr when the pass does not qualify.
+    const void* fn_dma[2] = {nullptr, nullptr};
+    unsigned grid_dma = 0;
+    int nt_dma = 0;
     int64_t off_src = 0;  // element offsets (complex units) into src / dst
     int64_t off_dst = 0;
     size_t bytes = 0;   // LK_COPY

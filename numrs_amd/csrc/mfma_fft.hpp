@@ -157,7 +157,7 @@ __device__ __forceinline__ void mfma_stages(cpx<float>* lds, unsigned lane, unsi
             for (int e = 0; e < 4; ++e) x[g][e] = cmul(lds[E::lidx(t, j + (4 * kb + e) * NB)], dirw<S>(row[e]));
         }
     }
-    if constexpr (!LAST) __syncthreads();  // every read of this stage done before its writes
+    if constexpr (!LAST) lds_barrier();  // every read of this stage done before its writes
 #pragma unroll
     for (unsigned g = 0; g < GPW; ++g) {
         unsigned t, j;
@@ -207,7 +207,7 @@ __device__ __forceinline__ void mfma_stages(cpx<float>* lds, unsigned lane, unsi
         }
     }
     if constexpr (!LAST) {
-        __syncthreads();
+        lds_barrier();
         mfma_stages<PREC, S, L, NT, ML, MS, s + 1>(lds, lane, wave, nvalid, tw, a0, a1, ob, so0, eso, d, b0base,
                                                     out_scale);
     }
@@ -282,9 +282,9 @@ k_c2c_mfma(const c32* in, c32* out, PassDesc d, Twiddles<float> tw, const void* 
             for (int e = 0; e < 16; ++e) m = fmaxf(m, fmaxf(fabsf(v[e].x), fabsf(v[e].y)));
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-            if (!first) __syncthreads();  // red[] and LDS read by the previous tile
+            if (!first) lds_barrier();  // red[] and LDS read by the previous tile
             if (lane == 0) red[wave] = m;
-            __syncthreads();
+            lds_barrier();
             m = 0.f;
 #pragma unroll
             for (int w = 0; w < W; ++w) m = fmaxf(m, red[w]);
@@ -295,11 +295,11 @@ k_c2c_mfma(const c32* in, c32* out, PassDesc d, Twiddles<float> tw, const void* 
 #pragma unroll
             for (int e = 0; e < 16; ++e) v[e] = cscale(v[e], sc);
         } else {
-            if (!first) __syncthreads();
+            if (!first) lds_barrier();
         }
         E::template compute<0>(v, tid, tw.stage);
         E::template store_lds<0>(v, tid, lds);
-        __syncthreads();
+        lds_barrier();
 
         // ---- radix-16 stages on the matrix cores ----------------------------
         mfma_stages<PREC, S, L, NT, ML, MS, 1>(lds, (unsigned)lane, (unsigned)wave, nvalid, tw, ar, ai, ob, so0, eso,

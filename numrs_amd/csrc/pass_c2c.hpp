@@ -169,4 +169,16 @@ struct KernelRef {
     int lds_bytes;    // static LDS
 };
 
+// DMA-staged FP32 column pass (k_c2c_dma): TI = 16, nt = L threads.
+// NRFFT_PASS_DMA = 0 builds the library without it (k_c2c everywhere).
+#ifndef NRFFT_PASS_DMA
+#define NRFFT_PASS_DMA 1
+#endif
+constexpr int DMA_TI = 16;
+struct DmaRef {
+    const void* fn;
+    int nt;
+    int lds_bytes;
+};
+
 }  // namespace nrfft

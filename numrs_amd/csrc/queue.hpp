@@ -140,9 +140,9 @@ __global__ __launch_bounds__(qpts<T>() / 16) void k_queue(cpx<T>* in, cpx<T>* ou
     cpx<T>* bufs[3] = {in, out, ws};
     const int tid = threadIdx.x;
     for (;;) {
-        __syncthreads();  // previous task's LDS readers are done; sh_task free
+        lds_barrier();  // previous task's LDS readers are done; sh_task free
         if (tid == 0) sh_task = (int)atomicAdd(&st->next[phase], 1u);
-        __syncthreads();
+        lds_barrier();
         const int k = sh_task;
         if (k >= count) break;
         const QTask q = tasks[base + k];

@@ -128,7 +128,7 @@ __global__ __launch_bounds__(NT) void k_realft(const cpx<T>* in, cpx<T>* out, in
                 lds[E::lidx(t, L - p)] = b;
             }
         }
-        __syncthreads();
+        lds_barrier();
         E::run_from_lds(tid, lds, tw.stage, [&](int t, int k, cpx<T> v) {
             if (TI == 1 || t < nvalid) ob[t * L + k] = v;
         });
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(NT) void k_convlv(const cpx<T>* data, const T* seco
                           xa[u], xb[u]);
         }
     }
-    __syncthreads();
+    lds_barrier();
 
     // 2. Spectrum of the response (or of data2).
     if (op == OP_CORREL) {
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(NT) void k_convlv(const cpx<T>* data, const T* seco
             untangle_pair(ya, yb, conj(w), (T)0.5, xa[u], xb[u]);
         }
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {
         const int q = tid + u * NT;
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(NT) void k_convlv(const cpx<T>* data, const T* seco
             lds[EF::lidx(t, L - p)] = xb[u];
         }
     }
-    __syncthreads();
+    lds_barrier();
 
     // 3. Inverse FFT: (n/2) * (result / no2) = result.
     EI::run_from_lds(tid, lds, tw.stage, [&](int t, int k, cpx<T> v) {

@@ -185,7 +185,7 @@ __global__ __launch_bounds__(NT) void k_cosft2(const T* y, T* out, int64_t nsig,
                 lds[E::lidx(t, L - p)] = zb;
             }
         }
-        __syncthreads();
+        lds_barrier();
         E::run_from_lds(tid, lds, tw.stage, [&](int t, int m, cpx<T> v) {
             if (TI == 1 || t < nvalid) {
                 T* os = out + (s0 + t) * N;

@@ -162,6 +162,38 @@ def test_fourn_f32(shape):
     assert rel_l2(y, oracle.fourn(x, list(shape), 1)) < TOL32
 
 
+@pytest.mark.parametrize("shape", [(512, 64), (256, 8), (2, 256, 32)])
+def test_fourn_f32_column_pass_tiles(shape):
+    # Strided passes of 256 / 512 points: 16-wide tiles take the
+    # LDS-DMA-staged column kernel (512 x 64, and 2 x 256 x 32 with a batch
+    # stride), an 8-wide tile (256 x 8) the plain one.
+    x = f32round(synth.normal(7, 2 * int(np.prod(shape))))
+    for isign in (1, -1):
+        y = x.astype(np.float32)
+        numrs_amd.fourn(y, list(shape), len(shape), isign)
+        assert rel_l2(y, oracle.fourn(x, list(shape), isign)) < TOL32, (shape, isign)
+
+
+def test_four1_f32_device_plan_alignment():
+    # Four-step at N = 2^18 (first pass over 512-point columns) on device
+    # buffers: input 16-byte aligned (LDS-DMA-staged pass) and offset by one
+    # complex value, 8 bytes (plain pass); both against the oracle.
+    import torch
+
+    n, batch = 1 << 18, 2
+    x = f32round(synth.uniform(11, 2 * n * batch))
+    ref = np.concatenate([oracle.four1_nr(x[i * 2 * n:(i + 1) * 2 * n], 1) for i in range(batch)])
+    plan = numrs_amd.C2CPlan("float32", [n], batch)
+    for off in (0, 2):
+        buf = torch.zeros(2 * n * batch + 4, dtype=torch.float32, device="cuda")
+        xin = buf[off:off + 2 * n * batch]
+        xin.copy_(torch.from_numpy(x.astype(np.float32)))
+        out = torch.empty(2 * n * batch, dtype=torch.float32, device="cuda")
+        plan.execute(xin, out, 1)
+        torch.cuda.synchronize()
+        assert rel_l2(out.cpu().numpy().astype(np.float64), ref) < TOL32, off
+
+
 # ---- convlv ------------------------------------------------------------------
 def test_convlv_reference_case():
     # Convolve.rs:346-360 (circular: index 0 is 1 + 4 = 5).
