@@ -26,7 +26,7 @@ KernelRef c2c_ref_one() {
     constexpr int ML = (MODE == MODE_JJ || MODE == MODE_JT) ? MAP_J : MAP_T;
     constexpr int MS = (MODE == MODE_JJ || MODE == MODE_TJ) ? MAP_J : MAP_T;
     constexpr int NST = RadixPlan<L>::NST;
-    constexpr int LDS = (NST > 1 ? Engine<T, S, L, TI, NT, ML, MS>::LDS_ELEMS : 1) * (int)sizeof(cpx<T>);
+    constexpr int LDS = NST > 1 ? kc2c_lds_bytes<T, S, L, TI, NT, ML, MS>() : (int)sizeof(cpx<T>);
     constexpr bool ok = (MODE != MODE_JJ || TILOG == 0) && NT >= 1 && NT <= 1024 && LDS <= lds_limit_bytes();
     if constexpr (ok) {
         return KernelRef{reinterpret_cast<const void*>(&k_c2c<T, S, L, TI, NT, ML, MS>), NT, TI, LDS};

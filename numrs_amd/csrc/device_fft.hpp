@@ -289,6 +289,9 @@ __device__ __forceinline__ void twiddle_run(const Twiddles<T>& tw, uint32_t m0, 
 // ---------------------------------------------------------------------------
 // The stage engine.
 // ---------------------------------------------------------------------------
+// Stage limit meaning "every stage" for the LDS twiddle copy (Engine::compute).
+constexpr int LT_ALL = 64;
+
 template <int L>
 struct StageTable {
     // offset of stage s inside L's block (which starts at 2L)
@@ -396,27 +399,32 @@ struct Engine {
         return o;
     }
     static constexpr int TW_ELEMS = twoff(NST);
-    template <int s = 1>
+    // Entries of the LDS copy that holds stages 1 .. min(LTM, NST-1) only.
+    static constexpr int tw_elems(int ltm) { return twoff((ltm < NST - 1 ? ltm : NST - 1) + 1); }
+    template <int LTM = LT_ALL, int s = 1>
     static __device__ __forceinline__ void stage_tw_to_lds(cpx<T>* ltw, const cpx<T>* stage_tw, int tid) {
-        if constexpr (s < NST) {
+        if constexpr (s < NST && s <= LTM) {
             constexpr int R = P::radix(s), NS = P::ns(s);
             const cpx<T>* src = stage_tw + 2 * L + StageTable<L>::off(s);
             for (int e = tid; e < NS * R; e += NT) ltw[twoff(s) + (e / R) * twrow(s) + (e % R)] = src[e];
-            stage_tw_to_lds<s + 1>(ltw, stage_tw, tid);
+            stage_tw_to_lds<LTM, s + 1>(ltw, stage_tw, tid);
         }
     }
 
-    // LT: the table is the LDS copy above, else the global stage table.
-    template <int s, bool LT = false>
-    static __device__ __forceinline__ void compute(cpx<T>* v, int tid, const cpx<T>* __restrict__ stage_tw) {
+    // Stages 1 .. LTM read their twiddles from the LDS copy ltw, the others
+    // from the global stage table.
+    template <int s, int LTM = 0>
+    static __device__ __forceinline__ void compute(cpx<T>* v, int tid, const cpx<T>* __restrict__ stage_tw,
+                                                   const cpx<T>* __restrict__ ltw = nullptr) {
         constexpr int R = P::radix(s), NB = L / R, BPT = EPT / R, NS = P::ns(s);
+        constexpr bool LT = s >= 1 && s <= LTM;
 #pragma unroll
         for (int b = 0; b < BPT; ++b) {
             if constexpr (NS > 1) {
                 int t, j;
                 bmap<map_of(s), NB>(tid + b * NT, t, j);
                 // Row k = j mod NS of this stage's table: W_{NS R}^{r k}, r < R.
-                const cpx<T>* row = LT ? stage_tw + twoff(s) + (j % NS) * twrow(s)
+                const cpx<T>* row = LT ? ltw + twoff(s) + (j % NS) * twrow(s)
                                        : stage_tw + 2 * L + StageTable<L>::off(s) + (j % NS) * R;
 #pragma unroll
                 for (int r = 1; r < R; ++r) v[b * R + r] = cmul(v[b * R + r], dirw<S>(row[r]));
@@ -465,16 +473,16 @@ struct Engine {
     // Stages 1..NST-1 (stage 0 has been computed on v).  The leading barrier
     // of each exchange protects the LDS buffer against the previous stage's
     // readers (WAR); the trailing one publishes the writes (RAW).
-    template <int s = 1, bool LT = false>
+    template <int s = 1, int LTM = 0>
     static __device__ __forceinline__ void middle(cpx<T>* v, int tid, cpx<T>* lds, const cpx<T>* stage_tw,
-                                                  bool lead_barrier) {
+                                                  bool lead_barrier, const cpx<T>* ltw = nullptr) {
         if constexpr (s < NST) {
             if (lead_barrier) lds_barrier();
             store_lds<s - 1>(v, tid, lds);
             lds_barrier();
             load_lds<s>(v, tid, lds);
-            compute<s, LT>(v, tid, stage_tw);
-            middle<s + 1, LT>(v, tid, lds, stage_tw, true);
+            compute<s, LTM>(v, tid, stage_tw, ltw);
+            middle<s + 1, LTM>(v, tid, lds, stage_tw, true, ltw);
         }
     }
 
@@ -503,13 +511,13 @@ struct Engine {
     // Transform whose output is left in LDS in natural order (index lidx(t, k)).
     // lead_barrier: a barrier is needed before the first LDS write (the buffer
     // was read earlier by other threads).
-    template <class LD>
+    template <int LTM = 0, class LD>
     static __device__ __forceinline__ void run_to_lds(int tid, cpx<T>* lds, const cpx<T>* stage_tw, LD&& ld,
-                                                      bool lead_barrier = false) {
+                                                      bool lead_barrier = false, const cpx<T>* ltw = nullptr) {
         cpx<T> v[EPT];
         load0(v, tid, ld);
         compute<0>(v, tid, stage_tw);
-        middle<1>(v, tid, lds, stage_tw, lead_barrier);
+        middle<1, LTM>(v, tid, lds, stage_tw, lead_barrier, ltw);
         if constexpr (NST > 1) lds_barrier();
         else if (lead_barrier) lds_barrier();
         store_lds<NST - 1>(v, tid, lds);
@@ -518,12 +526,13 @@ struct Engine {
 
     // Transform whose natural-order input is already in LDS (a barrier must
     // separate the writes of that input from this call).
-    template <class ST>
-    static __device__ __forceinline__ void run_from_lds(int tid, cpx<T>* lds, const cpx<T>* stage_tw, ST&& st) {
+    template <int LTM = 0, class ST>
+    static __device__ __forceinline__ void run_from_lds(int tid, cpx<T>* lds, const cpx<T>* stage_tw, ST&& st,
+                                                        const cpx<T>* ltw = nullptr) {
         cpx<T> v[EPT];
         load_lds<0>(v, tid, lds);
         compute<0>(v, tid, stage_tw);
-        middle<1>(v, tid, lds, stage_tw, true);
+        middle<1, LTM>(v, tid, lds, stage_tw, true, ltw);
         store_last(v, tid, per_elem(st));
     }
 };

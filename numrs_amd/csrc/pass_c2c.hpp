@@ -46,12 +46,35 @@ constexpr int waves_per_eu_target() {
     constexpr int cap = PF ? 4 : 8;
     return w < 1 ? 1 : (w > cap ? cap : w);
 }
-template <typename T, int L, int TI, int NT>
-constexpr int kc2c_waves =
-    waves_per_eu_target<(RadixPlan<L>::NST > 1 ? (TI * L + (NRFFT_LDS_SWIZZLE ? 0 : TI * L / NRFFT_LDS_PAD + 1)) *
-                                                     (int)sizeof(cpx<T>)
-                                               : 0),
-                        NT, NRFFT_PREFETCH != 0>();
+// Stages 1 .. kc2c_ltm read their twiddles from an LDS copy: the most
+// stages whose copy fits beside the exchange buffer without lowering the
+// groups a CU holds (LDS- or wave-limited).  An LDS read replaces an L2 load
+// whose latency each exchange otherwise exposes.  NRFFT_PASS_LDS_TW = 0
+// reads every stage's twiddles from the global table.
+#ifndef NRFFT_PASS_LDS_TW
+#define NRFFT_PASS_LDS_TW 1
+#endif
+template <typename T, int S, int L, int TI, int NT, int ML, int MS>
+constexpr int kc2c_ltm() {
+    using E = Engine<T, S, L, TI, NT, ML, MS>;
+    if (!NRFFT_PASS_LDS_TW || E::NST <= 1) return 0;
+    const int xb = E::LDS_ELEMS * (int)sizeof(cpx<T>);
+    const int by_waves = 32 / (NT < 64 ? 1 : NT / 64);
+    const int g0 = (163840 / xb < by_waves) ? 163840 / xb : by_waves;
+    for (int ltm = E::NST - 1; ltm >= 1; --ltm) {
+        const int by_lds = 163840 / (xb + E::tw_elems(ltm) * (int)sizeof(cpx<T>));
+        if ((by_lds < by_waves ? by_lds : by_waves) >= g0) return ltm;
+    }
+    return 0;
+}
+template <typename T, int S, int L, int TI, int NT, int ML, int MS>
+constexpr int kc2c_lds_bytes() {
+    using E = Engine<T, S, L, TI, NT, ML, MS>;
+    constexpr int ltm = kc2c_ltm<T, S, L, TI, NT, ML, MS>();
+    return E::NST > 1 ? (E::LDS_ELEMS + (ltm > 0 ? E::tw_elems(ltm) : 0)) * (int)sizeof(cpx<T>) : 0;
+}
+template <typename T, int S, int L, int TI, int NT, int ML, int MS>
+constexpr int kc2c_waves = waves_per_eu_target<kc2c_lds_bytes<T, S, L, TI, NT, ML, MS>(), NT, NRFFT_PREFETCH != 0>();
 
 // Thread index laundered through a register move the compiler cannot hoist,
 // so per-tile address arithmetic is recomputed inside the persistent loop
@@ -63,11 +86,13 @@ __device__ __forceinline__ int fresh_tid(int tid) {
 }
 
 template <typename T, int S, int L, int TI, int NT, int ML, int MS>
-__global__ __attribute__((amdgpu_flat_work_group_size(1, NT), amdgpu_waves_per_eu(kc2c_waves<T, L, TI, NT>))) void
+__global__ __attribute__((amdgpu_flat_work_group_size(1, NT), amdgpu_waves_per_eu(kc2c_waves<T, S, L, TI, NT, ML, MS>))) void
 k_c2c(const cpx<T>* in, cpx<T>* out, PassDesc d, Twiddles<T> tw) {
     using E = Engine<T, S, L, TI, NT, ML, MS>;
     constexpr int EPT = E::EPT;
+    constexpr int LTM = kc2c_ltm<T, S, L, TI, NT, ML, MS>();
     __shared__ cpx<T> lds[E::NST > 1 ? E::LDS_ELEMS : 1];
+    __shared__ __align__(16) cpx<T> ltw[LTM > 0 ? E::tw_elems(LTM) : 1];
     const int tid = threadIdx.x;
     const int64_t tiles0 = (d.nb0 + TI - 1) / TI;
     const int64_t ntiles = tiles0 * d.nb1 * d.nb2;
@@ -102,6 +127,8 @@ k_c2c(const cpx<T>* in, cpx<T>* out, PassDesc d, Twiddles<T> tw) {
         });
     };
 
+    // Published by the first exchange barrier, before stage 1 reads it.
+    if constexpr (LTM > 0) E::template stage_tw_to_lds<LTM>(ltw, tw.stage, tid);
 #if NRFFT_PREFETCH
     Ctx cn = ctx_of(tile);
     cpx<T> vn[EPT];
@@ -129,7 +156,7 @@ k_c2c(const cpx<T>* in, cpx<T>* out, PassDesc d, Twiddles<T> tw) {
         load(v, c, ltid);
 #endif
         E::template compute<0>(v, ltid, tw.stage);
-        E::template middle<1>(v, ltid, lds, tw.stage, !first);
+        E::template middle<1, LTM>(v, ltid, lds, tw.stage, !first, ltw);
         if (d.tw_logn) {
             const int logn = d.tw_logn, sh = d.tw_b0_shift;
             const uint32_t mask = (1u << logn) - 1u;

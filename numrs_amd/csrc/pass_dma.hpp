@@ -53,7 +53,7 @@ __device__ __forceinline__ void dma_exchanges(c32* v, int tid, c32* lds, const c
         lds_barrier();
         E::template load_lds<s>(v, tid, lds);
         if constexpr (s < LAST) {
-            E::template compute<s, true>(v, tid, ltw);
+            E::template compute<s, LT_ALL>(v, tid, ltw, ltw);
             dma_exchanges<E, s + 1, LAST>(v, tid, lds, ltw);
         }
     }
@@ -186,15 +186,15 @@ k_c2c_dma(const c32* in, c32* out, PassDesc d, Twiddles<float> tw) {
                 dma(cn.ib);
             }
             E::template load_lds<1>(v, ltid, lds);
-            E::template compute<1, true>(v, ltid, ltw);
-            E::template middle<2, true>(v, ltid, lds, ltw, true);
+            E::template compute<1, LT_ALL>(v, ltid, tw.stage, ltw);
+            E::template middle<2, LT_ALL>(v, ltid, lds, tw.stage, true, ltw);
         } else {
             // Stages 1 .. NST-2 through the exchange buffer, then the last
             // stage's inputs; once every wave has read them the buffer takes
             // the next tile while the last stage computes and stores.
             E::template load_lds<1>(v, ltid, lds);
             if constexpr (E::NST > 2) {
-                E::template compute<1, true>(v, ltid, ltw);
+                E::template compute<1, LT_ALL>(v, ltid, tw.stage, ltw);
                 dma_exchanges<E, 2, E::NST - 1>(v, ltid, lds, ltw);
             }
             lds_barrier();
@@ -202,7 +202,7 @@ k_c2c_dma(const c32* in, c32* out, PassDesc d, Twiddles<float> tw) {
                 cn = ctx_of(next);
                 dma(cn.ib);
             }
-            E::template compute<E::NST - 1, true>(v, ltid, ltw);
+            E::template compute<E::NST - 1, LT_ALL>(v, ltid, tw.stage, ltw);
         }
 
         // Output k = j + r NBL of transform t: tile base c.ob (scalar

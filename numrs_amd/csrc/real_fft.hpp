@@ -141,8 +141,19 @@ __global__ __launch_bounds__(NT) void k_realft(const cpx<T>* in, cpx<T>* out, in
 // Convolve.rs:41-63) for OP_CONV / OP_DECONV, or data2 (n values) for
 // OP_CORREL.  ans may alias data.
 // ---------------------------------------------------------------------------
+// Waves per SIMD k_convlv is compiled for.  Capping the FP32 kernels at 4
+// (128 VGPRs) lets two 512-thread groups share a CU at L = 8192 (they need
+// 168 VGPRs otherwise): 0.42 -> 0.33 ms for 4096 x 2^14 on MI355X, with the
+// signal spectrum partly spilled across the response FFT; also faster at
+// L = 512.  At L = 32..128 the cap costs 33-41 spilled VGPRs and 25-35%, at
+// L = 1024..4096 it measured equal, so those keep the compiler's choice.
+template <typename T, int L>
+constexpr int conv_waves() {
+    return (sizeof(T) == 4 && (L == 512 || L == 8192)) ? 4 : 1;
+}
+
 template <typename T, int L, int TI, int NT>
-__global__ __launch_bounds__(NT) void k_convlv(const cpx<T>* data, const T* second, cpx<T>* ans, int64_t nsig,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(conv_waves<T, L>()))) void k_convlv(const cpx<T>* data, const T* second, cpx<T>* ans, int64_t nsig,
                                                int64_t m, int op, Twiddles<T> tw) {
     using EF = Engine<T, 1, L, TI, NT, MAP_J, MAP_J>;
     using EI = Engine<T, -1, L, TI, NT, MAP_J, MAP_J>;
@@ -151,18 +162,23 @@ __global__ __launch_bounds__(NT) void k_convlv(const cpx<T>* data, const T* seco
     constexpr int PPT = (TI * NP + NT - 1) / NT;
     constexpr int LOG2N = ilog2c(L) + 1;
     constexpr int64_t N = 2 * int64_t(L);
+    // Twiddles of every stage but the last come from an LDS copy (a few KB):
+    // shorter latency than the L2 and fewer registers held while they load.
+    constexpr int LTM = EF::NST >= 2 ? EF::NST - 2 : 0;
     __shared__ cpx<T> lds[EF::LDS_ELEMS];
+    __shared__ __align__(16) cpx<T> ltw[EF::tw_elems(LTM) > 0 ? EF::tw_elems(LTM) : 1];
     const int tid = threadIdx.x;
     const int64_t s0 = int64_t(blockIdx.x) * TI;
     const int nvalid = (nsig - s0) < TI ? int(nsig - s0) : TI;
     const cpx<T>* db = data + s0 * L;
     cpx<T>* ob = ans + s0 * L;
     const T inv_no2 = (T)1 / (T)L;
+    EF::template stage_tw_to_lds<LTM>(ltw, tw.stage, tid);  // published by the first exchange barrier
 
     // 1. Spectrum of the signal, kept in registers pair by pair.
-    EF::run_to_lds(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
+    EF::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
         return (TI == 1 || t < nvalid) ? db[t * L + i] : cpx<T>{(T)0, (T)0};
-    });
+    }, false, ltw);
     cpx<T> xa[PPT], xb[PPT];
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {
@@ -182,16 +198,16 @@ __global__ __launch_bounds__(NT) void k_convlv(const cpx<T>* data, const T* seco
     // 2. Spectrum of the response (or of data2).
     if (op == OP_CORREL) {
         const cpx<T>* sb = reinterpret_cast<const cpx<T>*>(second) + s0 * L;
-        EF::run_to_lds(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
+        EF::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
             return (TI == 1 || t < nvalid) ? sb[t * L + i] : cpx<T>{(T)0, (T)0};
-        });
+        }, false, ltw);
     } else {
         const T* rb = second + s0 * m;
-        EF::run_to_lds(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
+        EF::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
             if (TI > 1 && t >= nvalid) return cpx<T>{(T)0, (T)0};
             const T* r = rb + t * m;
             return cpx<T>{placed_response(r, 2 * int64_t(i), N, m), placed_response(r, 2 * int64_t(i) + 1, N, m)};
-        });
+        }, false, ltw);
     }
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {
@@ -232,9 +248,9 @@ __global__ __launch_bounds__(NT) void k_convlv(const cpx<T>* data, const T* seco
     lds_barrier();
 
     // 3. Inverse FFT: (n/2) * (result / no2) = result.
-    EI::run_from_lds(tid, lds, tw.stage, [&](int t, int k, cpx<T> v) {
+    EI::template run_from_lds<LTM>(tid, lds, tw.stage, [&](int t, int k, cpx<T> v) {
         if (TI == 1 || t < nvalid) ob[t * L + k] = v;
-    });
+    }, ltw);
 }
 
 }  // namespace nrfft
