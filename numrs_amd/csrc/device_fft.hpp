@@ -306,6 +306,8 @@ template <typename T, int S, int L, int TI, int NT, int MAPL, int MAPS>
 struct Engine {
     using P = RadixPlan<L>;
     static constexpr int EPT = TI * L / NT;
+    static constexpr int THREADS = NT;
+    static constexpr int CSIZE = (int)sizeof(cpx<T>);
     static_assert(L <= (1 << TW_STAGE_MAX_LOGL), "per-stage twiddle tables cover L <= 2^14 only");
     static_assert(TI * L % NT == 0, "tile must split evenly over threads");
     static constexpr int NST = P::NST;
@@ -536,5 +538,22 @@ struct Engine {
         store_last(v, tid, per_elem(st));
     }
 };
+
+// Largest LTM (stages 1 .. LTM read twiddles from an LDS copy) whose copy
+// fits beside the kernel's other LDS bytes without lowering the groups a CU
+// holds (MI355X: 160 KB LDS, at most 32 waves per CU); 0 = none.
+constexpr int groups_per_cu(int lds_bytes, int nt) {
+    const int by_waves = 32 / (nt < 64 ? 1 : nt / 64);
+    const int by_lds = lds_bytes > 0 ? 163840 / lds_bytes : by_waves;
+    return by_lds < by_waves ? by_lds : by_waves;
+}
+template <class E>
+constexpr int engine_ltm(int other_lds_bytes) {
+    if (E::NST <= 1) return 0;
+    const int g0 = groups_per_cu(other_lds_bytes, E::THREADS);
+    for (int ltm = E::NST - 1; ltm >= 1; --ltm)
+        if (groups_per_cu(other_lds_bytes + E::tw_elems(ltm) * E::CSIZE, E::THREADS) >= g0) return ltm;
+    return 0;
+}
 
 }  // namespace nrfft

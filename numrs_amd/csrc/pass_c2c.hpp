@@ -57,15 +57,7 @@ constexpr int waves_per_eu_target() {
 template <typename T, int S, int L, int TI, int NT, int ML, int MS>
 constexpr int kc2c_ltm() {
     using E = Engine<T, S, L, TI, NT, ML, MS>;
-    if (!NRFFT_PASS_LDS_TW || E::NST <= 1) return 0;
-    const int xb = E::LDS_ELEMS * (int)sizeof(cpx<T>);
-    const int by_waves = 32 / (NT < 64 ? 1 : NT / 64);
-    const int g0 = (163840 / xb < by_waves) ? 163840 / xb : by_waves;
-    for (int ltm = E::NST - 1; ltm >= 1; --ltm) {
-        const int by_lds = 163840 / (xb + E::tw_elems(ltm) * (int)sizeof(cpx<T>));
-        if ((by_lds < by_waves ? by_lds : by_waves) >= g0) return ltm;
-    }
-    return 0;
+    return (NRFFT_PASS_LDS_TW && E::NST > 1) ? engine_ltm<E>(E::LDS_ELEMS * E::CSIZE) : 0;
 }
 template <typename T, int S, int L, int TI, int NT, int ML, int MS>
 constexpr int kc2c_lds_bytes() {

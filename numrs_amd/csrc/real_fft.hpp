@@ -90,17 +90,20 @@ __global__ __launch_bounds__(NT) void k_realft(const cpx<T>* in, cpx<T>* out, in
     using E = Engine<T, DIR, L, TI, NT, MAP_J, MAP_J>;
     constexpr int NP = L / 2;  // pairs per signal
     constexpr int LOG2L = ilog2c(L) + 1;
+    constexpr int LTM = engine_ltm<E>(E::LDS_ELEMS * E::CSIZE);
     __shared__ cpx<T> lds[E::LDS_ELEMS];
+    __shared__ __align__(16) cpx<T> ltw[LTM > 0 ? E::tw_elems(LTM) : 1];
     const int tid = threadIdx.x;
     const int64_t s0 = int64_t(blockIdx.x) * TI;
     const int nvalid = (nsig - s0) < TI ? int(nsig - s0) : TI;
     const cpx<T>* ib = in + s0 * L;
     cpx<T>* ob = out + s0 * L;
     const T c2 = (T)(-0.5 * DIR);
+    if constexpr (LTM > 0) E::template stage_tw_to_lds<LTM>(ltw, tw.stage, tid);  // read after the first exchange
     if constexpr (DIR > 0) {
-        E::run_to_lds(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
+        E::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
             return (TI == 1 || t < nvalid) ? ib[t * L + i] : cpx<T>{(T)0, (T)0};
-        });
+        }, false, ltw);
         for (int q = tid; q < TI * NP; q += NT) {
             const int t = q / NP, p = q % NP;
             if (TI > 1 && t >= nvalid) continue;
@@ -129,9 +132,9 @@ __global__ __launch_bounds__(NT) void k_realft(const cpx<T>* in, cpx<T>* out, in
             }
         }
         lds_barrier();
-        E::run_from_lds(tid, lds, tw.stage, [&](int t, int k, cpx<T> v) {
+        E::template run_from_lds<LTM>(tid, lds, tw.stage, [&](int t, int k, cpx<T> v) {
             if (TI == 1 || t < nvalid) ob[t * L + k] = v;
-        });
+        }, ltw);
     }
 }
 

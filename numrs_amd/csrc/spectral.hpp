@@ -71,15 +71,18 @@ template <typename T, int L, int TI, int NT>
 __global__ __launch_bounds__(NT) void k_twofft(const T* d1, const T* d2, cpx<T>* f1, cpx<T>* f2, int64_t nsig,
                                                Twiddles<T> tw) {
     using E = Engine<T, 1, L, TI, NT, MAP_J, MAP_J>;
+    constexpr int LTM = engine_ltm<E>(E::LDS_ELEMS * E::CSIZE);
     __shared__ cpx<T> lds[E::LDS_ELEMS];
+    __shared__ __align__(16) cpx<T> ltw[LTM > 0 ? E::tw_elems(LTM) : 1];
     const int tid = threadIdx.x;
     const int64_t s0 = int64_t(blockIdx.x) * TI;
     const int nvalid = (nsig - s0) < TI ? int(nsig - s0) : TI;
     const T* a = d1 + s0 * L;
     const T* b = d2 + s0 * L;
-    E::run_to_lds(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
+    if constexpr (LTM > 0) E::template stage_tw_to_lds<LTM>(ltw, tw.stage, tid);  // read after the first exchange
+    E::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
         return (TI == 1 || t < nvalid) ? cpx<T>{a[t * L + i], b[t * L + i]} : cpx<T>{(T)0, (T)0};
-    });
+    }, false, ltw);
     constexpr int NP = L / 2 + 1;
     for (int q = tid; q < TI * NP; q += NT) {
         const int t = q / NP, k = q % NP;
@@ -104,16 +107,19 @@ __global__ __launch_bounds__(NT) void k_cosft1(const T* y, T* out, int64_t nsig,
     using E = Engine<T, 1, L, TI, NT, MAP_J, MAP_J>;
     constexpr int64_t S = L + 1;
     constexpr int LOG2N2 = ilog2c(L) + 1;  // log2 of the 2n-point real transform
+    constexpr int LTM = engine_ltm<E>(E::LDS_ELEMS * E::CSIZE);
     __shared__ cpx<T> lds[E::LDS_ELEMS];
+    __shared__ __align__(16) cpx<T> ltw[LTM > 0 ? E::tw_elems(LTM) : 1];
     const int tid = threadIdx.x;
     const int64_t s0 = int64_t(blockIdx.x) * TI;
     const int nvalid = (nsig - s0) < TI ? int(nsig - s0) : TI;
     const T* yb = y + s0 * S;
-    E::run_to_lds(tid, lds, tw.stage, [&](int t, int m) -> cpx<T> {
+    if constexpr (LTM > 0) E::template stage_tw_to_lds<LTM>(ltw, tw.stage, tid);  // read after the first exchange
+    E::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int m) -> cpx<T> {
         if (!(TI == 1 || t < nvalid)) return cpx<T>{(T)0, (T)0};
         const T* ys = yb + t * S;
         return cpx<T>{cf1_ext(ys, 2 * m, L), cf1_ext(ys, 2 * m + 1, L)};
-    });
+    }, false, ltw);
     constexpr int NP = L / 2 + 1;
     const T half = (T)0.5;
     for (int q = tid; q < TI * NP; q += NT) {
@@ -140,18 +146,21 @@ __global__ __launch_bounds__(NT) void k_cosft2(const T* y, T* out, int64_t nsig,
     using E = Engine<T, DIR, L, TI, NT, MAP_J, MAP_J>;
     constexpr int64_t N = 2 * L;
     constexpr int LOGN = ilog2c(L) + 1;
+    constexpr int LTM = engine_ltm<E>(E::LDS_ELEMS * E::CSIZE);
     __shared__ cpx<T> lds[E::LDS_ELEMS];
+    __shared__ __align__(16) cpx<T> ltw[LTM > 0 ? E::tw_elems(LTM) : 1];
     const int tid = threadIdx.x;
     const int64_t s0 = int64_t(blockIdx.x) * TI;
     const int nvalid = (nsig - s0) < TI ? int(nsig - s0) : TI;
     const T* yb = y + s0 * N;
     constexpr int NP = L / 2 + 1;
+    if constexpr (LTM > 0) E::template stage_tw_to_lds<LTM>(ltw, tw.stage, tid);  // read after the first exchange
     if constexpr (DIR > 0) {
-        E::run_to_lds(tid, lds, tw.stage, [&](int t, int m) -> cpx<T> {
+        E::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int m) -> cpx<T> {
             if (!(TI == 1 || t < nvalid)) return cpx<T>{(T)0, (T)0};
             const T* ys = yb + t * N;
             return cpx<T>{ys[cf2_pos(2 * m, L)], ys[cf2_pos(2 * m + 1, L)]};
-        });
+        }, false, ltw);
         for (int q = tid; q < TI * NP; q += NT) {
             const int t = q / NP, p = q % NP;
             if (TI > 1 && t >= nvalid) continue;
@@ -186,13 +195,13 @@ __global__ __launch_bounds__(NT) void k_cosft2(const T* y, T* out, int64_t nsig,
             }
         }
         lds_barrier();
-        E::run_from_lds(tid, lds, tw.stage, [&](int t, int m, cpx<T> v) {
+        E::template run_from_lds<LTM>(tid, lds, tw.stage, [&](int t, int m, cpx<T> v) {
             if (TI == 1 || t < nvalid) {
                 T* os = out + (s0 + t) * N;
                 os[cf2_pos(2 * m, L)] = v.x;
                 os[cf2_pos(2 * m + 1, L)] = v.y;
             }
-        });
+        }, ltw);
     }
 }
 
