@@ -69,9 +69,18 @@ struct QState {
     uint32_t next[2];  // task counters of the two phases
 };
 
+// Stages 1 .. NST-2 of a tile class read their twiddles from an LDS copy
+// (at most the 8192-point tables: 612 entries); a group refreshes it when
+// its next task has another transform length.
+template <int L>
+constexpr int q_ltm() {
+    return RadixPlan<L>::NST >= 3 ? RadixPlan<L>::NST - 2 : 0;
+}
+constexpr int Q_LTW_ELEMS = 612;
+
 template <typename T, int S, int C>
-__device__ __forceinline__ void q_run_class(const QTask& q, cpx<T>* const* bufs, cpx<T>* lds, const Twiddles<T>& tw,
-                                            int tid) {
+__device__ __forceinline__ void q_run_class(const QTask& q, cpx<T>* const* bufs, cpx<T>* lds, cpx<T>* ltw,
+                                            bool fresh_tw, const Twiddles<T>& tw, int tid) {
     constexpr int LOGL = C / Q_MODES + Q_MIN_LOGL;
     constexpr int MODE = C % Q_MODES;
     constexpr int L = 1 << LOGL;
@@ -85,6 +94,12 @@ __device__ __forceinline__ void q_run_class(const QTask& q, cpx<T>* const* bufs,
         using E = Engine<T, S, L, TI, NT, ML, MS>;
         static_assert(E::LDS_ELEMS <= q_lds_elems<T>(), "queue LDS too small for a tile class");
         static_assert(TI * L == qpts<T>() && E::EPT == 16, "queue tile shape");
+        constexpr int LTM = q_ltm<L>();
+        static_assert(LTM == 0 || E::tw_elems(LTM) <= Q_LTW_ELEMS, "queue twiddle copy too small");
+        // Published by the first exchange barrier; the loop-top barrier of
+        // k_queue ordered the previous task's reads before this overwrite.
+        if constexpr (LTM > 0)
+            if (fresh_tw) E::template stage_tw_to_lds<LTM>(ltw, tw.stage, tid);
         const cpx<T>* ib = bufs[q.src] + q.ioff;
         cpx<T>* ob = bufs[q.dst] + q.ooff;
         const int nvalid = q.nvalid, si0 = q.si0, esi = q.esi, so0 = q.so0, eso = q.eso;
@@ -94,7 +109,7 @@ __device__ __forceinline__ void q_run_class(const QTask& q, cpx<T>* const* bufs,
             return cpx<T>{(T)0, (T)0};
         });
         E::template compute<0>(v, tid, tw.stage);
-        E::template middle<1>(v, tid, lds, tw.stage, false);
+        E::template middle<1, LTM>(v, tid, lds, tw.stage, false, ltw);
         if (q.tw_logn) {
             const int logn = q.tw_logn;
             const uint32_t mask = (1u << logn) - 1u;
@@ -119,9 +134,10 @@ __device__ __forceinline__ void q_run_class(const QTask& q, cpx<T>* const* bufs,
 }
 
 template <typename T, int S, int... Cs>
-__device__ __forceinline__ void q_dispatch(int cls, const QTask& q, cpx<T>* const* bufs, cpx<T>* lds,
-                                           const Twiddles<T>& tw, int tid, std::integer_sequence<int, Cs...>) {
-    ((cls == Cs ? q_run_class<T, S, Cs>(q, bufs, lds, tw, tid) : void()), ...);
+__device__ __forceinline__ void q_dispatch(int cls, const QTask& q, cpx<T>* const* bufs, cpx<T>* lds, cpx<T>* ltw,
+                                           bool fresh_tw, const Twiddles<T>& tw, int tid,
+                                           std::integer_sequence<int, Cs...>) {
+    ((cls == Cs ? q_run_class<T, S, Cs>(q, bufs, lds, ltw, fresh_tw, tw, tid) : void()), ...);
 }
 
 // Reset of the queue counters (a fill kernel, so the sequence can live in a
@@ -136,9 +152,11 @@ __global__ __launch_bounds__(qpts<T>() / 16) void k_queue(cpx<T>* in, cpx<T>* ou
                                                             int base, int count, int phase, QState* st,
                                                             Twiddles<T> tw) {
     __shared__ cpx<T> lds[q_lds_elems<T>()];
+    __shared__ __align__(16) cpx<T> ltw[Q_LTW_ELEMS];
     __shared__ int sh_task;
     cpx<T>* bufs[3] = {in, out, ws};
     const int tid = threadIdx.x;
+    int tw_logl = -1;  // transform length whose twiddles ltw holds
     for (;;) {
         lds_barrier();  // previous task's LDS readers are done; sh_task free
         if (tid == 0) sh_task = (int)atomicAdd(&st->next[phase], 1u);
@@ -146,7 +164,10 @@ __global__ __launch_bounds__(qpts<T>() / 16) void k_queue(cpx<T>* in, cpx<T>* ou
         const int k = sh_task;
         if (k >= count) break;
         const QTask q = tasks[base + k];
-        q_dispatch<T, S>(q.cls, q, bufs, lds, tw, fresh_tid(tid), std::make_integer_sequence<int, q_nclasses<T>()>{});
+        const int logl = q.cls / Q_MODES + Q_MIN_LOGL;
+        q_dispatch<T, S>(q.cls, q, bufs, lds, ltw, logl != tw_logl, tw, fresh_tid(tid),
+                         std::make_integer_sequence<int, q_nclasses<T>()>{});
+        tw_logl = logl;
     }
 }
 
