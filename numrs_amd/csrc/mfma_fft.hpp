@@ -20,7 +20,8 @@
 // largest magnitude is < 1, and the 1/4 in F keeps a stage's outputs within
 // 4x of its inputs; the exact inverse scale is applied at the final store.
 // Twiddles stay FP32 (tables generated in FP64): the consuming stage reads its
-// four contiguous per-stage table entries, as the VALU engine does.
+// four contiguous per-stage table entries from the group's LDS copy of the
+// stage tables, as the VALU engine does.
 //
 // Operand lane layout of v_mfma_f32_16x16x32_{f16,bf16} (checked on MI355X):
 //   A: lane l holds A[l % 16][8 (l / 16) + e],  e = 0..7
@@ -109,15 +110,18 @@ __device__ __forceinline__ f4v split_product(const typename MfmaTraits<PREC>::V*
 }
 
 template <int PREC, int L, int NT>
-constexpr int kmfma_waves = waves_per_eu_target<(16 * L + 16 * L / NRFFT_LDS_PAD + 1) * 8, NT, false>();
+constexpr int kmfma_waves =
+    waves_per_eu_target<(16 * L + 16 * L / NRFFT_LDS_PAD + 1 + Engine<float, 1, L, 16, NT, MAP_T, MAP_T>::TW_ELEMS) * 8,
+                        NT, false>();
 
 // Radix-16 MFMA stages s .. NST-1 of one tile whose stage s - 1 output is in
 // LDS (a barrier has published it).  Groups of 16 vectors: 16 transforms
 // with the same butterfly (column layout) or 16 consecutive butterflies of one
 // transform (row layout).
 template <int PREC, int S, int L, int NT, int ML, int MS, int s>
-__device__ __forceinline__ void mfma_stages(cpx<float>* lds, unsigned lane, unsigned wave, int nvalid,
-                                            const Twiddles<float>& tw, const typename MfmaTraits<PREC>::V* a0,
+__device__ __forceinline__ void mfma_stages(cpx<float>* lds, const cpx<float>* ltw, unsigned lane, unsigned wave,
+                                            int nvalid, const Twiddles<float>& tw,
+                                            const typename MfmaTraits<PREC>::V* a0,
                                             const typename MfmaTraits<PREC>::V* a1, c32* ob, int so0, int eso,
                                             const PassDesc& d, int64_t b0base, float out_scale) {
     using E = Engine<float, S, L, 16, NT, ML, MS>;
@@ -140,14 +144,15 @@ __device__ __forceinline__ void mfma_stages(cpx<float>* lds, unsigned lane, unsi
             j = (gid % (NB / 16)) * 16 + n;
         }
     };
-    // inputs: points 4 kb + e of butterfly j, twiddled by row j mod NS of the stage table
-    const cpx<float>* stab = tw.stage + (2 * L + StageTable<L>::off(s)) + 4 * kb;
+    // inputs: points 4 kb + e of butterfly j, twiddled by row j mod NS of the
+    // stage table (its LDS copy, rows E::twrow(s) entries apart)
+    const cpx<float>* stab = ltw + E::twoff(s) + 4 * kb;
     cpx<float> x[GPW][4];
 #pragma unroll
     for (unsigned g = 0; g < GPW; ++g) {
         unsigned t, j;
         vec(g, t, j);
-        const cpx<float>* row = stab + (j & (NS - 1)) * 16;
+        const cpx<float>* row = stab + (j & (NS - 1)) * E::twrow(s);
         if constexpr (E::template affine<NB>) {
             const cpx<float>* src = lds + E::lidx(t, j + 4 * kb * NB);
 #pragma unroll
@@ -208,8 +213,8 @@ __device__ __forceinline__ void mfma_stages(cpx<float>* lds, unsigned lane, unsi
     }
     if constexpr (!LAST) {
         lds_barrier();
-        mfma_stages<PREC, S, L, NT, ML, MS, s + 1>(lds, lane, wave, nvalid, tw, a0, a1, ob, so0, eso, d, b0base,
-                                                    out_scale);
+        mfma_stages<PREC, S, L, NT, ML, MS, s + 1>(lds, ltw, lane, wave, nvalid, tw, a0, a1, ob, so0, eso, d,
+                                                    b0base, out_scale);
     }
 }
 
@@ -227,6 +232,7 @@ k_c2c_mfma(const c32* in, c32* out, PassDesc d, Twiddles<float> tw, const void* 
     static_assert(((L / 16) % W) == 0, "groups must split evenly over waves");
 
     __shared__ cpx<float> lds[E::LDS_ELEMS];
+    __shared__ __align__(16) cpx<float> ltw[E::TW_ELEMS];  // stage twiddles, stages 1 .. NST-1
     __shared__ float red[W];
 
     const int tid0 = threadIdx.x;
@@ -255,6 +261,7 @@ k_c2c_mfma(const c32* in, c32* out, PassDesc d, Twiddles<float> tw, const void* 
         }
     }
 
+    E::stage_tw_to_lds(ltw, tw.stage, tid0);  // published by the first stage-0 exchange barrier
     bool first = true;
     for (;;) {
         const int tid = fresh_tid(tid0);
@@ -302,8 +309,8 @@ k_c2c_mfma(const c32* in, c32* out, PassDesc d, Twiddles<float> tw, const void* 
         lds_barrier();
 
         // ---- radix-16 stages on the matrix cores ----------------------------
-        mfma_stages<PREC, S, L, NT, ML, MS, 1>(lds, (unsigned)lane, (unsigned)wave, nvalid, tw, ar, ai, ob, so0, eso,
-                                               d, b0base, inv_scale * (float)(1 << (2 * A)));
+        mfma_stages<PREC, S, L, NT, ML, MS, 1>(lds, ltw, (unsigned)lane, (unsigned)wave, nvalid, tw, ar, ai, ob, so0,
+                                               eso, d, b0base, inv_scale * (float)(1 << (2 * A)));
 
         const int64_t next = tile + Gd;
         if (next >= ntiles) break;
