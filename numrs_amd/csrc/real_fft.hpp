@@ -100,11 +100,22 @@ __global__ __launch_bounds__(NT) void k_realft(const cpx<T>* in, cpx<T>* out, in
     cpx<T>* ob = out + s0 * L;
     const T c2 = (T)(-0.5 * DIR);
     if constexpr (LTM > 0) E::template stage_tw_to_lds<LTM>(ltw, tw.stage, tid);  // read after the first exchange
+    // Untangle twiddle of pair p = (tid + u NT) mod NP: W^(tid mod NP) (two
+    // table loads, once) times W^((u NT) mod NP), the same for every thread
+    // (an LDS table read after the first barrier).
+    constexpr int PPT = (TI * NP + NT - 1) / NT;
+    __shared__ cpx<T> uw[PPT];
+    if (tid < PPT) uw[tid] = twiddle_big<DIR>(tw, uint32_t((int64_t(tid) * NT) % NP), LOG2L);
+    const cpx<T> wt = twiddle_big<DIR>(tw, uint32_t(tid % NP), LOG2L);
+    auto wpair = [&](int u) -> cpx<T> { return NP <= NT ? wt : cmul(wt, uw[u]); };
     if constexpr (DIR > 0) {
         E::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
             return (TI == 1 || t < nvalid) ? ib[t * L + i] : cpx<T>{(T)0, (T)0};
         }, false, ltw);
-        for (int q = tid; q < TI * NP; q += NT) {
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            const int q = tid + u * NT;
+            if ((TI * NP) % NT != 0 && q >= TI * NP) break;
             const int t = q / NP, p = q % NP;
             if (TI > 1 && t >= nvalid) continue;
             if (p == 0) {
@@ -112,13 +123,17 @@ __global__ __launch_bounds__(NT) void k_realft(const cpx<T>* in, cpx<T>* out, in
                 if (L > 1) ob[t * L + NP] = lds[E::lidx(t, NP)];
             } else {
                 cpx<T> a, b;
-                untangle_pair(lds[E::lidx(t, p)], lds[E::lidx(t, L - p)], twiddle_big<DIR>(tw, p, LOG2L), c2, a, b);
+                untangle_pair(lds[E::lidx(t, p)], lds[E::lidx(t, L - p)], wpair(u), c2, a, b);
                 ob[t * L + p] = a;
                 ob[t * L + L - p] = b;
             }
         }
     } else {
-        for (int q = tid; q < TI * NP; q += NT) {
+        lds_barrier();  // uw published
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+            const int q = tid + u * NT;
+            if ((TI * NP) % NT != 0 && q >= TI * NP) break;
             const int t = q / NP, p = q % NP;
             const bool ok = TI == 1 || t < nvalid;
             if (p == 0) {
@@ -126,7 +141,7 @@ __global__ __launch_bounds__(NT) void k_realft(const cpx<T>* in, cpx<T>* out, in
                 lds[E::lidx(t, NP)] = ok ? ib[t * L + NP] : cpx<T>{(T)0, (T)0};
             } else {
                 cpx<T> a{(T)0, (T)0}, b{(T)0, (T)0};
-                if (ok) untangle_pair(ib[t * L + p], ib[t * L + L - p], twiddle_big<DIR>(tw, p, LOG2L), c2, a, b);
+                if (ok) untangle_pair(ib[t * L + p], ib[t * L + L - p], wpair(u), c2, a, b);
                 lds[E::lidx(t, p)] = a;
                 lds[E::lidx(t, L - p)] = b;
             }
@@ -177,6 +192,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(conv_waves<T
     cpx<T>* ob = ans + s0 * L;
     const T inv_no2 = (T)1 / (T)L;
     EF::template stage_tw_to_lds<LTM>(ltw, tw.stage, tid);  // published by the first exchange barrier
+    // Untangle twiddles W_N^p of bin pair p = (tid + u NT) mod NP: the
+    // per-thread W^(tid mod NP) (two table loads, once) times the factor
+    // W^((u NT) mod NP) shared by every thread (an LDS table of PPT entries),
+    // instead of two table loads per pair after a barrier.
+    __shared__ cpx<T> uw[PPT];
+    if (tid < PPT) uw[tid] = twiddle_big<1>(tw, uint32_t((int64_t(tid) * NT) % NP), LOG2N);
+    const cpx<T> wt = twiddle_big<1>(tw, uint32_t(tid % NP), LOG2N);
+    auto wpair = [&](int u) -> cpx<T> { return NP <= NT ? wt : cmul(wt, uw[u]); };
 
     // 1. Spectrum of the signal, kept in registers pair by pair.
     EF::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
@@ -192,8 +215,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(conv_waves<T
             xa[u] = dc_forward(lds[EF::lidx(t, 0)]);
             xb[u] = lds[EF::lidx(t, NP)];
         } else {
-            untangle_pair(lds[EF::lidx(t, p)], lds[EF::lidx(t, L - p)], twiddle_big<1>(tw, p, LOG2N), (T)-0.5,
-                          xa[u], xb[u]);
+            untangle_pair(lds[EF::lidx(t, p)], lds[EF::lidx(t, L - p)], wpair(u), (T)-0.5, xa[u], xb[u]);
         }
     }
     lds_barrier();
@@ -226,7 +248,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(conv_waves<T
             xb[u] = spec_apply(op, xb[u], rh, inv_no2);
         } else {
             cpx<T> ra, rb2;
-            const cpx<T> w = twiddle_big<1>(tw, p, LOG2N);
+            const cpx<T> w = wpair(u);
             untangle_pair(lds[EF::lidx(t, p)], lds[EF::lidx(t, L - p)], w, (T)-0.5, ra, rb2);
             const cpx<T> ya = spec_apply(op, xa[u], ra, inv_no2);
             const cpx<T> yb = spec_apply(op, xb[u], rb2, inv_no2);
