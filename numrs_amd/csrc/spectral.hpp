@@ -57,6 +57,28 @@ __device__ __forceinline__ void cf2_fwd_out(const Twiddles<T>& tw, cpx<T> vk, in
     os[n - k] = u.y;
 }
 
+// Same with the rotation twiddle exp(i pi k / 2n) given.
+template <typename T>
+__device__ __forceinline__ void cf2_fwd_out_w(cpx<T> rot, cpx<T> vk, int64_t k, T* os, int64_t n) {
+    const cpx<T> u = cmul(rot, vk);
+    os[k] = u.x;
+    os[n - k] = u.y;
+}
+
+// Twiddles W_M^p of the bin pairs p = tid + u NT, u < PPT, of a group that
+// holds one signal: W_M^tid (per thread, loaded once) times W_M^(u NT)
+// (the same for every thread: a small LDS table written before the FFT's
+// first barrier) -- instead of two table loads per pair after a barrier.
+template <typename T, int PPT>
+struct PairTw {
+    cpx<T> wt;
+    __device__ __forceinline__ void init(cpx<T>* tab, const Twiddles<T>& tw, int tid, int nt, int logm) {
+        if (tid < PPT) tab[tid] = twiddle_big<1>(tw, uint32_t(tid * nt), logm);
+        wt = twiddle_big<1>(tw, uint32_t(tid), logm);
+    }
+    __device__ __forceinline__ cpx<T> at(const cpx<T>* tab, int u) const { return cmul(wt, tab[u]); }
+};
+
 // cosft2 inverse: V_k from Y_k, Y_{n-k}.
 template <typename T>
 __device__ __forceinline__ cpx<T> cf2_inv_in(const Twiddles<T>& tw, const T* ys, int64_t k, int logn, int64_t n) {
@@ -115,6 +137,10 @@ __global__ __launch_bounds__(NT) void k_cosft1(const T* y, T* out, int64_t nsig,
     const int nvalid = (nsig - s0) < TI ? int(nsig - s0) : TI;
     const T* yb = y + s0 * S;
     if constexpr (LTM > 0) E::template stage_tw_to_lds<LTM>(ltw, tw.stage, tid);  // read after the first exchange
+    constexpr int PPT1 = (L / 2 + 1 + NT - 1) / NT;
+    __shared__ cpx<T> uw[PPT1];
+    PairTw<T, PPT1> ptw;
+    if constexpr (TI == 1) ptw.init(uw, tw, tid, NT, LOG2N2);
     E::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int m) -> cpx<T> {
         if (!(TI == 1 || t < nvalid)) return cpx<T>{(T)0, (T)0};
         const T* ys = yb + t * S;
@@ -122,7 +148,8 @@ __global__ __launch_bounds__(NT) void k_cosft1(const T* y, T* out, int64_t nsig,
     }, false, ltw);
     constexpr int NP = L / 2 + 1;
     const T half = (T)0.5;
-    for (int q = tid; q < TI * NP; q += NT) {
+    int u = 0;
+    for (int q = tid; q < TI * NP; q += NT, ++u) {
         const int t = q / NP, p = q % NP;
         if (TI > 1 && t >= nvalid) continue;
         T* os = out + (s0 + t) * S;
@@ -132,8 +159,8 @@ __global__ __launch_bounds__(NT) void k_cosft1(const T* y, T* out, int64_t nsig,
             os[L] = half * (z0.x - z0.y);
         } else {
             cpx<T> fa, fb;
-            untangle_pair(lds[E::lidx(t, p)], lds[E::lidx(t, L - p)], twiddle_big<1>(tw, (uint32_t)p, LOG2N2),
-                          (T)-0.5, fa, fb);
+            const cpx<T> w = TI == 1 ? ptw.at(uw, u) : twiddle_big<1>(tw, (uint32_t)p, LOG2N2);
+            untangle_pair(lds[E::lidx(t, p)], lds[E::lidx(t, L - p)], w, (T)-0.5, fa, fb);
             os[p] = half * fa.x;
             if (L - p != p) os[L - p] = half * fb.x;
         }
@@ -155,13 +182,25 @@ __global__ __launch_bounds__(NT) void k_cosft2(const T* y, T* out, int64_t nsig,
     const T* yb = y + s0 * N;
     constexpr int NP = L / 2 + 1;
     if constexpr (LTM > 0) E::template stage_tw_to_lds<LTM>(ltw, tw.stage, tid);  // read after the first exchange
+    // One signal per group: untangle twiddles W_n^p and rotations W_4n^p,
+    // W_4n^(L-p) = W_8 conj(W_4n^p) from per-thread and shared factors.
+    constexpr int PPT2 = (NP + NT - 1) / NT;
+    __shared__ cpx<T> uwn[PPT2], uw4[PPT2];
+    PairTw<T, PPT2> pn, p4;
+    cpx<T> w8{(T)0, (T)0};
+    if constexpr (TI == 1) {
+        pn.init(uwn, tw, tid, NT, LOGN);
+        p4.init(uw4, tw, tid, NT, LOGN + 2);
+        w8 = twiddle_big<1>(tw, uint32_t(L), LOGN + 2);
+    }
     if constexpr (DIR > 0) {
         E::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int m) -> cpx<T> {
             if (!(TI == 1 || t < nvalid)) return cpx<T>{(T)0, (T)0};
             const T* ys = yb + t * N;
             return cpx<T>{ys[cf2_pos(2 * m, L)], ys[cf2_pos(2 * m + 1, L)]};
         }, false, ltw);
-        for (int q = tid; q < TI * NP; q += NT) {
+        int u = 0;
+        for (int q = tid; q < TI * NP; q += NT, ++u) {
             const int t = q / NP, p = q % NP;
             if (TI > 1 && t >= nvalid) continue;
             T* os = out + (s0 + t) * N;
@@ -169,6 +208,12 @@ __global__ __launch_bounds__(NT) void k_cosft2(const T* y, T* out, int64_t nsig,
                 const cpx<T> z0 = lds[E::lidx(t, 0)];
                 os[0] = z0.x + z0.y;
                 os[L] = (z0.x - z0.y) * cos_quarter_pi<T>();
+            } else if constexpr (TI == 1) {
+                cpx<T> va, vb;
+                untangle_pair(lds[E::lidx(t, p)], lds[E::lidx(t, L - p)], pn.at(uwn, u), (T)-0.5, va, vb);
+                const cpx<T> r4 = p4.at(uw4, u);
+                cf2_fwd_out_w(r4, va, p, os, N);
+                if (L - p != p) cf2_fwd_out_w(cmul(w8, conj(r4)), vb, L - p, os, N);
             } else {
                 cpx<T> va, vb;
                 untangle_pair(lds[E::lidx(t, p)], lds[E::lidx(t, L - p)], twiddle_big<1>(tw, (uint32_t)p, LOGN),
@@ -178,7 +223,9 @@ __global__ __launch_bounds__(NT) void k_cosft2(const T* y, T* out, int64_t nsig,
             }
         }
     } else {
-        for (int q = tid; q < TI * NP; q += NT) {
+        if constexpr (TI == 1) lds_barrier();  // uwn, uw4 published
+        int u = 0;
+        for (int q = tid; q < TI * NP; q += NT, ++u) {
             const int t = q / NP, p = q % NP;
             const bool ok = TI == 1 || t < nvalid;
             const T* ys = yb + t * N;
@@ -187,7 +234,13 @@ __global__ __launch_bounds__(NT) void k_cosft2(const T* y, T* out, int64_t nsig,
                 lds[E::lidx(t, 0)] = ok ? dc_inverse(cpx<T>{ys[0], sqrt2 * ys[L]}) : cpx<T>{(T)0, (T)0};
             } else {
                 cpx<T> za{(T)0, (T)0}, zb{(T)0, (T)0};
-                if (ok)
+                if constexpr (TI == 1) {
+                    // V_k = conj(W_4n^k) (Y_k + i Y_{n-k}) for k = p and L - p
+                    const cpx<T> r4 = p4.at(uw4, u);
+                    const cpx<T> va = cmul(conj(r4), cpx<T>{ys[p], ys[N - p]});
+                    const cpx<T> vb = cmul(conj(cmul(w8, conj(r4))), cpx<T>{ys[L - p], ys[N - (L - p)]});
+                    untangle_pair(va, vb, conj(pn.at(uwn, u)), (T)0.5, za, zb);
+                } else if (ok)
                     untangle_pair(cf2_inv_in(tw, ys, p, LOGN, N), cf2_inv_in(tw, ys, L - p, LOGN, N),
                                   twiddle_big<-1>(tw, (uint32_t)p, LOGN), (T)0.5, za, zb);
                 lds[E::lidx(t, p)] = za;
