@@ -285,34 +285,56 @@ __global__ void k_twofft_pack(const T* d1, const T* d2, cpx<T>* z, int64_t total
         z[q] = cpx<T>{d1[q], d2[q]};
 }
 
+// The multi-pass helpers below run on a 2-D grid: signals over blockIdx.y
+// (stride gridDim.y), the elements of one signal over the x grid (stride
+// gridDim.x * blockDim.x), so no index is divided by a runtime count.
+__device__ __forceinline__ int64_t x_first() { return blockIdx.x * int64_t(blockDim.x) + threadIdx.x; }
+__device__ __forceinline__ int64_t x_stride() { return int64_t(gridDim.x) * blockDim.x; }
+
+// W_M^p for the x index p of such a loop: W_M^threadIdx (per thread, once)
+// times W_M^(p - threadIdx), the same for the whole block (one cache line per
+// wave) -- a direct table lookup of consecutive p would step through the
+// fine table 2^(26 - log2 M) entries at a time, one cache line per lane.
+template <int S, typename T>
+struct RowTw {
+    cpx<T> wt;
+    int logm;
+    __device__ __forceinline__ RowTw(const Twiddles<T>& tw, int logm_) : logm(logm_) {
+        wt = twiddle_big<S>(tw, uint32_t(threadIdx.x), logm);
+    }
+    __device__ __forceinline__ cpx<T> at(const Twiddles<T>& tw, int64_t p) const {
+        return cmul(wt, twiddle_big<S>(tw, uint32_t(p - threadIdx.x), logm));
+    }
+};
+
 // In place on f1 (which holds Z), second spectrum to f2.
 template <typename T>
 __global__ void k_twofft_split(cpx<T>* f1, cpx<T>* f2, int64_t n, int64_t nsig) {
     const int64_t np = n / 2 + 1;
-    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < np * nsig;
-         q += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t s = q / np, k = q % np;
-        const int64_t r = (n - k) & (n - 1);
+    for (int64_t s = blockIdx.y; s < nsig; s += gridDim.y) {
         cpx<T>* o1 = f1 + s * n;
         cpx<T>* o2 = f2 + s * n;
-        cpx<T> g1, g2;
-        twofft_pair(o1[k], o1[r], g1, g2);
-        o1[k] = g1;
-        o2[k] = g2;
-        if (r != k) {
-            o1[r] = conj(g1);
-            o2[r] = conj(g2);
+        for (int64_t k = x_first(); k < np; k += x_stride()) {
+            const int64_t r = (n - k) & (n - 1);
+            cpx<T> g1, g2;
+            twofft_pair(o1[k], o1[r], g1, g2);
+            o1[k] = g1;
+            o2[k] = g2;
+            if (r != k) {
+                o1[r] = conj(g1);
+                o2[r] = conj(g2);
+            }
         }
     }
 }
 
 template <typename T>
 __global__ void k_cosft1_pack(const T* y, cpx<T>* z, int64_t L, int64_t nsig) {
-    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < L * nsig;
-         q += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t s = q / L, m = q % L;
+    for (int64_t s = blockIdx.y; s < nsig; s += gridDim.y) {
         const T* ys = y + s * (L + 1);
-        z[q] = cpx<T>{cf1_ext(ys, 2 * m, L), cf1_ext(ys, 2 * m + 1, L)};
+        cpx<T>* zs = z + s * L;
+        for (int64_t m = x_first(); m < L; m += x_stride())
+            zs[m] = cpx<T>{cf1_ext(ys, 2 * m, L), cf1_ext(ys, 2 * m + 1, L)};
     }
 }
 
@@ -320,49 +342,54 @@ template <typename T>
 __global__ void k_cosft1_post(const cpx<T>* z, T* out, int64_t L, int64_t nsig, int log2n2, Twiddles<T> tw) {
     const int64_t np = L / 2 + 1;
     const T half = (T)0.5;
-    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < np * nsig;
-         q += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t s = q / np, p = q % np;
+    const RowTw<1, T> wp(tw, log2n2);
+    for (int64_t s = blockIdx.y; s < nsig; s += gridDim.y) {
         const cpx<T>* zs = z + s * L;
         T* os = out + s * (L + 1);
-        if (p == 0) {
-            os[0] = half * (zs[0].x + zs[0].y);
-            os[L] = half * (zs[0].x - zs[0].y);
-        } else {
-            cpx<T> fa, fb;
-            untangle_pair(zs[p], zs[L - p], twiddle_big<1>(tw, (uint32_t)p, log2n2), (T)-0.5, fa, fb);
-            os[p] = half * fa.x;
-            if (L - p != p) os[L - p] = half * fb.x;
+        for (int64_t p = x_first(); p < np; p += x_stride()) {
+            if (p == 0) {
+                os[0] = half * (zs[0].x + zs[0].y);
+                os[L] = half * (zs[0].x - zs[0].y);
+            } else {
+                cpx<T> fa, fb;
+                untangle_pair(zs[p], zs[L - p], wp.at(tw, p), (T)-0.5, fa, fb);
+                os[p] = half * fa.x;
+                if (L - p != p) os[L - p] = half * fb.x;
+            }
         }
     }
 }
 
 template <typename T>
 __global__ void k_cosft2_fpack(const T* y, cpx<T>* z, int64_t L, int64_t nsig) {
-    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < L * nsig;
-         q += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t s = q / L, m = q % L;
+    for (int64_t s = blockIdx.y; s < nsig; s += gridDim.y) {
         const T* ys = y + s * 2 * L;
-        z[q] = cpx<T>{ys[cf2_pos(2 * m, L)], ys[cf2_pos(2 * m + 1, L)]};
+        cpx<T>* zs = z + s * L;
+        for (int64_t m = x_first(); m < L; m += x_stride())
+            zs[m] = cpx<T>{ys[cf2_pos(2 * m, L)], ys[cf2_pos(2 * m + 1, L)]};
     }
 }
 
 template <typename T>
 __global__ void k_cosft2_fpost(const cpx<T>* z, T* out, int64_t L, int64_t nsig, int logn, Twiddles<T> tw) {
     const int64_t np = L / 2 + 1, n = 2 * L;
-    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < np * nsig;
-         q += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t s = q / np, p = q % np;
+    // exp(i pi (L - p) / 2n) = W_8 conj(exp(i pi p / 2n))
+    const cpx<T> w8 = twiddle_big<1>(tw, (uint32_t)L, logn + 2);
+    const RowTw<1, T> wn(tw, logn), w4(tw, logn + 2);
+    for (int64_t s = blockIdx.y; s < nsig; s += gridDim.y) {
         const cpx<T>* zs = z + s * L;
         T* os = out + s * n;
-        if (p == 0) {
-            os[0] = zs[0].x + zs[0].y;
-            os[L] = (zs[0].x - zs[0].y) * cos_quarter_pi<T>();
-        } else {
-            cpx<T> va, vb;
-            untangle_pair(zs[p], zs[L - p], twiddle_big<1>(tw, (uint32_t)p, logn), (T)-0.5, va, vb);
-            cf2_fwd_out(tw, va, p, logn, os, n);
-            if (L - p != p) cf2_fwd_out(tw, vb, L - p, logn, os, n);
+        for (int64_t p = x_first(); p < np; p += x_stride()) {
+            if (p == 0) {
+                os[0] = zs[0].x + zs[0].y;
+                os[L] = (zs[0].x - zs[0].y) * cos_quarter_pi<T>();
+            } else {
+                cpx<T> va, vb;
+                untangle_pair(zs[p], zs[L - p], wn.at(tw, p), (T)-0.5, va, vb);
+                const cpx<T> r4 = w4.at(tw, p);
+                cf2_fwd_out_w(r4, va, p, os, n);
+                if (L - p != p) cf2_fwd_out_w(cmul(w8, conj(r4)), vb, L - p, os, n);
+            }
         }
     }
 }
@@ -370,31 +397,37 @@ __global__ void k_cosft2_fpost(const cpx<T>* z, T* out, int64_t L, int64_t nsig,
 template <typename T>
 __global__ void k_cosft2_ipre(const T* y, cpx<T>* z, int64_t L, int64_t nsig, int logn, Twiddles<T> tw) {
     const int64_t np = L / 2 + 1, n = 2 * L;
-    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < np * nsig;
-         q += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t s = q / np, p = q % np;
+    const cpx<T> w8 = twiddle_big<1>(tw, (uint32_t)L, logn + 2);
+    const RowTw<1, T> wn(tw, logn), w4(tw, logn + 2);
+    for (int64_t s = blockIdx.y; s < nsig; s += gridDim.y) {
         const T* ys = y + s * n;
         cpx<T>* zs = z + s * L;
-        if (p == 0) {
-            zs[0] = dc_inverse(cpx<T>{ys[0], (T)1.41421356237309504880 * ys[L]});
-        } else {
-            cpx<T> za, zb;
-            untangle_pair(cf2_inv_in(tw, ys, p, logn, n), cf2_inv_in(tw, ys, L - p, logn, n),
-                          twiddle_big<-1>(tw, (uint32_t)p, logn), (T)0.5, za, zb);
-            zs[p] = za;
-            zs[L - p] = zb;
+        for (int64_t p = x_first(); p < np; p += x_stride()) {
+            if (p == 0) {
+                zs[0] = dc_inverse(cpx<T>{ys[0], (T)1.41421356237309504880 * ys[L]});
+            } else {
+                // V_k = exp(-i pi k / 2n) (Y_k + i Y_{n-k}) for k = p and L - p
+                const cpx<T> r4 = w4.at(tw, p);
+                const cpx<T> va = cmul(conj(r4), cpx<T>{ys[p], ys[n - p]});
+                const cpx<T> vb = cmul(conj(cmul(w8, conj(r4))), cpx<T>{ys[L - p], ys[n - (L - p)]});
+                cpx<T> za, zb;
+                untangle_pair(va, vb, conj(wn.at(tw, p)), (T)0.5, za, zb);
+                zs[p] = za;
+                zs[L - p] = zb;
+            }
         }
     }
 }
 
 template <typename T>
 __global__ void k_cosft2_iunpack(const cpx<T>* z, T* out, int64_t L, int64_t nsig) {
-    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < L * nsig;
-         q += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t s = q / L, m = q % L;
+    for (int64_t s = blockIdx.y; s < nsig; s += gridDim.y) {
+        const cpx<T>* zs = z + s * L;
         T* os = out + s * 2 * L;
-        os[cf2_pos(2 * m, L)] = z[q].x;
-        os[cf2_pos(2 * m + 1, L)] = z[q].y;
+        for (int64_t m = x_first(); m < L; m += x_stride()) {
+            os[cf2_pos(2 * m, L)] = zs[m].x;
+            os[cf2_pos(2 * m + 1, L)] = zs[m].y;
+        }
     }
 }
 
@@ -406,24 +439,37 @@ __global__ void k_rlft3_copycol(const cpx<T>* data, cpx<T>* speq, int64_t plane,
         speq[q] = data[q * h];
 }
 
-// rlft3 untangle, one thread per (s, i1, i2, k3), k3 < max(1, h/2).
+// rlft3 untangle, one thread per (s, i1, i2, k3), k3 < max(1, h/2); every
+// extent is a power of two, so the index splits into bit fields.
 template <typename T, int DIR>
 __global__ void k_rlft3_untangle(cpx<T>* data, cpx<T>* speq, int64_t nn1, int64_t nn2, int64_t h, int64_t nsig,
                                  int lognn3, Twiddles<T> tw) {
     const int64_t nk = h > 1 ? h / 2 : 1;
     const int64_t plane = nn1 * nn2;
     const int64_t total = plane * nk * nsig;
+    const int lk = __builtin_ctzll((unsigned long long)nk), l2 = __builtin_ctzll((unsigned long long)nn2),
+              l1 = __builtin_ctzll((unsigned long long)nn1);
     const T c2 = (T)(-0.5 * DIR);
+    // k3 = q mod nk with q = (block base) + threadIdx.x: when nk <= blockDim.x
+    // (both powers of two) k3 = threadIdx.x mod nk for every q of the thread;
+    // otherwise W^k3 = W^threadIdx.x W^(k3 - threadIdx.x), the second factor
+    // the same for the whole block.
+    const cpx<T> wt = twiddle_big<DIR>(tw, uint32_t(nk <= (int64_t)blockDim.x ? (threadIdx.x & (nk - 1)) : threadIdx.x),
+                                       lognn3);
+    auto k3tw = [&](int64_t k3) -> cpx<T> {
+        if (nk <= (int64_t)blockDim.x) return wt;
+        return cmul(wt, twiddle_big<DIR>(tw, uint32_t(k3 - threadIdx.x), lognn3));
+    };
     for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < total; q += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t k3 = q % nk;
-        const int64_t r = q / nk;
-        const int64_t i2 = r % nn2, i1 = (r / nn2) % nn1, s = r / plane;
+        const int64_t k3 = q & (nk - 1);
+        const int64_t r = q >> lk;
+        const int64_t i2 = r & (nn2 - 1), i1 = (r >> l2) & (nn1 - 1), s = r >> (l1 + l2);
         const int64_t j1 = (nn1 - i1) & (nn1 - 1), j2 = (nn2 - i2) & (nn2 - 1);
         cpx<T>* ds = data + s * plane * h;
         cpx<T>* ss = speq + s * plane;
         cpx<T>& a = ds[(i1 * nn2 + i2) * h + k3];
         cpx<T>& b = k3 == 0 ? ss[j1 * nn2 + j2] : ds[(j1 * nn2 + j2) * h + (h - k3)];
-        const cpx<T> w = k3 == 0 ? cpx<T>{(T)1, (T)0} : twiddle_big<DIR>(tw, (uint32_t)k3, lognn3);
+        const cpx<T> w = k3 == 0 ? cpx<T>{(T)1, (T)0} : k3tw(k3);
         cpx<T> oa, ob;
         untangle_pair(a, b, w, c2, oa, ob);
         a = oa;

@@ -169,6 +169,15 @@ static unsigned grid_of(int64_t work, int nt) {
     return (unsigned)g;
 }
 
+// 2-D grid of the multi-pass helpers: y over signals, x over the `per`
+// elements of one signal, 256 threads a block, at most ~64K blocks.
+static dim3 grid2(int64_t per, int64_t nsig) {
+    const int64_t gy = std::max<int64_t>(1, std::min<int64_t>(nsig, 65535));
+    const int64_t cap = std::max<int64_t>(1, 65536 / gy);
+    const int64_t gx = std::max<int64_t>(1, std::min<int64_t>((per + 255) / 256, cap));
+    return dim3((unsigned)gx, (unsigned)gy);
+}
+
 template <typename T>
 static int exec_spec_t(const Plan& p, const Launch& l, void* const* bufs, int isign, hipStream_t s, DeviceCtx* ctx) {
     Twiddles<T> tw = ctx_twiddles<T>(ctx);
@@ -191,7 +200,7 @@ static int exec_spec_t(const Plan& p, const Launch& l, void* const* bufs, int is
                                (const T*)bufs[B_AUX], (cpx<T>*)dst, nsig * a[1]);
             break;
         case SP_TW_SPLIT:
-            hipLaunchKernelGGL((k_twofft_split<T>), dim3(grid_of(nsig * (a[1] / 2 + 1), 256)), dim3(256), 0, s,
+            hipLaunchKernelGGL((k_twofft_split<T>), grid2(a[1] / 2 + 1, nsig), dim3(256), 0, s,
                                (cpx<T>*)src, (cpx<T>*)dst, a[1], nsig);
             break;
         case SP_CF1_FUSED: {
@@ -202,11 +211,11 @@ static int exec_spec_t(const Plan& p, const Launch& l, void* const* bufs, int is
             return NRFFT_OK;
         }
         case SP_CF1_PACK:
-            hipLaunchKernelGGL((k_cosft1_pack<T>), dim3(grid_of(nsig * a[1], 256)), dim3(256), 0, s, (const T*)src,
+            hipLaunchKernelGGL((k_cosft1_pack<T>), grid2(a[1], nsig), dim3(256), 0, s, (const T*)src,
                                (cpx<T>*)dst, a[1], nsig);
             break;
         case SP_CF1_POST:
-            hipLaunchKernelGGL((k_cosft1_post<T>), dim3(grid_of(nsig * (a[1] / 2 + 1), 256)), dim3(256), 0, s,
+            hipLaunchKernelGGL((k_cosft1_post<T>), grid2(a[1] / 2 + 1, nsig), dim3(256), 0, s,
                                (const cpx<T>*)src, (T*)dst, a[1], nsig, ilog2((size_t)a[1]) + 1, tw);
             break;
         case SP_CF2_FUSED: {
@@ -226,19 +235,19 @@ static int exec_spec_t(const Plan& p, const Launch& l, void* const* bufs, int is
                                    (T*)dst, nsig);
             break;
         case SP_CF2_FPACK:
-            hipLaunchKernelGGL((k_cosft2_fpack<T>), dim3(grid_of(nsig * a[1], 256)), dim3(256), 0, s, (const T*)src,
+            hipLaunchKernelGGL((k_cosft2_fpack<T>), grid2(a[1], nsig), dim3(256), 0, s, (const T*)src,
                                (cpx<T>*)dst, a[1], nsig);
             break;
         case SP_CF2_FPOST:
-            hipLaunchKernelGGL((k_cosft2_fpost<T>), dim3(grid_of(nsig * (a[1] / 2 + 1), 256)), dim3(256), 0, s,
+            hipLaunchKernelGGL((k_cosft2_fpost<T>), grid2(a[1] / 2 + 1, nsig), dim3(256), 0, s,
                                (const cpx<T>*)src, (T*)dst, a[1], nsig, ilog2((size_t)a[1]) + 1, tw);
             break;
         case SP_CF2_IPRE:
-            hipLaunchKernelGGL((k_cosft2_ipre<T>), dim3(grid_of(nsig * (a[1] / 2 + 1), 256)), dim3(256), 0, s,
+            hipLaunchKernelGGL((k_cosft2_ipre<T>), grid2(a[1] / 2 + 1, nsig), dim3(256), 0, s,
                                (const T*)src, (cpx<T>*)dst, a[1], nsig, ilog2((size_t)a[1]) + 1, tw);
             break;
         case SP_CF2_IUNPACK:
-            hipLaunchKernelGGL((k_cosft2_iunpack<T>), dim3(grid_of(nsig * a[1], 256)), dim3(256), 0, s,
+            hipLaunchKernelGGL((k_cosft2_iunpack<T>), grid2(a[1], nsig), dim3(256), 0, s,
                                (const cpx<T>*)src, (T*)dst, a[1], nsig);
             break;
         case SP_R3_COPY:
