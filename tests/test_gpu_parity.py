@@ -233,6 +233,38 @@ def test_convlv_batch_per_signal_f32():
         assert rel_l2(y[i], ref) < 1e-5
 
 
+@pytest.mark.parametrize("n,m", [(64, 7), (1024, 257), (2048, 101), (8192, 257), (16384, 257)])
+@pytest.mark.parametrize("isign", [1, -1])
+def test_convlv_batch_f32_lengths(n, m, isign):
+    # FP32 one-launch convlv / deconvolution across tile shapes (TI > 1 up
+    # to one signal per group; L = 512 and 8192 are the 4-waves-per-SIMD
+    # builds), per-signal responses, against the FP64 oracle on the
+    # FP32-rounded inputs.
+    b = 6
+    d = f32round(synth.normal(n + 3, n * b)).reshape(b, n)
+    if isign == 1:
+        x = np.arange(m) - (m - 1) / 2
+        r = np.stack([f32round(np.exp(-0.5 * (x / max(m / 6, 0.5)) ** 2) * (1 + 0.1 * synth.normal(40 + i, m)))
+                      for i in range(b)])
+    else:
+        # deconvolution needs a response whose spectrum stays away from 0:
+        # a unit centre tap plus taps summing to |.| <= 0.3, so |R| >= 0.7
+        r = np.stack([f32round(0.3 / m * synth.normal(40 + i, m)) for i in range(b)])
+        r[:, (m - 1) // 2] = 1.0
+    y = numrs_amd.convlv_batch_responses(d.astype(np.float32), r.astype(np.float32), isign)
+    for i in range(b):
+        ref = oracle.convlv(d[i], r[i], isign)
+        assert rel_l2(y[i], ref) < 1e-5, (n, m, isign, i)
+
+
+@pytest.mark.parametrize("n", [1024, 16384])
+def test_correl_f32(n):
+    d1 = f32round(synth.normal(n + 11, n))
+    d2 = f32round(synth.normal(n + 12, n))
+    y = numrs_amd.correl(d1.astype(np.float32), d2.astype(np.float32))
+    assert rel_l2(y, oracle.correl(d1, d2)) < 1e-5, n
+
+
 def test_convlv_batch_shared_response():
     data = [synth.normal(5 + i, 256) for i in range(4)]
     resp = [0.5, 1.0, 0.5]
