@@ -452,6 +452,7 @@ int dev_alloc(DevBuf& b, size_t bytes, hipStream_t s) {
 
 // Host-pointer C2C (four1 / fourn / batch): in place on host data.
 int host_c2c(int dtype, void* data, const std::vector<size_t>& nn, size_t batch, int isign) {
+    if (const size_t chunk = host_pipe_chunk(dtype, nn, batch)) return host_c2c_pipelined(dtype, data, nn, batch, chunk, isign);
     std::shared_ptr<Plan> p;
     int rc = cached_plan(PK_C2C, dtype, nn, batch, 0, &p);
     if (rc) return rc;

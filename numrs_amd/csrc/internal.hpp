@@ -145,6 +145,8 @@ struct DevBuf {
     }
 };
 int dev_alloc(DevBuf& b, size_t bytes, hipStream_t s);
+int host_c2c_pipelined(int dtype, void* data, const std::vector<size_t>& nn, size_t batch, size_t chunk, int isign);
+size_t host_pipe_chunk(int dtype, const std::vector<size_t>& nn, size_t batch);
 
 }  // namespace nrfft
 

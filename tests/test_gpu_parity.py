@@ -106,6 +106,30 @@ def test_four1_batch_f32_2pow20():
         assert rel_l2(bufs[i], ref) < TOL32
 
 
+@pytest.mark.parametrize("dtype,logn,batch", [(np.float32, 16, 256), (np.float64, 15, 200)])
+def test_four1_batch_host_pipelined(dtype, logn, batch):
+    # nrfft_four1_batch_* on host arrays large enough to be streamed through
+    # HBM in 32 MB chunks (4 chunks; the FP64 case ends with a short one):
+    # transforms on both sides of every chunk boundary against the oracle.
+    import ctypes
+
+    from numrs_amd import _lib
+
+    n = 1 << logn
+    x = synth.uniform(500 + logn, 2 * n * batch)
+    if dtype == np.float32:
+        x = f32round(x)
+    y = x.astype(dtype)
+    ptr = y.ctypes.data_as(ctypes.POINTER(ctypes.c_float if dtype == np.float32 else ctypes.c_double))
+    fn = _lib.lib().nrfft_four1_batch_f32 if dtype == np.float32 else _lib.lib().nrfft_four1_batch_f64
+    assert fn(ptr, n, batch, -1) == 0, _lib.last_error()
+    chunk = (32 << 20) // (n * 2 * np.dtype(dtype).itemsize)
+    for i in sorted({0, chunk - 1, chunk, 2 * chunk + 1, batch - 2, batch - 1}):
+        ref = oracle.four1_nr(x[i * 2 * n:(i + 1) * 2 * n], -1)
+        got = y[i * 2 * n:(i + 1) * 2 * n].astype(np.float64)
+        assert rel_l2(got, ref) < (TOL32 if dtype == np.float32 else TOL64), (i, chunk)
+
+
 # ---- realft ------------------------------------------------------------------
 @pytest.mark.parametrize("logn", list(range(1, 16)) + [17, 19])
 def test_realft_f64(logn):
