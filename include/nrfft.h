@@ -169,8 +169,10 @@ int nrfft_rlft3_f32(float* data, float* speq, size_t nn1, size_t nn2, size_t nn3
 /* Four_FS.rs:55 FourFS::fourfs(nn, ndim, isign) -- the out-of-core n-D FFT:
  * in place on a host array of prod(nn) interleaved complex values (C order;
  * an mmap-ed file works), streamed through HBM with at most device_bytes of
- * device memory (0: 1 GiB).  Results equal fourn's.  Strided axes up to 8192
- * points; a 1-D transform larger than half the budget runs a four-step over
+ * device memory (0: 1 GiB).  Results equal fourn's.  An array that fits the
+ * budget makes one round trip; a larger one streams axis by axis in chunks
+ * through pinned host staging.  Strided axes up to 8192 points; a 1-D
+ * transform larger than half the budget runs a four-step over
  * the host array and needs `scratch` (same size as data), else scratch may
  * be NULL.  Errors follow Four_FS.rs:328-339 (NRFFT_E_DIMS "Invalid
  * dimensions" / "Invalid dimension size", NRFFT_E_ISIGN). */

@@ -4,6 +4,7 @@
 
 #include <memory>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "pass_c2c.hpp"
@@ -145,6 +146,25 @@ struct DevBuf {
     }
 };
 int dev_alloc(DevBuf& b, size_t bytes, hipStream_t s);
+// A batch of chunks streamed host -> HBM -> host through pinned staging:
+// gather(i, pinned) fills the input slot and returns its bytes; compute(i,
+// dev, ws, stream, &out, &out_bytes) transforms the uploaded chunk in the
+// device slot `dev` (slot_bytes) with the shared workspace `ws` (ws_bytes)
+// and names the device bytes to download; scatter(i, pinned, bytes) stores
+// them.  Two slots overlap transfers, transforms and host copies; one slot
+// runs the chunks in turn.  Chunks must cover disjoint host ranges.
+struct PipeJob {
+    size_t chunks = 0;
+    size_t slot_bytes = 0;
+    size_t ws_bytes = 0;
+    int slots = 2;
+    std::function<size_t(size_t, char*)> gather;
+    std::function<int(size_t, char*, void*, hipStream_t, const char**, size_t*)> compute;
+    std::function<void(size_t, const char*, size_t)> scatter;
+};
+int run_pipe(const PipeJob& job);
+// dst[r * dpitch ..+ width] = src[r * spitch ..+ width] for r < rows, on the host copy threads.
+void par_copy2d(char* dst, size_t dpitch, const char* src, size_t spitch, size_t width, size_t rows);
 int host_c2c_pipelined(int dtype, void* data, const std::vector<size_t>& nn, size_t batch, size_t chunk, int isign);
 size_t host_pipe_chunk(int dtype, const std::vector<size_t>& nn, size_t batch);
 

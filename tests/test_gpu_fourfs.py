@@ -14,9 +14,12 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("shape,budget", [
     ((32, 64, 128), 128 << 10),     # column bands on both strided axes
-    ((32, 64, 128), 64 << 20),      # whole slabs batched
+    ((32, 64, 128), 2 << 20),       # whole slabs batched (axis 1), bands (axis 0)
+    ((32, 64, 128), 64 << 20),      # fits: one upload, the in-memory fourn plan
     ((8, 16384), 1 << 20),          # last axis four-step rows in chunks, short strided axis
-    ((4096, 4), 256 << 10),         # long strided axis in bands of 2 columns
+    ((4096, 4), 128 << 10),         # long strided axis in bands of 1 column
+    ((2048, 16), 256 << 10),        # long strided axis in bands of 2 columns
+    ((4, 65536), 2 << 20),          # a row and its workspace fill the budget: one slot
 ])
 @pytest.mark.parametrize("isign", [1, -1])
 def test_fourfs_nd(shape, budget, isign):
@@ -32,6 +35,14 @@ def test_fourfs_1d(n, budget):
     g = x.copy()
     numrs_amd.fourfs(g, [n], 1, 1, device_bytes=budget)
     assert rel_l2(g, oracle.four1_nr(x, 1)) < 1e-12, (n, budget)
+
+
+def test_fourfs_1d_f32():
+    n = 1 << 18
+    x = synth.uniform(95, 2 * n).astype(np.float32)
+    g = x.copy()
+    numrs_amd.fourfs(g, [n], 1, -1, device_bytes=512 << 10)
+    assert rel_l2(g, oracle.four1_nr(x.astype(np.float64), -1)) < 2e-6
 
 
 def test_fourfs_f32():
