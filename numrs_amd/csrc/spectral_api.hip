@@ -61,6 +61,18 @@ static void tag_launches(Plan& p, size_t from, int force, int only) {
     }
 }
 
+// The inner C2C of a multi-pass plan runs with the tuning of a C2C plan of
+// its length and batch (four-step split, tile widths, batch chunking).
+static void use_c2c_tune(Plan& p, int logn) {
+    Plan q;
+    q.kind = PK_C2C;
+    q.dtype = p.dtype;
+    q.nn = {size_t(1) << logn};
+    q.batch = p.batch;
+    tune_lookup(q);
+    p.tune = q.tune;
+}
+
 int build_plan_spectral(Plan& p) {
     const int64_t batch = (int64_t)p.batch;
     size_t tmp = 0;
@@ -74,6 +86,7 @@ int build_plan_spectral(Plan& p) {
                 return NRFFT_OK;
             }
             p.launches.push_back(spec_launch(SP_TW_PACK, B_IN, B_OUT, {batch, n}));
+            use_c2c_tune(p, logn);
             const size_t from = p.launches.size();
             if ((rc = plan_c2c_1d(p, logn, batch, B_OUT, B_OUT, &tmp))) return rc;
             tag_launches(p, from, 1, 0);
@@ -89,6 +102,7 @@ int build_plan_spectral(Plan& p) {
                 return NRFFT_OK;
             }
             p.launches.push_back(spec_launch(SP_CF1_PACK, B_IN, B_TMP2, {batch, n}));
+            use_c2c_tune(p, logn);
             const size_t from = p.launches.size();
             if ((rc = plan_c2c_1d(p, logn, batch, B_TMP2, B_TMP2, &tmp))) return rc;
             tag_launches(p, from, 1, 0);
@@ -108,6 +122,7 @@ int build_plan_spectral(Plan& p) {
                 p.launches.push_back(spec_launch(SP_CF2_FUSED, B_IN, B_OUT, {batch, L}));
                 return NRFFT_OK;
             }
+            use_c2c_tune(p, logl);
             // forward: permute-pack -> C2C(+1) -> untangle and rotate
             Launch a = spec_launch(SP_CF2_FPACK, B_IN, B_TMP2, {batch, L});
             a.args[7] = 1;

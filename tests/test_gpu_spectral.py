@@ -218,3 +218,45 @@ def test_device_plans_batched():
         assert rel_l2(gd[i * vol:(i + 1) * vol], rd) < 1e-12 and rel_l2(gs[i * ns:(i + 1) * ns], rs) < 1e-12
     plan.execute(t, sp, -1)
     assert rel_l2(t.cpu().numpy(), 0.5 * vol * x) < 1e-12
+
+
+def test_device_plans_multipass_chunked_f32():
+    # FP32 at n = 2^20, 36 transforms: the inner C2C runs the tuned four-step
+    # in chunks of 16 transforms (16, 16, 4).  Transforms of the first, a
+    # middle and the tail chunk against FP64 numpy / scipy.
+    import scipy.fft as sfft
+    import torch
+
+    n, b, pick = 1 << 20, 36, (0, 17, 35)
+    tol = 2e-6
+    d1, d2 = _in(80, b * n, np.float32), _in(81, b * n, np.float32)
+    f1 = torch.empty(2 * b * n, dtype=torch.float32, device="cuda")
+    f2 = torch.empty_like(f1)
+    numrs_amd.TwofftPlan("float32", n, b).execute(torch.from_numpy(d1).cuda(), torch.from_numpy(d2).cuda(), f1, f2)
+    g1, g2 = f1.cpu().numpy(), f2.cpu().numpy()
+    for i in pick:
+        for g, d in ((g1, d1), (g2, d2)):
+            z = n * np.fft.ifft(d[i * n:(i + 1) * n].astype(np.float64))   # sum x_j exp(+2 pi i jk / n)
+            ref = np.stack([z.real, z.imag], axis=1).ravel()
+            assert rel_l2(g[2 * i * n:2 * (i + 1) * n], ref) < tol, ("twofft", i)
+    del f1, f2
+
+    y = _in(82, b * (n + 1), np.float32)
+    t = torch.from_numpy(y).cuda()
+    numrs_amd.Cosft1Plan("float32", n, b).execute(t, t)
+    got = t.cpu().numpy()
+    for i in pick:
+        seg = slice(i * (n + 1), (i + 1) * (n + 1))
+        assert rel_l2(got[seg], 0.5 * sfft.dct(y[seg].astype(np.float64), type=1)) < tol, ("cosft1", i)
+
+    y = _in(83, b * n, np.float32)
+    plan = numrs_amd.Cosft2Plan("float32", n, b)
+    for isign in (1, -1):
+        t = torch.from_numpy(y).cuda()
+        o = torch.empty_like(t)
+        plan.execute(t, o, isign)
+        got = o.cpu().numpy()
+        for i in pick:
+            seg = slice(i * n, (i + 1) * n)
+            exact = 0.5 * sfft.dct(y[seg].astype(np.float64), type=2 if isign == 1 else 3)
+            assert rel_l2(got[seg], exact) < tol, ("cosft2", isign, i)
