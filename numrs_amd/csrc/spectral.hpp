@@ -338,95 +338,215 @@ __global__ void k_cosft1_pack(const T* y, cpx<T>* z, int64_t L, int64_t nsig) {
     }
 }
 
+// Bin pairs per thread in the multi-pass untangle kernels: a block takes
+// PAIR_U * blockDim.x consecutive pairs p = base + threadIdx.x + u blockDim.x,
+// issues every load of its pairs (data and block-uniform twiddle factors)
+// before the first store, then finishes them -- one pair per thread left too
+// few loads in flight to cover HBM latency.
+// Indices inside one signal are 32-bit (n <= 2^26): 64-bit index arithmetic
+// on eight address streams spilled registers.  Blocks are 256 threads.
+constexpr int PAIR_U = 4;
+constexpr int PAIR_NT = 256;
+__device__ __forceinline__ int pair_first() { return int(blockIdx.x) * PAIR_NT * PAIR_U; }
+__device__ __forceinline__ int pair_stride() { return int(gridDim.x) * PAIR_NT * PAIR_U; }
+__device__ __forceinline__ int pair_at(int p0, int u) { return p0 + int(threadIdx.x) + u * PAIR_NT; }
+
+// The pairs p < L/2 of a signal; the thread of p = 0 also finishes the
+// self-paired bin p = L/2, so the spans of a signal divide evenly among blocks.
 template <typename T>
-__global__ void k_cosft1_post(const cpx<T>* z, T* out, int64_t L, int64_t nsig, int log2n2, Twiddles<T> tw) {
-    const int64_t np = L / 2 + 1;
+__global__ __launch_bounds__(PAIR_NT) void k_cosft1_post(const cpx<T>* z, T* out, int64_t L64, int64_t nsig,
+                                                         int log2n2, Twiddles<T> tw) {
+    const int L = int(L64), nq = L / 2;
     const T half = (T)0.5;
     const RowTw<1, T> wp(tw, log2n2);
     for (int64_t s = blockIdx.y; s < nsig; s += gridDim.y) {
-        const cpx<T>* zs = z + s * L;
-        T* os = out + s * (L + 1);
-        for (int64_t p = x_first(); p < np; p += x_stride()) {
-            if (p == 0) {
-                os[0] = half * (zs[0].x + zs[0].y);
-                os[L] = half * (zs[0].x - zs[0].y);
-            } else {
-                cpx<T> fa, fb;
-                untangle_pair(zs[p], zs[L - p], wp.at(tw, p), (T)-0.5, fa, fb);
-                os[p] = half * fa.x;
-                if (L - p != p) os[L - p] = half * fb.x;
+        const cpx<T>* zs = z + s * L64;
+        T* os = out + s * (L64 + 1);
+        for (int p0 = pair_first(); p0 < nq; p0 += pair_stride()) {
+            cpx<T> a[PAIR_U], b[PAIR_U], w[PAIR_U];
+#pragma unroll
+            for (int u = 0; u < PAIR_U; ++u) {
+                const int p = pair_at(p0, u);
+                if (p < nq) {
+                    a[u] = zs[p];
+                    b[u] = zs[(L - p) & (L - 1)];
+                    w[u] = wp.at(tw, p);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < PAIR_U; ++u) {
+                const int p = pair_at(p0, u);
+                if (p >= nq) continue;
+                if (p == 0) {
+                    os[0] = half * (a[u].x + a[u].y);
+                    os[L] = half * (a[u].x - a[u].y);
+                    const cpx<T> m = zs[nq];
+                    cpx<T> fa, fb;
+                    untangle_pair(m, m, twiddle_big<1>(tw, uint32_t(nq), log2n2), (T)-0.5, fa, fb);
+                    os[nq] = half * fa.x;
+                } else {
+                    cpx<T> fa, fb;
+                    untangle_pair(a[u], b[u], w[u], (T)-0.5, fa, fb);
+                    os[p] = half * fa.x;
+                    os[L - p] = half * fb.x;
+                }
+            }
+        }
+    }
+}
+
+// cosft2's even/odd permutation, 16 bytes of the real array per thread: the
+// four reals y[4q .. 4q+3] are z[q].x, z[L-1-q].y, z[q].y, z[L-1-q].x
+// (y[2i] = x_i and y[2(2L-1-i)+1] = x_i for the packed reals x of z), so
+// every lane reads or writes one contiguous aligned group and whole cache
+// lines come from one wave.  q < L/2.
+template <typename T>
+__global__ __launch_bounds__(PAIR_NT) void k_cosft2_fpack(const T* y, cpx<T>* z, int64_t L64, int64_t nsig) {
+    const int L = int(L64), nq = L / 2;
+    for (int64_t s = blockIdx.y; s < nsig; s += gridDim.y) {
+        const T* ys = y + s * 2 * L64;
+        cpx<T>* zs = z + s * L64;
+        for (int q0 = pair_first(); q0 < nq; q0 += pair_stride()) {
+            T v[PAIR_U][4];
+#pragma unroll
+            for (int u = 0; u < PAIR_U; ++u) {
+                const int q = pair_at(q0, u);
+                if (q < nq) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) v[u][k] = ys[4 * q + k];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < PAIR_U; ++u) {
+                const int q = pair_at(q0, u);
+                if (q >= nq) continue;
+                zs[q] = cpx<T>{v[u][0], v[u][2]};
+                zs[L - 1 - q] = cpx<T>{v[u][3], v[u][1]};
             }
         }
     }
 }
 
 template <typename T>
-__global__ void k_cosft2_fpack(const T* y, cpx<T>* z, int64_t L, int64_t nsig) {
-    for (int64_t s = blockIdx.y; s < nsig; s += gridDim.y) {
-        const T* ys = y + s * 2 * L;
-        cpx<T>* zs = z + s * L;
-        for (int64_t m = x_first(); m < L; m += x_stride())
-            zs[m] = cpx<T>{ys[cf2_pos(2 * m, L)], ys[cf2_pos(2 * m + 1, L)]};
-    }
-}
-
-template <typename T>
-__global__ void k_cosft2_fpost(const cpx<T>* z, T* out, int64_t L, int64_t nsig, int logn, Twiddles<T> tw) {
-    const int64_t np = L / 2 + 1, n = 2 * L;
+__global__ __launch_bounds__(PAIR_NT) void k_cosft2_fpost(const cpx<T>* z, T* out, int64_t L64, int64_t nsig, int logn,
+                                                          Twiddles<T> tw) {
+    const int L = int(L64), nq = L / 2, n = 2 * L;
     // exp(i pi (L - p) / 2n) = W_8 conj(exp(i pi p / 2n))
     const cpx<T> w8 = twiddle_big<1>(tw, (uint32_t)L, logn + 2);
     const RowTw<1, T> wn(tw, logn), w4(tw, logn + 2);
     for (int64_t s = blockIdx.y; s < nsig; s += gridDim.y) {
-        const cpx<T>* zs = z + s * L;
-        T* os = out + s * n;
-        for (int64_t p = x_first(); p < np; p += x_stride()) {
-            if (p == 0) {
-                os[0] = zs[0].x + zs[0].y;
-                os[L] = (zs[0].x - zs[0].y) * cos_quarter_pi<T>();
-            } else {
-                cpx<T> va, vb;
-                untangle_pair(zs[p], zs[L - p], wn.at(tw, p), (T)-0.5, va, vb);
-                const cpx<T> r4 = w4.at(tw, p);
-                cf2_fwd_out_w(r4, va, p, os, n);
-                if (L - p != p) cf2_fwd_out_w(cmul(w8, conj(r4)), vb, L - p, os, n);
+        const cpx<T>* zs = z + s * L64;
+        T* os = out + s * 2 * L64;
+        for (int p0 = pair_first(); p0 < nq; p0 += pair_stride()) {
+            cpx<T> a[PAIR_U], b[PAIR_U], wu[PAIR_U], r4[PAIR_U];
+#pragma unroll
+            for (int u = 0; u < PAIR_U; ++u) {
+                const int p = pair_at(p0, u);
+                if (p < nq) {
+                    a[u] = zs[p];
+                    b[u] = zs[(L - p) & (L - 1)];
+                    wu[u] = wn.at(tw, p);
+                    r4[u] = w4.at(tw, p);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < PAIR_U; ++u) {
+                const int p = pair_at(p0, u);
+                if (p >= nq) continue;
+                if (p == 0) {
+                    os[0] = a[u].x + a[u].y;
+                    os[L] = (a[u].x - a[u].y) * cos_quarter_pi<T>();
+                    const cpx<T> m = zs[nq];
+                    cpx<T> va, vb;
+                    untangle_pair(m, m, twiddle_big<1>(tw, uint32_t(nq), logn), (T)-0.5, va, vb);
+                    cf2_fwd_out_w(twiddle_big<1>(tw, uint32_t(nq), logn + 2), va, nq, os, n);
+                } else {
+                    cpx<T> va, vb;
+                    untangle_pair(a[u], b[u], wu[u], (T)-0.5, va, vb);
+                    cf2_fwd_out_w(r4[u], va, p, os, n);
+                    cf2_fwd_out_w(cmul(w8, conj(r4[u])), vb, L - p, os, n);
+                }
             }
         }
     }
 }
 
 template <typename T>
-__global__ void k_cosft2_ipre(const T* y, cpx<T>* z, int64_t L, int64_t nsig, int logn, Twiddles<T> tw) {
-    const int64_t np = L / 2 + 1, n = 2 * L;
+__global__ __launch_bounds__(PAIR_NT) void k_cosft2_ipre(const T* y, cpx<T>* z, int64_t L64, int64_t nsig, int logn,
+                                                         Twiddles<T> tw) {
+    const int L = int(L64), nq = L / 2, n = 2 * L;
     const cpx<T> w8 = twiddle_big<1>(tw, (uint32_t)L, logn + 2);
     const RowTw<1, T> wn(tw, logn), w4(tw, logn + 2);
     for (int64_t s = blockIdx.y; s < nsig; s += gridDim.y) {
-        const T* ys = y + s * n;
-        cpx<T>* zs = z + s * L;
-        for (int64_t p = x_first(); p < np; p += x_stride()) {
-            if (p == 0) {
-                zs[0] = dc_inverse(cpx<T>{ys[0], (T)1.41421356237309504880 * ys[L]});
-            } else {
-                // V_k = exp(-i pi k / 2n) (Y_k + i Y_{n-k}) for k = p and L - p
-                const cpx<T> r4 = w4.at(tw, p);
-                const cpx<T> va = cmul(conj(r4), cpx<T>{ys[p], ys[n - p]});
-                const cpx<T> vb = cmul(conj(cmul(w8, conj(r4))), cpx<T>{ys[L - p], ys[n - (L - p)]});
-                cpx<T> za, zb;
-                untangle_pair(va, vb, conj(wn.at(tw, p)), (T)0.5, za, zb);
-                zs[p] = za;
-                zs[L - p] = zb;
+        const T* ys = y + s * 2 * L64;
+        cpx<T>* zs = z + s * L64;
+        for (int p0 = pair_first(); p0 < nq; p0 += pair_stride()) {
+            // Y_p, Y_{n-p}, Y_{L-p}, Y_{L+p} (p = 0: Y_0 and Y_L)
+            T y0[PAIR_U], y1[PAIR_U], y2[PAIR_U], y3[PAIR_U];
+            cpx<T> wu[PAIR_U], r4[PAIR_U];
+#pragma unroll
+            for (int u = 0; u < PAIR_U; ++u) {
+                const int p = pair_at(p0, u);
+                if (p < nq) {
+                    y0[u] = ys[p];
+                    y1[u] = ys[(n - p) & (n - 1)];
+                    y2[u] = ys[L - p];
+                    y3[u] = ys[L + p];
+                    wu[u] = wn.at(tw, p);
+                    r4[u] = w4.at(tw, p);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < PAIR_U; ++u) {
+                const int p = pair_at(p0, u);
+                if (p >= nq) continue;
+                if (p == 0) {
+                    zs[0] = dc_inverse(cpx<T>{y0[u], (T)1.41421356237309504880 * y2[u]});
+                    // the self-paired bin p = L/2 (k = L - p = p)
+                    const cpx<T> rm = twiddle_big<1>(tw, uint32_t(nq), logn + 2);
+                    const cpx<T> vm = cmul(conj(rm), cpx<T>{ys[nq], ys[n - nq]});
+                    cpx<T> za, zb;
+                    untangle_pair(vm, vm, conj(twiddle_big<1>(tw, uint32_t(nq), logn)), (T)0.5, za, zb);
+                    zs[nq] = zb;
+                } else {
+                    // V_k = exp(-i pi k / 2n) (Y_k + i Y_{n-k}) for k = p and L - p
+                    const cpx<T> va = cmul(conj(r4[u]), cpx<T>{y0[u], y1[u]});
+                    const cpx<T> vb = cmul(conj(cmul(w8, conj(r4[u]))), cpx<T>{y2[u], y3[u]});
+                    cpx<T> za, zb;
+                    untangle_pair(va, vb, conj(wu[u]), (T)0.5, za, zb);
+                    zs[p] = za;
+                    zs[L - p] = zb;
+                }
             }
         }
     }
 }
 
 template <typename T>
-__global__ void k_cosft2_iunpack(const cpx<T>* z, T* out, int64_t L, int64_t nsig) {
+__global__ __launch_bounds__(PAIR_NT) void k_cosft2_iunpack(const cpx<T>* z, T* out, int64_t L64, int64_t nsig) {
+    const int L = int(L64), nq = L / 2;
     for (int64_t s = blockIdx.y; s < nsig; s += gridDim.y) {
-        const cpx<T>* zs = z + s * L;
-        T* os = out + s * 2 * L;
-        for (int64_t m = x_first(); m < L; m += x_stride()) {
-            os[cf2_pos(2 * m, L)] = zs[m].x;
-            os[cf2_pos(2 * m + 1, L)] = zs[m].y;
+        const cpx<T>* zs = z + s * L64;
+        T* os = out + s * 2 * L64;
+        for (int q0 = pair_first(); q0 < nq; q0 += pair_stride()) {
+            cpx<T> a[PAIR_U], b[PAIR_U];
+#pragma unroll
+            for (int u = 0; u < PAIR_U; ++u) {
+                const int q = pair_at(q0, u);
+                if (q < nq) {
+                    a[u] = zs[q];
+                    b[u] = zs[L - 1 - q];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < PAIR_U; ++u) {
+                const int q = pair_at(q0, u);
+                if (q >= nq) continue;
+                os[4 * q] = a[u].x;
+                os[4 * q + 1] = b[u].y;
+                os[4 * q + 2] = a[u].y;
+                os[4 * q + 3] = b[u].x;
+            }
         }
     }
 }

@@ -186,11 +186,20 @@ static unsigned grid_of(int64_t work, int nt) {
 
 // 2-D grid of the multi-pass helpers: y over signals, x over the `per`
 // elements of one signal, 256 threads a block, at most ~64K blocks.
-static dim3 grid2(int64_t per, int64_t nsig) {
+static dim3 grid2(int64_t per, int64_t nsig, int64_t max_blocks = 65536) {
     const int64_t gy = std::max<int64_t>(1, std::min<int64_t>(nsig, 65535));
-    const int64_t cap = std::max<int64_t>(1, 65536 / gy);
+    const int64_t cap = std::max<int64_t>(1, max_blocks / gy);
     const int64_t gx = std::max<int64_t>(1, std::min<int64_t>((per + 255) / 256, cap));
     return dim3((unsigned)gx, (unsigned)gy);
+}
+
+// Grid of the PAIR_U kernels (each thread PAIR_U items of a signal): at most
+// PAIR_BLOCKS blocks, so a block walks several spans and its per-thread
+// twiddle factor (one scattered table read per thread) is paid once per
+// block, not once per span.
+constexpr int64_t PAIR_BLOCKS = 8192;
+static dim3 grid_pairs(int64_t items, int64_t nsig) {
+    return grid2((items + PAIR_U - 1) / PAIR_U, nsig, PAIR_BLOCKS);
 }
 
 template <typename T>
@@ -230,7 +239,7 @@ static int exec_spec_t(const Plan& p, const Launch& l, void* const* bufs, int is
                                (cpx<T>*)dst, a[1], nsig);
             break;
         case SP_CF1_POST:
-            hipLaunchKernelGGL((k_cosft1_post<T>), grid2(a[1] / 2 + 1, nsig), dim3(256), 0, s,
+            hipLaunchKernelGGL((k_cosft1_post<T>), grid_pairs(a[1] / 2, nsig), dim3(256), 0, s,
                                (const cpx<T>*)src, (T*)dst, a[1], nsig, ilog2((size_t)a[1]) + 1, tw);
             break;
         case SP_CF2_FUSED: {
@@ -250,19 +259,19 @@ static int exec_spec_t(const Plan& p, const Launch& l, void* const* bufs, int is
                                    (T*)dst, nsig);
             break;
         case SP_CF2_FPACK:
-            hipLaunchKernelGGL((k_cosft2_fpack<T>), grid2(a[1], nsig), dim3(256), 0, s, (const T*)src,
+            hipLaunchKernelGGL((k_cosft2_fpack<T>), grid_pairs(a[1] / 2, nsig), dim3(256), 0, s, (const T*)src,
                                (cpx<T>*)dst, a[1], nsig);
             break;
         case SP_CF2_FPOST:
-            hipLaunchKernelGGL((k_cosft2_fpost<T>), grid2(a[1] / 2 + 1, nsig), dim3(256), 0, s,
+            hipLaunchKernelGGL((k_cosft2_fpost<T>), grid_pairs(a[1] / 2, nsig), dim3(256), 0, s,
                                (const cpx<T>*)src, (T*)dst, a[1], nsig, ilog2((size_t)a[1]) + 1, tw);
             break;
         case SP_CF2_IPRE:
-            hipLaunchKernelGGL((k_cosft2_ipre<T>), grid2(a[1] / 2 + 1, nsig), dim3(256), 0, s,
+            hipLaunchKernelGGL((k_cosft2_ipre<T>), grid_pairs(a[1] / 2, nsig), dim3(256), 0, s,
                                (const T*)src, (cpx<T>*)dst, a[1], nsig, ilog2((size_t)a[1]) + 1, tw);
             break;
         case SP_CF2_IUNPACK:
-            hipLaunchKernelGGL((k_cosft2_iunpack<T>), grid2(a[1], nsig), dim3(256), 0, s,
+            hipLaunchKernelGGL((k_cosft2_iunpack<T>), grid_pairs(a[1] / 2, nsig), dim3(256), 0, s,
                                (const cpx<T>*)src, (T*)dst, a[1], nsig);
             break;
         case SP_R3_COPY:
