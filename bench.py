@@ -394,6 +394,23 @@ def _extra_convlv() -> dict:
         "GFLOPs": round(3 * 2.5 * n * 14 * b / (ms * 1e-3) / 1e9, 1), "launches": plan.num_launches}
     del plan, d, r, a
     torch.cuda.empty_cache()
+
+    # batched realft beyond one group: 64 x 2^21 FP32 reals, forward, in place
+    # (C2C of 2^20 points in 16-transform chunks + the untangle pass).
+    b, n = 64, 1 << 21
+    x = torch.empty(b * n, dtype=torch.float32, device="cuda")
+    fill_normal(x, 2, b * n)
+    plan = numrs_amd.RealFTPlan("float32", n, b)
+    for _ in range(3):
+        plan.execute(x, 1)
+    ms = time_events(lambda: plan.execute(x, 1), 20)
+    byt = 2.0 * n * 4 * b
+    res["realft_f32_64x2p21_fwd"] = {
+        "ms": round(ms, 4), "GBps": round(byt / (ms * 1e-3) / 1e9, 1),
+        "frac_hbm": round(byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "GFLOPs": round(2.5 * n * 21 * b / (ms * 1e-3) / 1e9, 1), "launches": plan.num_launches}
+    del plan, x
+    torch.cuda.empty_cache()
     return res
 
 

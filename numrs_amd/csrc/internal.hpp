@@ -127,6 +127,9 @@ int build_plan_spectral(Plan& p);
 int exec_spectral(const Plan& p, const Launch& l, void* const* bufs, int isign, hipStream_t stream, DeviceCtx* ctx);
 int plan_c2c_nd(Plan& p, const std::vector<size_t>& nn, int64_t batch);
 void tune_lookup(Plan& p);
+// Give a plan whose C2C runs inside a larger transform the tuning of a plain
+// C2C plan of length 2^logn and the plan's batch.
+void use_c2c_tune(Plan& p, int logn);
 bool tune_find(int kind, int dtype, const std::vector<size_t>& nn, size_t batch, Tune* out);
 void tune_store(int kind, int dtype, const std::vector<size_t>& nn, size_t batch, const Tune& t);
 Tune default_tune(const Plan& p);

@@ -23,21 +23,41 @@ constexpr int REAL_MAX_LOGL = 13;  // fused single-group path: n/2 <= 8192
 // ---------------------------------------------------------------------------
 // Simple kernels of the multi-pass path (n/2 > 8192) and of small sizes.
 // ---------------------------------------------------------------------------
-// In-place untangle of packed spectra, one thread per bin pair.
+// In-place untangle of packed spectra: bin pairs (p, L - p), p < L/2, PAIR_U
+// per thread with every load before the first store; p = 0 is the packed
+// DC / Nyquist slot.  The self-paired bin L/2 meets the twiddle W^{L/2} = i
+// (forward) or -i (inverse), for which the untangle is the identity.
 template <typename T, int DIR>
-__global__ void k_untangle(cpx<T>* data, int64_t L, int64_t nsig, int logn, Twiddles<T> tw) {
-    const int64_t np = L / 2;
-    const int64_t total = np * nsig;
-    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < total; q += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t s = q / np, p = q % np;
-        cpx<T>* d = data + s * L;
-        if (p == 0) {
-            d[0] = DIR > 0 ? dc_forward(d[0]) : dc_inverse(d[0]);
-        } else {
-            cpx<T> a, b;
-            untangle_pair(d[p], d[L - p], twiddle_big<DIR>(tw, (uint32_t)p, logn), (T)(-0.5 * DIR), a, b);
-            d[p] = a;
-            d[L - p] = b;
+__global__ __launch_bounds__(PAIR_NT) void k_untangle(cpx<T>* data, int64_t L64, int64_t nsig, int logn,
+                                                      Twiddles<T> tw) {
+    const int L = int(L64), nq = L / 2;
+    const RowTw<DIR, T> wr(tw, logn);
+    for (int64_t s = blockIdx.y; s < nsig; s += gridDim.y) {
+        cpx<T>* d = data + s * L64;
+        for (int p0 = pair_first(); p0 < nq; p0 += pair_stride()) {
+            cpx<T> a[PAIR_U], b[PAIR_U], w[PAIR_U];
+#pragma unroll
+            for (int u = 0; u < PAIR_U; ++u) {
+                const int p = pair_at(p0, u);
+                if (p < nq) {
+                    a[u] = d[p];
+                    b[u] = d[(L - p) & (L - 1)];
+                    w[u] = wr.at(tw, p);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < PAIR_U; ++u) {
+                const int p = pair_at(p0, u);
+                if (p >= nq) continue;
+                if (p == 0) {
+                    d[0] = DIR > 0 ? dc_forward(a[u]) : dc_inverse(a[u]);
+                } else {
+                    cpx<T> x, y;
+                    untangle_pair(a[u], b[u], w[u], (T)(-0.5 * DIR), x, y);
+                    d[p] = x;
+                    d[L - p] = y;
+                }
+            }
         }
     }
 }
@@ -204,6 +224,7 @@ int build_plan_real(Plan& p) {
             // The direction-dependent order is resolved at execution time: the
             // plan holds both sequences, tagged by args[3] (1 = fwd only, -1 = inv only).
             size_t tmp = 0;
+            use_c2c_tune(p, logl);
             Plan fw = p, iv = p;
             fw.launches.clear();
             iv.launches.clear();
@@ -238,6 +259,7 @@ int build_plan_real(Plan& p) {
     // Multi-pass: spectra of the signal (into out) and of the response
     // (into TMP2), spectrum op + inverse untangle, inverse C2C.
     size_t tmp = 0;
+    use_c2c_tune(p, logl);
     int rc;
     if (p.kind == PK_CONVLV) {
         p.launches.push_back(real_launch(RS_PLACE, B_AUX, B_TMP2, {batch, (int64_t)n, m}));
@@ -303,10 +325,10 @@ static int exec_real_t(const Plan& p, const Launch& l, void* const* bufs, int is
                                                   hipMemcpyDeviceToDevice, stream));
             const int logn = ilog2((size_t)a[1]) + 1;
             if (a[2] > 0)
-                hipLaunchKernelGGL((k_untangle<T, 1>), dim3(grid_for(a[0] * a[1] / 2, 256)), dim3(256), 0, stream,
+                hipLaunchKernelGGL((k_untangle<T, 1>), grid_pairs(a[1] / 2, a[0]), dim3(PAIR_NT), 0, stream,
                                    (cpx<T>*)dst, a[1], a[0], logn, tw);
             else
-                hipLaunchKernelGGL((k_untangle<T, -1>), dim3(grid_for(a[0] * a[1] / 2, 256)), dim3(256), 0, stream,
+                hipLaunchKernelGGL((k_untangle<T, -1>), grid_pairs(a[1] / 2, a[0]), dim3(PAIR_NT), 0, stream,
                                    (cpx<T>*)dst, a[1], a[0], logn, tw);
             HIPCHK(hipGetLastError());
             return NRFFT_OK;

@@ -33,6 +33,48 @@ __device__ __forceinline__ void untangle_pair(cpx<T> A, cpx<T> B, cpx<T> w, T c2
     olk = conj(h1 - wh);
 }
 
+// W_M^p for the index p of a loop over a signal's elements or pairs: W_M^threadIdx (per thread, once)
+// times W_M^(p - threadIdx), the same for the whole block (one cache line per
+// wave) -- a direct table lookup of consecutive p would step through the
+// fine table 2^(26 - log2 M) entries at a time, one cache line per lane.
+template <int S, typename T>
+struct RowTw {
+    cpx<T> wt;
+    int logm;
+    __device__ __forceinline__ RowTw(const Twiddles<T>& tw, int logm_) : logm(logm_) {
+        wt = twiddle_big<S>(tw, uint32_t(threadIdx.x), logm);
+    }
+    __device__ __forceinline__ cpx<T> at(const Twiddles<T>& tw, int64_t p) const {
+        return cmul(wt, twiddle_big<S>(tw, uint32_t(p - threadIdx.x), logm));
+    }
+};
+
+// Bin pairs per thread in the multi-pass untangle kernels: a block takes
+// PAIR_U * blockDim.x consecutive pairs p = base + threadIdx.x + u blockDim.x,
+// issues every load of its pairs (data and block-uniform twiddle factors)
+// before the first store, then finishes them -- one pair per thread left too
+// few loads in flight to cover HBM latency.
+// Indices inside one signal are 32-bit (n <= 2^26): 64-bit index arithmetic
+// on eight address streams spilled registers.  Blocks are 256 threads.
+constexpr int PAIR_U = 4;
+constexpr int PAIR_NT = 256;
+__device__ __forceinline__ int pair_first() { return int(blockIdx.x) * PAIR_NT * PAIR_U; }
+__device__ __forceinline__ int pair_stride() { return int(gridDim.x) * PAIR_NT * PAIR_U; }
+__device__ __forceinline__ int pair_at(int p0, int u) { return p0 + int(threadIdx.x) + u * PAIR_NT; }
+
+// Grid of the PAIR_U kernels: y over signals, x over spans of PAIR_NT *
+// PAIR_U items of one signal, at most PAIR_BLOCKS blocks, so a block walks
+// several spans and its per-thread twiddle factor (one scattered table read
+// per thread) is paid once per block, not once per span.
+constexpr int64_t PAIR_BLOCKS = 8192;
+inline dim3 grid_pairs(int64_t items, int64_t nsig) {
+    const int64_t gy = nsig < 1 ? 1 : (nsig > 65535 ? 65535 : nsig);
+    const int64_t cap = PAIR_BLOCKS / gy > 0 ? PAIR_BLOCKS / gy : 1;
+    int64_t gx = (items + int64_t(PAIR_NT) * PAIR_U - 1) / (int64_t(PAIR_NT) * PAIR_U);
+    gx = gx < 1 ? 1 : (gx > cap ? cap : gx);
+    return dim3((unsigned)gx, (unsigned)gy);
+}
+
 // Slot 0 of the packed array: forward (Z_0 -> [F_0, F_{n/2}]) and inverse.
 template <typename T>
 __device__ __forceinline__ cpx<T> dc_forward(cpx<T> z0) { return {z0.x + z0.y, z0.x - z0.y}; }

@@ -291,22 +291,6 @@ __global__ void k_twofft_pack(const T* d1, const T* d2, cpx<T>* z, int64_t total
 __device__ __forceinline__ int64_t x_first() { return blockIdx.x * int64_t(blockDim.x) + threadIdx.x; }
 __device__ __forceinline__ int64_t x_stride() { return int64_t(gridDim.x) * blockDim.x; }
 
-// W_M^p for the x index p of such a loop: W_M^threadIdx (per thread, once)
-// times W_M^(p - threadIdx), the same for the whole block (one cache line per
-// wave) -- a direct table lookup of consecutive p would step through the
-// fine table 2^(26 - log2 M) entries at a time, one cache line per lane.
-template <int S, typename T>
-struct RowTw {
-    cpx<T> wt;
-    int logm;
-    __device__ __forceinline__ RowTw(const Twiddles<T>& tw, int logm_) : logm(logm_) {
-        wt = twiddle_big<S>(tw, uint32_t(threadIdx.x), logm);
-    }
-    __device__ __forceinline__ cpx<T> at(const Twiddles<T>& tw, int64_t p) const {
-        return cmul(wt, twiddle_big<S>(tw, uint32_t(p - threadIdx.x), logm));
-    }
-};
-
 // In place on f1 (which holds Z), second spectrum to f2.
 template <typename T>
 __global__ void k_twofft_split(cpx<T>* f1, cpx<T>* f2, int64_t n, int64_t nsig) {
@@ -337,19 +321,6 @@ __global__ void k_cosft1_pack(const T* y, cpx<T>* z, int64_t L, int64_t nsig) {
             zs[m] = cpx<T>{cf1_ext(ys, 2 * m, L), cf1_ext(ys, 2 * m + 1, L)};
     }
 }
-
-// Bin pairs per thread in the multi-pass untangle kernels: a block takes
-// PAIR_U * blockDim.x consecutive pairs p = base + threadIdx.x + u blockDim.x,
-// issues every load of its pairs (data and block-uniform twiddle factors)
-// before the first store, then finishes them -- one pair per thread left too
-// few loads in flight to cover HBM latency.
-// Indices inside one signal are 32-bit (n <= 2^26): 64-bit index arithmetic
-// on eight address streams spilled registers.  Blocks are 256 threads.
-constexpr int PAIR_U = 4;
-constexpr int PAIR_NT = 256;
-__device__ __forceinline__ int pair_first() { return int(blockIdx.x) * PAIR_NT * PAIR_U; }
-__device__ __forceinline__ int pair_stride() { return int(gridDim.x) * PAIR_NT * PAIR_U; }
-__device__ __forceinline__ int pair_at(int p0, int u) { return p0 + int(threadIdx.x) + u * PAIR_NT; }
 
 // The pairs p < L/2 of a signal; the thread of p = 0 also finishes the
 // self-paired bin p = L/2, so the spans of a signal divide evenly among blocks.

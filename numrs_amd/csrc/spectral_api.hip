@@ -61,18 +61,6 @@ static void tag_launches(Plan& p, size_t from, int force, int only) {
     }
 }
 
-// The inner C2C of a multi-pass plan runs with the tuning of a C2C plan of
-// its length and batch (four-step split, tile widths, batch chunking).
-static void use_c2c_tune(Plan& p, int logn) {
-    Plan q;
-    q.kind = PK_C2C;
-    q.dtype = p.dtype;
-    q.nn = {size_t(1) << logn};
-    q.batch = p.batch;
-    tune_lookup(q);
-    p.tune = q.tune;
-}
-
 int build_plan_spectral(Plan& p) {
     const int64_t batch = (int64_t)p.batch;
     size_t tmp = 0;
@@ -191,15 +179,6 @@ static dim3 grid2(int64_t per, int64_t nsig, int64_t max_blocks = 65536) {
     const int64_t cap = std::max<int64_t>(1, max_blocks / gy);
     const int64_t gx = std::max<int64_t>(1, std::min<int64_t>((per + 255) / 256, cap));
     return dim3((unsigned)gx, (unsigned)gy);
-}
-
-// Grid of the PAIR_U kernels (each thread PAIR_U items of a signal): at most
-// PAIR_BLOCKS blocks, so a block walks several spans and its per-thread
-// twiddle factor (one scattered table read per thread) is paid once per
-// block, not once per span.
-constexpr int64_t PAIR_BLOCKS = 8192;
-static dim3 grid_pairs(int64_t items, int64_t nsig) {
-    return grid2((items + PAIR_U - 1) / PAIR_U, nsig, PAIR_BLOCKS);
 }
 
 template <typename T>

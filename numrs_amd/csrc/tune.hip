@@ -85,6 +85,18 @@ Tune default_tune(const Plan& p) {
     return t;
 }
 
+// The inner C2C of a multi-pass plan runs with the tuning of a C2C plan of
+// its length and batch (four-step split, tile widths, batch chunking).
+void use_c2c_tune(Plan& p, int logn) {
+    Plan q;
+    q.kind = PK_C2C;
+    q.dtype = p.dtype;
+    q.nn = {size_t(1) << logn};
+    q.batch = p.batch;
+    tune_lookup(q);
+    p.tune = q.tune;
+}
+
 void tune_lookup(Plan& p) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     if (!g_tune_env_loaded) {

@@ -160,6 +160,32 @@ def test_realft_f32(logn):
     assert rel_l2(y, (n / 2) * x) < 1e-5
 
 
+def test_realft_plan_multipass_chunked_f32():
+    # 36 FP32 signals of 2^21 reals: the 2^20-point C2C runs in chunks of 16
+    # transforms (16, 16, 4), then the untangle pass.  Signals of the first,
+    # a middle and the tail chunk against numpy's FP64 rfft (NR's sign:
+    # F = conj(rfft), packed [F_0, F_{n/2}, Re F_1, Im F_1, ...]); then the
+    # inverse returns (n/2) x.
+    import torch
+
+    n, b = 1 << 21, 36
+    x = synth.normal(30, b * n).astype(np.float32)
+    t = torch.from_numpy(x).cuda()
+    plan = numrs_amd.RealFTPlan("float32", n, b)
+    plan.execute(t, 1)
+    got = t.cpu().numpy()
+    for i in (0, 17, 35):
+        f = np.conj(np.fft.rfft(x[i * n:(i + 1) * n].astype(np.float64)))
+        ref = np.empty(n)
+        ref[0], ref[1] = f[0].real, f[n // 2].real
+        ref[2::2], ref[3::2] = f[1:n // 2].real, f[1:n // 2].imag
+        assert rel_l2(got[i * n:(i + 1) * n], ref) < TOL32, i
+    plan.execute(t, -1)
+    back = t.cpu().numpy()
+    for i in (0, 17, 35):
+        assert rel_l2(back[i * n:(i + 1) * n], (n / 2) * x[i * n:(i + 1) * n].astype(np.float64)) < TOL32, i
+
+
 # ---- fourn -------------------------------------------------------------------
 @pytest.mark.parametrize("shape", [(2, 2), (4, 8, 16), (16, 2), (2, 64, 4), (32, 32, 32), (8, 4096), (64, 256, 2),
                                    (2, 2, 2, 2, 2)])
