@@ -277,6 +277,10 @@ int plan_c2c_1d(Plan& p, int logn, int64_t batch, int src, int dst, size_t* tmp_
         b.si1 = N;
         b.so1 = N;
         b.nb2 = 1;
+        // The plan's result: stored non-temporally, so the writes do not
+        // evict the next chunk's input and intermediate from L2 / MALL
+        // (64 x 2^20 FP32: 0.487 -> 0.454 ms on MI355X).
+        b.nt_out = dst == B_OUT ? 1 : 0;
         rc = plan_add_c2c_pass(p, MODE_JT, l2, tb, b, B_TMP, dst);
         if (rc) return rc;
         p.launches.back().off_dst = c0 * N;
@@ -338,6 +342,8 @@ int plan_c2c_nd(Plan& p, const std::vector<size_t>& nn, int64_t batch) {
             if (rc) return rc;
         }
     }
+    // Later axis passes re-read what earlier ones wrote: plain stores only.
+    for (auto& l : p.launches) l.desc.nt_out = 0;
     p.tmp_elems = tmp;
     return NRFFT_OK;
 }

@@ -190,10 +190,18 @@ __device__ __forceinline__ void mfma_stages(cpx<float>* lds, const cpx<float>* l
                 }
                 c32* obt = ob + (int64_t)t * so0 + (int64_t)(j + 2 * kb * NB) * eso;
                 const int64_t st = (int64_t)NB * eso;
-                obt[0] = o[0];
-                obt[st] = o[1];
-                obt[8 * st] = o[2];
-                obt[9 * st] = o[3];
+                if (d.nt_out) {  // the plan's result, not re-read by it
+                    using V2 = float __attribute__((ext_vector_type(2)));
+                    __builtin_nontemporal_store(V2{o[0].x, o[0].y}, reinterpret_cast<V2*>(obt));
+                    __builtin_nontemporal_store(V2{o[1].x, o[1].y}, reinterpret_cast<V2*>(obt + st));
+                    __builtin_nontemporal_store(V2{o[2].x, o[2].y}, reinterpret_cast<V2*>(obt + 8 * st));
+                    __builtin_nontemporal_store(V2{o[3].x, o[3].y}, reinterpret_cast<V2*>(obt + 9 * st));
+                } else {
+                    obt[0] = o[0];
+                    obt[st] = o[1];
+                    obt[8 * st] = o[2];
+                    obt[9 * st] = o[3];
+                }
             }
         } else {
             const unsigned i0 = (j / NS) * NS * 16 + (j & (NS - 1)) + 2 * kb * NS;

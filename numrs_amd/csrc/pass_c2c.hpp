@@ -8,6 +8,8 @@
 // is stored, and groups own disjoint element sets when in == out with equal
 // in/out strides, so such passes may run in place.
 #pragma once
+#include <type_traits>
+
 #include "device_fft.hpp"
 
 namespace nrfft {
@@ -21,6 +23,7 @@ struct PassDesc {
     int32_t tw_logn;        // 0 = no four-step twiddle
     int32_t tw_b0_shift;
     int32_t xcd_remap;      // 1: give each XCD a contiguous run of tiles
+    int32_t nt_out;         // 1: non-temporal output stores (the plan does not re-read them)
 };
 
 // Persistent pass kernel: group g walks tiles g, g + G, g + 2G, ... (G =
@@ -81,6 +84,7 @@ template <typename T, int S, int L, int TI, int NT, int ML, int MS>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, NT), amdgpu_waves_per_eu(kc2c_waves<T, S, L, TI, NT, ML, MS>))) void
 k_c2c(const cpx<T>* in, cpx<T>* out, PassDesc d, Twiddles<T> tw) {
     using E = Engine<T, S, L, TI, NT, ML, MS>;
+    using V2 = T __attribute__((ext_vector_type(2)));
     constexpr int EPT = E::EPT;
     constexpr int LTM = kc2c_ltm<T, S, L, TI, NT, ML, MS>();
     __shared__ cpx<T> lds[E::NST > 1 ? E::LDS_ELEMS : 1];
@@ -126,52 +130,72 @@ k_c2c(const cpx<T>* in, cpx<T>* out, PassDesc d, Twiddles<T> tw) {
     cpx<T> vn[EPT];
     load(vn, cn, fresh_tid(tid));
 #endif
-    bool first = true;
-    for (;;) {
-        cpx<T> v[EPT];
+    // The tile loop, with plain or non-temporal result stores (NTS): a
+    // runtime choice between two stores of the same value would be merged
+    // into one plain store by the compiler.
+    auto tiles = [&](auto nts) {
+        constexpr bool NTS = decltype(nts)::value;
+        bool first = true;
+        for (;;) {
+            cpx<T> v[EPT];
 #if NRFFT_PREFETCH
 #pragma unroll
-        for (int e = 0; e < EPT; ++e) v[e] = vn[e];
-        const Ctx c = cn;
-        const int64_t next = tile + G;
-        const bool has_next = next < ntiles;
-        if (has_next) {
-            cn = ctx_of(next);
-            load(vn, cn, fresh_tid(tid));
-        }
-        const int ltid = fresh_tid(tid);
+            for (int e = 0; e < EPT; ++e) v[e] = vn[e];
+            const Ctx c = cn;
+            const int64_t next = tile + G;
+            const bool has_next = next < ntiles;
+            if (has_next) {
+                cn = ctx_of(next);
+                load(vn, cn, fresh_tid(tid));
+            }
+            const int ltid = fresh_tid(tid);
 #else
-        const Ctx c = ctx_of(tile);
-        const int64_t next = tile + G;
-        const bool has_next = next < ntiles;
-        const int ltid = fresh_tid(tid);
-        load(v, c, ltid);
+            const Ctx c = ctx_of(tile);
+            const int64_t next = tile + G;
+            const bool has_next = next < ntiles;
+            const int ltid = fresh_tid(tid);
+            load(v, c, ltid);
 #endif
-        E::template compute<0>(v, ltid, tw.stage);
-        E::template middle<1, LTM>(v, ltid, lds, tw.stage, !first, ltw);
-        if (d.tw_logn) {
-            const int logn = d.tw_logn, sh = d.tw_b0_shift;
-            const uint32_t mask = (1u << logn) - 1u;
-            E::store_last(v, ltid, [&](int t, int j, const cpx<T>* w) {
-                if (TI == 1 || t < c.nvalid) {
-                    const uint32_t n2 = uint32_t((c.b0base + t) >> sh);
-                    cpx<T> f[E::RLAST];
-                    twiddle_run<E::RLAST, S>(tw, (uint32_t(j) * n2) & mask, (uint32_t(E::NBLAST) * n2) & mask, logn, f);
+            E::template compute<0>(v, ltid, tw.stage);
+            E::template middle<1, LTM>(v, ltid, lds, tw.stage, !first, ltw);
+            if (d.tw_logn) {
+                const int logn = d.tw_logn, sh = d.tw_b0_shift;
+                const uint32_t mask = (1u << logn) - 1u;
+                E::store_last(v, ltid, [&](int t, int j, const cpx<T>* w) {
+                    if (TI == 1 || t < c.nvalid) {
+                        const uint32_t n2 = uint32_t((c.b0base + t) >> sh);
+                        cpx<T> f[E::RLAST];
+                        twiddle_run<E::RLAST, S>(tw, (uint32_t(j) * n2) & mask, (uint32_t(E::NBLAST) * n2) & mask, logn, f);
 #pragma unroll
-                    for (int r = 0; r < E::RLAST; ++r) c.ob[t * so0 + (j + r * E::NBLAST) * eso] = cmul(w[r], f[r]);
-                }
-            });
-        } else {
-            E::store_last(v, ltid, [&](int t, int j, const cpx<T>* w) {
-                if (TI == 1 || t < c.nvalid) {
+                        for (int r = 0; r < E::RLAST; ++r) c.ob[t * so0 + (j + r * E::NBLAST) * eso] = cmul(w[r], f[r]);
+                    }
+                });
+            } else {
+                E::store_last(v, ltid, [&](int t, int j, const cpx<T>* w) {
+                    if (TI == 1 || t < c.nvalid) {
 #pragma unroll
-                    for (int r = 0; r < E::RLAST; ++r) c.ob[t * so0 + (j + r * E::NBLAST) * eso] = w[r];
-                }
-            });
+                        for (int r = 0; r < E::RLAST; ++r) {
+                            cpx<T>* o = &c.ob[t * so0 + (j + r * E::NBLAST) * eso];
+                            if constexpr (NTS)
+                                __builtin_nontemporal_store(V2{w[r].x, w[r].y}, reinterpret_cast<V2*>(o));
+                            else
+                                *o = w[r];
+                        }
+                    }
+                });
+            }
+            if (!has_next) break;
+            first = false;
+            tile = next;
         }
-        if (!has_next) break;
-        first = false;
-        tile = next;
+    };
+    if constexpr (ML == MAP_J && MS == MAP_T) {  // a four-step pass B: may store the plan's result
+        if (d.nt_out)
+            tiles(std::true_type{});
+        else
+            tiles(std::false_type{});
+    } else {
+        tiles(std::false_type{});
     }
 }
 
