@@ -277,10 +277,12 @@ int plan_c2c_1d(Plan& p, int logn, int64_t batch, int src, int dst, size_t* tmp_
         b.si1 = N;
         b.so1 = N;
         b.nb2 = 1;
-        // The plan's result: stored non-temporally, so the writes do not
+        // The transform's result: stored non-temporally, so the writes do not
         // evict the next chunk's input and intermediate from L2 / MALL
-        // (64 x 2^20 FP32: 0.487 -> 0.454 ms on MI355X).
-        b.nt_out = dst == B_OUT ? 1 : 0;
+        // (64 x 2^20 FP32: 0.487 -> 0.454 ms on MI355X).  Whatever reads it
+        // next (the caller, or a finishing kernel of a real or spectral plan)
+        // runs after the last chunk.
+        b.nt_out = 1;
         rc = plan_add_c2c_pass(p, MODE_JT, l2, tb, b, B_TMP, dst);
         if (rc) return rc;
         p.launches.back().off_dst = c0 * N;

@@ -160,6 +160,25 @@ def test_realft_f32(logn):
     assert rel_l2(y, (n / 2) * x) < 1e-5
 
 
+@pytest.mark.parametrize("logn,batch", [(16, 300), (17, 150), (18, 70), (19, 40), (21, 10), (22, 6)])
+def test_c2c_f32_default_tuning(logn, batch):
+    # The FP32 four-step defaults: 512-point pass-B rows below 2^19, 2048-point
+    # rows above; 64 / 128 MiB chunks, each batch here ending in a partial
+    # chunk.  First and last transform against numpy (isign = +1 is N ifft).
+    import torch
+
+    n = 1 << logn
+    x = synth.uniform(logn, 2 * n * batch).astype(np.float32)
+    t = torch.from_numpy(x).cuda()
+    o = torch.empty_like(t)
+    numrs_amd.C2CPlan("float32", [n], batch).execute(t, o, 1)
+    got = o.cpu().numpy()
+    for i in (0, batch - 1):
+        xc = x[2 * i * n:2 * (i + 1) * n].astype(np.float64).view(np.complex128)
+        z = n * np.fft.ifft(xc)
+        assert rel_l2(got[2 * i * n:2 * (i + 1) * n], z.view(np.float64)) < TOL32, (logn, i)
+
+
 def test_realft_plan_multipass_chunked_f32():
     # 36 FP32 signals of 2^21 reals: the 2^20-point C2C runs in chunks of 16
     # transforms (16, 16, 4), then the untangle pass.  Signals of the first,
