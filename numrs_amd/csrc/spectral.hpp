@@ -288,28 +288,27 @@ __device__ __forceinline__ int64_t x_stride() { return int64_t(gridDim.x) * bloc
 // twofft beyond one group: Z1, Z2 are the n/2-point spectra of data1, data2
 // read as complex values; each realft untangle (W = exp(2 pi i / n)) gives
 // the bins k <= n/2 of its real input, written as the full n-bin spectrum
-// (F[n - k] = conj F[k]).  Pairs (p, N - p), N = n/2, p < N/2; the thread of
-// p = 0 also writes DC / Nyquist and the self-paired bin N/2 (F = Z there).
+// (F[n - k] = conj F[k]).  Grid y runs over (signal, array) pairs, so a
+// thread keeps two load and four store streams.  Pairs (p, N - p), N = n/2,
+// p < N/2; the thread of p = 0 also writes DC / Nyquist and the self-paired
+// bin N/2 (F = Z there).
 template <typename T>
 __global__ __launch_bounds__(PAIR_NT) void k_twofft_real(const cpx<T>* z1, const cpx<T>* z2, cpx<T>* f1, cpx<T>* f2,
                                                          int64_t n64, int64_t nsig, int logn, Twiddles<T> tw) {
     const int n = int(n64), N = n / 2, nq = N / 2;
     const RowTw<1, T> wr(tw, logn);
-    for (int64_t s = blockIdx.y; s < nsig; s += gridDim.y) {
-        const cpx<T>* a1 = z1 + s * int64_t(N);
-        const cpx<T>* a2 = z2 + s * int64_t(N);
-        cpx<T>* o1 = f1 + s * n64;
-        cpx<T>* o2 = f2 + s * n64;
+    for (int64_t y = blockIdx.y; y < 2 * nsig; y += gridDim.y) {
+        const int64_t s = y >> 1;
+        const cpx<T>* a = ((y & 1) ? z2 : z1) + s * int64_t(N);
+        cpx<T>* o = ((y & 1) ? f2 : f1) + s * n64;
         for (int p0 = pair_first(); p0 < nq; p0 += pair_stride()) {
-            cpx<T> x1[PAIR_U], y1[PAIR_U], x2[PAIR_U], y2[PAIR_U], w[PAIR_U];
+            cpx<T> x[PAIR_U], z[PAIR_U], w[PAIR_U];
 #pragma unroll
             for (int u = 0; u < PAIR_U; ++u) {
                 const int p = pair_at(p0, u);
                 if (p < nq) {
-                    x1[u] = a1[p];
-                    y1[u] = a1[(N - p) & (N - 1)];
-                    x2[u] = a2[p];
-                    y2[u] = a2[(N - p) & (N - 1)];
+                    x[u] = a[p];
+                    z[u] = a[(N - p) & (N - 1)];
                     w[u] = wr.at(tw, p);
                 }
             }
@@ -318,27 +317,18 @@ __global__ __launch_bounds__(PAIR_NT) void k_twofft_real(const cpx<T>* z1, const
                 const int p = pair_at(p0, u);
                 if (p >= nq) continue;
                 if (p == 0) {
-                    o1[0] = cpx<T>{x1[u].x + x1[u].y, (T)0};
-                    o1[N] = cpx<T>{x1[u].x - x1[u].y, (T)0};
-                    o2[0] = cpx<T>{x2[u].x + x2[u].y, (T)0};
-                    o2[N] = cpx<T>{x2[u].x - x2[u].y, (T)0};
-                    const cpx<T> m1 = a1[nq], m2 = a2[nq];
-                    o1[nq] = m1;
-                    o1[N + nq] = conj(m1);
-                    o2[nq] = m2;
-                    o2[N + nq] = conj(m2);
+                    o[0] = cpx<T>{x[u].x + x[u].y, (T)0};
+                    o[N] = cpx<T>{x[u].x - x[u].y, (T)0};
+                    const cpx<T> m = a[nq];
+                    o[nq] = m;
+                    o[N + nq] = conj(m);
                 } else {
                     cpx<T> g, h;
-                    untangle_pair(x1[u], y1[u], w[u], (T)-0.5, g, h);
-                    o1[p] = g;
-                    o1[N - p] = h;
-                    o1[n - p] = conj(g);
-                    o1[N + p] = conj(h);
-                    untangle_pair(x2[u], y2[u], w[u], (T)-0.5, g, h);
-                    o2[p] = g;
-                    o2[N - p] = h;
-                    o2[n - p] = conj(g);
-                    o2[N + p] = conj(h);
+                    untangle_pair(x[u], z[u], w[u], (T)-0.5, g, h);
+                    o[p] = g;
+                    o[N - p] = h;
+                    o[n - p] = conj(g);
+                    o[N + p] = conj(h);
                 }
             }
         }

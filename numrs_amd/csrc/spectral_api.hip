@@ -209,7 +209,7 @@ static int exec_spec_t(const Plan& p, const Launch& l, void* const* bufs, int is
         }
         case SP_TW_REAL: {
             const cpx<T>* z1 = (const cpx<T>*)src;
-            hipLaunchKernelGGL((k_twofft_real<T>), grid_pairs(a[1] / 4, nsig), dim3(PAIR_NT), 0, s, z1,
+            hipLaunchKernelGGL((k_twofft_real<T>), grid_pairs(a[1] / 4, 2 * nsig), dim3(PAIR_NT), 0, s, z1,
                                z1 + nsig * (a[1] / 2), (cpx<T>*)dst, (cpx<T>*)bufs[B_AUX2], a[1], nsig,
                                ilog2((size_t)a[1]), tw);
             break;

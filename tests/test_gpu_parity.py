@@ -179,6 +179,23 @@ def test_c2c_f32_default_tuning(logn, batch):
         assert rel_l2(got[2 * i * n:2 * (i + 1) * n], z.view(np.float64)) < TOL32, (logn, i)
 
 
+@pytest.mark.parametrize("logn,batch", [(17, 40), (19, 20), (21, 6)])
+def test_c2c_f64_default_tuning(logn, batch):
+    # FP64 four-step defaults with a partial last chunk; first and last
+    # transform against numpy.
+    import torch
+
+    n = 1 << logn
+    x = synth.uniform(logn + 40, 2 * n * batch)
+    t = torch.from_numpy(x).cuda()
+    o = torch.empty_like(t)
+    numrs_amd.C2CPlan("float64", [n], batch).execute(t, o, 1)
+    got = o.cpu().numpy()
+    for i in (0, batch - 1):
+        z = n * np.fft.ifft(x[2 * i * n:2 * (i + 1) * n].view(np.complex128))
+        assert rel_l2(got[2 * i * n:2 * (i + 1) * n], z.view(np.float64)) < TOL64, (logn, i)
+
+
 def test_realft_plan_multipass_chunked_f32():
     # 36 FP32 signals of 2^21 reals: the 2^20-point C2C runs in chunks of 16
     # transforms (16, 16, 4), then the untangle pass.  Signals of the first,
