@@ -62,38 +62,62 @@ __global__ __launch_bounds__(PAIR_NT) void k_untangle(cpx<T>* data, int64_t L64,
     }
 }
 
-// Padded responses (Convolve.rs:41-63) for nsig signals.
+// Padded responses (Convolve.rs:41-63) for nsig signals: grid y over
+// signals, x over the n slots of one signal.
 template <typename T>
 __global__ void k_place(const T* r, T* out, int64_t n, int64_t m, int64_t nsig) {
-    const int64_t total = n * nsig;
-    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < total; q += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t s = q / n, e = q % n;
-        out[q] = placed_response(r + s * m, e, n, m);
+    for (int64_t s = blockIdx.y; s < nsig; s += gridDim.y) {
+        const T* rs = r + s * m;
+        T* os = out + s * n;
+        for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < n; e += int64_t(gridDim.x) * blockDim.x)
+            os[e] = placed_response(rs, e, n, m);
     }
 }
 
-// a <- inverse-untangle(op(a, b) / no2) on forward-untangled packed spectra.
+// a <- inverse-untangle(op(a, b) / no2) on forward-untangled packed spectra:
+// bin pairs (p, L - p), p < L/2, PAIR_U per thread with every load first;
+// the thread of p = 0 also takes the packed DC / Nyquist slot and the
+// self-paired bin L/2.
 template <typename T>
-__global__ void k_spec(cpx<T>* a, const cpx<T>* b, int64_t L, int64_t nsig, int op, int logn, Twiddles<T> tw) {
-    const int64_t np = L / 2;
-    const int64_t total = np * nsig;
+__global__ __launch_bounds__(PAIR_NT) void k_spec(cpx<T>* a, const cpx<T>* b, int64_t L64, int64_t nsig, int op,
+                                                  int logn, Twiddles<T> tw) {
+    const int L = int(L64), np = L / 2;
     const T inv_no2 = (T)1 / (T)L;
-    for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < total; q += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t s = q / np, p = q % np;
-        cpx<T>* x = a + s * L;
-        const cpx<T>* y = b + s * L;
-        if (p == 0) {
-            cpx<T> y0{spec_apply_real(op, x[0].x, y[0].x, inv_no2), spec_apply_real(op, x[0].y, y[0].y, inv_no2)};
-            x[0] = dc_inverse(y0);
-            x[np] = spec_apply(op, x[np], y[np], inv_no2);
-        } else {
-            const cpx<T> w = twiddle_big<1>(tw, (uint32_t)p, logn);
-            const cpx<T> ya = spec_apply(op, x[p], y[p], inv_no2);
-            const cpx<T> yb = spec_apply(op, x[L - p], y[L - p], inv_no2);
-            cpx<T> o1, o2;
-            untangle_pair(ya, yb, conj(w), (T)0.5, o1, o2);
-            x[p] = o1;
-            x[L - p] = o2;
+    const RowTw<1, T> wr(tw, logn);
+    for (int64_t s = blockIdx.y; s < nsig; s += gridDim.y) {
+        cpx<T>* x = a + s * L64;
+        const cpx<T>* y = b + s * L64;
+        for (int p0 = pair_first(); p0 < np; p0 += pair_stride()) {
+            cpx<T> xa[PAIR_U], xb[PAIR_U], ya[PAIR_U], yb[PAIR_U], w[PAIR_U];
+#pragma unroll
+            for (int u = 0; u < PAIR_U; ++u) {
+                const int p = pair_at(p0, u);
+                if (p < np) {
+                    xa[u] = x[p];
+                    xb[u] = x[(L - p) & (L - 1)];
+                    ya[u] = y[p];
+                    yb[u] = y[(L - p) & (L - 1)];
+                    w[u] = wr.at(tw, p);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < PAIR_U; ++u) {
+                const int p = pair_at(p0, u);
+                if (p >= np) continue;
+                if (p == 0) {
+                    const cpx<T> y0{spec_apply_real(op, xa[u].x, ya[u].x, inv_no2),
+                                    spec_apply_real(op, xa[u].y, ya[u].y, inv_no2)};
+                    x[0] = dc_inverse(y0);
+                    x[np] = spec_apply(op, x[np], y[np], inv_no2);
+                } else {
+                    const cpx<T> sa = spec_apply(op, xa[u], ya[u], inv_no2);
+                    const cpx<T> sb = spec_apply(op, xb[u], yb[u], inv_no2);
+                    cpx<T> o1, o2;
+                    untangle_pair(sa, sb, conj(w[u]), (T)0.5, o1, o2);
+                    x[p] = o1;
+                    x[L - p] = o2;
+                }
+            }
         }
     }
 }
@@ -334,14 +358,14 @@ static int exec_real_t(const Plan& p, const Launch& l, void* const* bufs, int is
             return NRFFT_OK;
         }
         case RS_PLACE:
-            hipLaunchKernelGGL((k_place<T>), dim3(grid_for(a[0] * a[1], 256)), dim3(256), 0, stream,
-                               (const T*)src, (T*)dst, a[1], a[2], a[0]);
+            hipLaunchKernelGGL((k_place<T>), grid_pairs(a[1], a[0]), dim3(PAIR_NT), 0, stream, (const T*)src, (T*)dst,
+                               a[1], a[2], a[0]);
             HIPCHK(hipGetLastError());
             return NRFFT_OK;
         case RS_SPEC: {
             int op = p.kind == PK_CORREL ? OP_CORREL : (isign > 0 ? OP_CONV : OP_DECONV);
             const int logn = ilog2((size_t)a[1]) + 1;
-            hipLaunchKernelGGL((k_spec<T>), dim3(grid_for(a[0] * a[1] / 2, 256)), dim3(256), 0, stream, (cpx<T>*)dst,
+            hipLaunchKernelGGL((k_spec<T>), grid_pairs(a[1] / 2, a[0]), dim3(PAIR_NT), 0, stream, (cpx<T>*)dst,
                                (const cpx<T>*)bufs[B_TMP2], a[1], a[0], op, logn, tw);
             HIPCHK(hipGetLastError());
             return NRFFT_OK;

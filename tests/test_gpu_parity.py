@@ -196,6 +196,27 @@ def test_c2c_f64_default_tuning(logn, batch):
         assert rel_l2(got[2 * i * n:2 * (i + 1) * n], z.view(np.float64)) < TOL64, (logn, i)
 
 
+@pytest.mark.parametrize("n,b,isign", [(1 << 16, 6, 1), (1 << 16, 6, -1), (1 << 21, 20, 1)])
+def test_convlv_batch_f32_multipass(n, b, isign):
+    # FP32 convlv beyond one group: response placement, forward C2Cs (in
+    # 16-transform chunks at n = 2^21) + untangles, the spectrum operation
+    # fused with the inverse untangle, inverse C2C; first and last signal
+    # against the oracle.
+    m = 257
+    d = f32round(synth.normal(n + 5, n * b)).reshape(b, n)
+    if isign == 1:
+        x = np.arange(m) - (m - 1) / 2
+        r = np.stack([f32round(np.exp(-0.5 * (x / 40.0) ** 2) * (1 + 0.1 * synth.normal(60 + i, m)))
+                      for i in range(b)])
+    else:
+        r = np.stack([f32round(0.3 / m * synth.normal(60 + i, m)) for i in range(b)])
+        r[:, (m - 1) // 2] = 1.0
+    y = numrs_amd.convlv_batch_responses(d.astype(np.float32), r.astype(np.float32), isign)
+    for i in (0, b - 1):
+        ref = oracle.convlv(d[i], r[i], isign)
+        assert rel_l2(y[i], ref) < 1e-5, (n, isign, i)
+
+
 def test_realft_plan_multipass_chunked_f32():
     # 36 FP32 signals of 2^21 reals: the 2^20-point C2C runs in chunks of 16
     # transforms (16, 16, 4), then the untangle pass.  Signals of the first,
