@@ -77,15 +77,16 @@ Tune default_tune(const Plan& p) {
             const size_t bytes = p.nn[0] * p.batch * (p.dtype == NRFFT_F32 ? 8 : 16);
             const size_t per = p.nn[0] * (p.dtype == NRFFT_F32 ? 8 : 16);
             if (logn >= 16) {
-                // Measured on MI355X (512 MiB FP32 / 256 MiB FP64 batches, pass B
+                // Measured on MI355X (256-512 MiB batches, pass B
                 // with non-temporal result stores): 2048-point rows in pass B
                 // from 2^19 (the column pass stays at most 2048 points); below,
                 // 512-point rows (FP32) or columns of 256-512 points (FP64);
                 // chunks of 64 MiB (short transforms) or 128 MiB whenever the
                 // batch is larger.  Against the rule below: FP32 2^17 0.237 ->
-                // 0.210 ms, 2^19 0.260 -> 0.213, 2^21 0.913 -> 0.495, 2^22 0.938
-                // -> 0.626; FP64 2^18 0.221 -> 0.198, 2^20 0.259 -> 0.211, 2^21
-                // 0.529 -> 0.229, 2^22 0.642 -> 0.313.
+                // 0.210 ms and 2^19 0.260 -> 0.213 (256 MiB), 2^21 0.913 -> 0.495
+                // and 2^22 0.938 -> 0.626 (512 MiB); FP64 (256 MiB) 2^18 0.221 ->
+                // 0.198, 2^20 0.259 -> 0.211, 2^21 0.529 -> 0.229, 2^22 0.642 ->
+                // 0.313.
                 const bool f32 = p.dtype == NRFFT_F32;
                 if (logn >= 19)
                     t.split_log1 = std::max(logn - C2C_MAX_LOGL_, std::min(logn - 11, 11));
