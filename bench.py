@@ -191,6 +191,11 @@ def headline(args) -> dict:
         "hbm_effective": {"GBps": round(compulsory / (ms * 1e-3) / 1e9, 1),
                           "frac": round(compulsory / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                           "note": "compulsory bytes (one read + one write of the batch) / whole step"},
+        "traffic_rate": ({"GBps": round(ratio * launch_bytes * nl / (ms * 1e-3) / 1e9, 1),
+                          "frac": round(ratio * launch_bytes * nl / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                          "note": "counter-measured bytes past L2 per step (rocprofv3 FETCH_SIZE + WRITE_SIZE of "
+                                  "every launch, from traffic_source) / whole step; each of the two passes "
+                                  "reads and writes every point"} if ratio else None),
         "launch_ms": [round(p, 4) for p in per],
     }
     # FP64 reference of the same FP32 inputs from the FP64 path (parity-tested
