@@ -130,6 +130,36 @@ def test_four1_batch_host_pipelined(dtype, logn, batch):
         assert rel_l2(got, ref) < (TOL32 if dtype == np.float32 else TOL64), (i, chunk)
 
 
+def test_four1_batch_host_pipelined_threads():
+    # Host-pointer batch calls from two threads at once (ctypes releases the
+    # GIL): both stream through the library's shared pinned staging and copy
+    # pool, which serialise the pipelined calls; each result against the oracle.
+    import ctypes
+    import threading
+
+    from numrs_amd import _lib
+
+    n, batch = 1 << 16, 160
+    xs = [f32round(synth.uniform(600 + k, 2 * n * batch)) for k in range(2)]
+    ys = [x.astype(np.float32) for x in xs]
+    rcs = [None, None]
+
+    def run(k):
+        ptr = ys[k].ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        rcs[k] = _lib.lib().nrfft_four1_batch_f32(ptr, n, batch, 1 if k == 0 else -1)
+
+    ts = [threading.Thread(target=run, args=(k,)) for k in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert rcs == [0, 0], _lib.last_error()
+    for k in range(2):
+        for i in (0, 63, 64, batch - 1):
+            ref = oracle.four1_nr(xs[k][i * 2 * n:(i + 1) * 2 * n], 1 if k == 0 else -1)
+            assert rel_l2(ys[k][i * 2 * n:(i + 1) * 2 * n].astype(np.float64), ref) < TOL32, (k, i)
+
+
 # ---- realft ------------------------------------------------------------------
 @pytest.mark.parametrize("logn", list(range(1, 16)) + [17, 19])
 def test_realft_f64(logn):
