@@ -1,7 +1,7 @@
 """GPU parity: the HIP kernels (through the C ABI) against the oracle.
 
 Tolerances (relative L2, stated per test):
-  FP64 path vs the oracle: 1e-12 (north-star bar for four1).
+  FP64 path vs the oracle: 1e-12 (the required bar for four1).
   FP32 path vs the FP64 oracle on the FP32-rounded inputs: 2e-6 for the
   C2C transforms (radix-16 Stockham, ~1e-7 per stage), 1e-5 for realft
   packed spectra and convolutions of N(0,1) data.
