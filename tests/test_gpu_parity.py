@@ -247,6 +247,28 @@ def test_convlv_batch_f32_multipass(n, b, isign):
         assert rel_l2(y[i], ref) < 1e-5, (n, isign, i)
 
 
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_c2c_max_length(dtype):
+    # The longest 1-D transform the planner takes, N = 2^26 (two passes of
+    # 8192 points): against numpy's FP64 FFT, then the round trip (x N).
+    import torch
+
+    n = 1 << 26
+    x = synth.uniform(26, 2 * n)
+    if dtype == "float32":
+        x = f32round(x)
+    t = torch.from_numpy(x.astype(np.float32 if dtype == "float32" else np.float64)).cuda()
+    o = torch.empty_like(t)
+    plan = numrs_amd.C2CPlan(dtype, [n], 1)
+    plan.execute(t, o, -1)
+    tol = TOL32 if dtype == "float32" else TOL64
+    ref = np.fft.fft(x.view(np.complex128))
+    assert rel_l2(o.cpu().numpy().astype(np.float64), ref.view(np.float64)) < tol
+    del ref
+    plan.execute(o, t, 1)
+    assert rel_l2(t.cpu().numpy().astype(np.float64), n * x) < 2 * tol
+
+
 def test_realft_plan_multipass_chunked_f32():
     # 36 FP32 signals of 2^21 reals: the 2^20-point C2C runs in chunks of 16
     # transforms (16, 16, 4), then the untangle pass.  Signals of the first,
