@@ -71,34 +71,36 @@ Tune default_tune(const Plan& p) {
     if (p.kind == PK_C2C && p.nn.size() == 1) {
         const int logn = ilog2(p.nn[0]);
         if (logn > C2C_MAX_LOGL_) {
-            t.split_log1 = std::max(logn - C2C_MAX_LOGL_, logn / 2 - 1);
-            t.tilog_a = 4;
-            t.tilog_b = 4;
-            const size_t bytes = p.nn[0] * p.batch * (p.dtype == NRFFT_F32 ? 8 : 16);
-            const size_t per = p.nn[0] * (p.dtype == NRFFT_F32 ? 8 : 16);
-            if (logn >= 16) {
-                // Measured on MI355X (256-512 MiB batches, pass B
-                // with non-temporal result stores): 2048-point rows in pass B
-                // from 2^19 (the column pass stays at most 2048 points); below,
-                // 512-point rows (FP32) or columns of 256-512 points (FP64);
-                // chunks of 64 MiB (short transforms) or 128 MiB whenever the
-                // batch is larger.  Against the rule below: FP32 2^17 0.237 ->
-                // 0.210 ms and 2^19 0.260 -> 0.213 (256 MiB), 2^21 0.913 -> 0.495
-                // and 2^22 0.938 -> 0.626 (512 MiB); FP64 (256 MiB) 2^18 0.221 ->
-                // 0.198, 2^20 0.259 -> 0.211, 2^21 0.529 -> 0.229, 2^22 0.642 ->
-                // 0.313.
-                const bool f32 = p.dtype == NRFFT_F32;
-                if (logn >= 19)
-                    t.split_log1 = std::max(logn - C2C_MAX_LOGL_, std::min(logn - 11, 11));
-                else
-                    t.split_log1 = f32 ? logn - 9 : std::min(9, logn - 8);
-                const size_t target = ((f32 ? logn <= 18 : logn <= 17) ? size_t(64) : size_t(128)) << 20;
-                const size_t chunk = std::max<size_t>(1, target / per);
-                if (chunk < p.batch) t.chunk = (int)chunk;
-            } else if (bytes > (size_t(256) << 20)) {
-                // Chunks of about 128 MB keep a chunk's intermediate in the MALL.
-                t.chunk = (int)std::max<size_t>(1, (size_t(128) << 20) / per);
-            }
+            // Four-step split and batch chunking per length, from a sweep on
+            // MI355X (256-512 MiB batches, pass B with non-temporal result
+            // stores).  From 2^19, 2048-point pass-B rows (the column pass at
+            // most 2048 points); below, the best split measured per length.
+            // Chunks of 32 MiB (2^14), 64 MiB (short transforms) or 128 MiB
+            // whenever the batch is larger, so a chunk's intermediate stays in
+            // the MALL.  Against the earlier rule (split near sqrt N, chunks
+            // only above 256 MiB): FP32 2^15 0.215 -> 0.203 ms, 2^17 0.237 ->
+            // 0.210, 2^19 0.260 -> 0.213 (256 MiB), 2^21 0.913 -> 0.495, 2^22
+            // 0.938 -> 0.626 (512 MiB); FP64 (256 MiB) 2^14 0.218 -> 0.206,
+            // 2^20 0.259 -> 0.211, 2^21 0.529 -> 0.229, 2^22 0.642 -> 0.313.
+            static const int short_f32[5] = {7, 8, 7, 8, 9}, short_f64[5] = {7, 8, 8, 9, 9};  // 2^14..2^18
+            const bool f32 = p.dtype == NRFFT_F32;
+            t.split_log1 = logn >= 19 ? std::max(logn - C2C_MAX_LOGL_, std::min(logn - 11, 11))
+                                      : (f32 ? short_f32 : short_f64)[logn - 14];
+            // A split-precision plan needs a pass of 2^8..2^10 points for its
+            // MFMA stages; if neither pass has one, the split near sqrt N.
+            auto mfma_len = [](int l) { return l >= 8 && l <= 10; };
+            if (p.extra != 0 && !mfma_len(t.split_log1) && !mfma_len(logn - t.split_log1))
+                t.split_log1 = std::max(logn - C2C_MAX_LOGL_, logn / 2 - 1);
+            // Tile widths (log2 transforms per tile; capped by LDS per length):
+            // 16 for FP32; FP64 8-wide pass-B tiles and 8-wide pass-A tiles from
+            // 512-point columns (FP64 2^14 0.222 -> 0.205 ms, 2^17 0.226 ->
+            // 0.215, 2^20 0.224 -> 0.209 against 16-wide).
+            t.tilog_a = f32 || t.split_log1 < 9 ? 4 : 3;
+            t.tilog_b = f32 ? 4 : 3;
+            const size_t per = p.nn[0] * (f32 ? 8 : 16);
+            const size_t mib = logn == 14 ? 32 : (f32 ? logn <= 18 : logn <= 17) ? 64 : 128;
+            const size_t chunk = std::max<size_t>(1, (mib << 20) / per);
+            if (chunk < p.batch) t.chunk = (int)chunk;
         }
     } else if (p.kind == PK_C2C) {
         t.tilog_a = 4;

@@ -30,7 +30,7 @@ def _run(x32: np.ndarray, n: int, batch: int, isign: int, split):
 
 
 @pytest.mark.parametrize("split", ["f16", "bf16", None])
-@pytest.mark.parametrize("logn,batch", [(8, 3), (9, 2), (10, 17), (16, 2), (18, 2), (20, 2)])
+@pytest.mark.parametrize("logn,batch", [(8, 3), (9, 2), (10, 17), (14, 4), (16, 2), (18, 2), (20, 2)])
 def test_split_c2c_accuracy(split, logn, batch):
     n = 1 << logn
     x32 = synth.uniform(3 + logn, 2 * n * batch).astype(np.float32)
