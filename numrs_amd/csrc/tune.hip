@@ -97,6 +97,10 @@ Tune default_tune(const Plan& p) {
             // 0.215, 2^20 0.224 -> 0.209 against 16-wide).
             t.tilog_a = f32 || t.split_log1 < 9 ? 4 : 3;
             t.tilog_b = f32 ? 4 : 3;
+            // A small problem (a few transforms) is launch-latency bound: 4-wide
+            // tiles spread each pass over more CUs (one FP64 2^16 transform:
+            // 10.6 -> 9.8 us).
+            if (p.batch * p.nn[0] <= (size_t(1) << 18)) t.tilog_a = t.tilog_b = 2;
             const size_t per = p.nn[0] * (f32 ? 8 : 16);
             const size_t mib = logn == 14 ? 32 : (f32 ? logn <= 18 : logn <= 17) ? 64 : 128;
             const size_t chunk = std::max<size_t>(1, (mib << 20) / per);
