@@ -176,3 +176,17 @@ def test_fourfs_validation(tmp_path):
         fs.fourfs([8], 1, 0)
     with pytest.raises(numrs_amd.FournError, match="Invalid dimensions"):
         fs.fourfs([8], 2, 1)
+
+
+def test_missing_library_fails_loudly():
+    # No CPU fallback: with the HIP library absent, a call raises instead of
+    # computing anything.
+    import os
+    import sys
+
+    code = ("import numrs_amd._lib as L; L.LIB_PATH = '/nonexistent/libnrfft.so'; "
+            "import numpy as np, numrs_amd; numrs_amd.four1(np.zeros(16), 8, 1)")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300,
+                         cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    assert out.returncode != 0
+    assert "is missing" in out.stderr
