@@ -321,6 +321,22 @@ def test_fourn_f32(shape):
     assert rel_l2(y, oracle.fourn(x, list(shape), 1)) < TOL32
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(4096, 64), (8192, 32), (2, 4096, 16), (4096, 4), (8192, 2, 8)])
+def test_fourn_long_strided_axis(dtype, shape):
+    # Strided axes of 4096 / 8192 points run as two column passes (64 x 64,
+    # 64 x 128) with the W_L^{k1 l2} twiddle between them; wide, narrow and
+    # batched (leading axis) layouts, both signs.
+    x = synth.normal(len(shape) + shape[0], 2 * int(np.prod(shape)))
+    if dtype == np.float32:
+        x = f32round(x)
+    for isign in (1, -1):
+        y = x.astype(dtype)
+        numrs_amd.fourn(y, list(shape), len(shape), isign)
+        tol = TOL32 if dtype == np.float32 else TOL64
+        assert rel_l2(y.astype(np.float64), oracle.fourn(x, list(shape), isign)) < tol, (shape, isign)
+
+
 @pytest.mark.parametrize("shape", [(512, 64), (256, 8), (2, 256, 32)])
 def test_fourn_f32_column_pass_tiles(shape):
     # Strided passes of 256 / 512 points: 16-wide tiles take the
