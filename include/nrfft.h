@@ -73,6 +73,12 @@ int nrfft_four1_batch_f32(float* data, size_t nn, size_t batch, int isign);
 int nrfft_realft_f64(double* data, size_t n, int isign);
 int nrfft_realft_f32(float* data, size_t n, int isign);
 
+/* RealFTProcessor::process_batch (Real_FT.rs:365-369) for `count` host
+ * buffers of n reals each (equal lengths, one isign), transformed in place by
+ * one batched execution.  Same semantics and checks as nrfft_realft_*. */
+int nrfft_realft_batch_f64(double* const* data, size_t n, size_t count, int isign);
+int nrfft_realft_batch_f32(float* const* data, size_t n, size_t count, int isign);
+
 /* fourn: the in-memory NR fourn that Real_FT3.rs:35 calls as
  * Fourn(&mut data, &nn, ndim, isign) (no such function exists in the
  * reference; Fourn.rs:56 is a data-less stub).  C order: nn[0] slowest.
@@ -241,11 +247,19 @@ int nrfft_gen_uniform_f64(void* d, size_t n, uint64_t seed, uint64_t start, void
  * (shape, batch, dtype).  best_tune receives 6 values.
  * nrfft_autotune_c2c times every candidate on the current device with HIP
  * events (iters executions each) and caches the fastest; plans created
- * afterwards for that key use it.  -1 leaves an entry at the planner default. */
+ * afterwards for that key use it.  -1 leaves an entry at the planner default.
+ * nrfft_plan_c2c_tuned builds one plan with the given tunables (ntune <= 7;
+ * tune[6], if given, is 0 for the plain form or NRFFT_SPLIT_F16 /
+ * NRFFT_SPLIT_BF16 for the split-precision form of an FP32 plan); it neither
+ * reads nor changes the cache.  Split-precision plans never use cached
+ * tunings (their split must leave a pass of 2^8..2^10 points). */
 int nrfft_plan_c2c_tuned(nrfft_plan* plan, int dtype, const size_t* nn, size_t ndim, size_t batch, const int* tune,
                          size_t ntune);
 int nrfft_autotune_c2c(int dtype, const size_t* nn, size_t ndim, size_t batch, int iters, int* best_tune,
                        double* best_ms);
+/* The tunables a plan was built with (up to 6 values, order as above; -1 =
+ * planner default). */
+int nrfft_plan_tune(nrfft_plan plan, int* tune, size_t ntune);
 /* Text cache file; also read at the first plan creation from $NRFFT_TUNE_CACHE. */
 int nrfft_tune_cache_save(const char* path);
 int nrfft_tune_cache_load(const char* path);

@@ -42,6 +42,7 @@ from .api import (  # noqa: F401
     real_to_complex,
     realft,
     realft_optimized,
+    realft_sequential,
     rlft3,
     rlft3_optimized,
     twofft,

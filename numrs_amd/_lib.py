@@ -50,6 +50,8 @@ _SIGS = {
     "nrfft_four1_batch_f32": (_i, [_fp, _sz, _sz, _i]),
     "nrfft_realft_f64": (_i, [_dp, _sz, _i]),
     "nrfft_realft_f32": (_i, [_fp, _sz, _i]),
+    "nrfft_realft_batch_f64": (_i, [ctypes.POINTER(_dp), _sz, _sz, _i]),
+    "nrfft_realft_batch_f32": (_i, [ctypes.POINTER(_fp), _sz, _sz, _i]),
     "nrfft_fourn_f64": (_i, [_dp, _szp, _sz, _i]),
     "nrfft_fourn_f32": (_i, [_fp, _szp, _sz, _i]),
     "nrfft_convlv_f64": (_i, [_dp, _sz, _dp, _sz, _i, _dp]),
@@ -88,6 +90,7 @@ _SIGS = {
     "nrfft_execute_c2c_launches": (_i, [_vp, _vp, _vp, _i, _vp, _i, _i]),
     "nrfft_plan_c2c_tuned": (_i, [ctypes.POINTER(_vp), _i, _szp, _sz, _sz, ctypes.POINTER(_i), _sz]),
     "nrfft_autotune_c2c": (_i, [_i, _szp, _sz, _sz, _i, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_double)]),
+    "nrfft_plan_tune": (_i, [_vp, ctypes.POINTER(_i), _sz]),
     "nrfft_tune_cache_save": (_i, [ctypes.c_char_p]),
     "nrfft_tune_cache_load": (_i, [ctypes.c_char_p]),
     "nrfft_plan_uneven": (_i, [ctypes.POINTER(_vp), _i, _sz, ctypes.POINTER(_i), _szp, _szp]),

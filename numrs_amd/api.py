@@ -203,10 +203,17 @@ def realft(data: np.ndarray, n: int, isign: int) -> None:
 
 
 realft_optimized = realft  # Real_FT.rs:216
+# Real_FT.rs:373 -- the same transform (sequential recurrence on the CPU).
+realft_sequential = realft
 
 
 class RealFTProcessor:
-    """Real_FT.rs:331-370."""
+    """Real_FT.rs:331-370.  ``process`` is one realft (the three Rust variants
+    it chooses between compute the same transform); ``process_batch`` takes
+    (data, n, isign) items like the Rust ``&mut [(&mut [f64], usize, i32)]``
+    and runs every group of equal (n, isign, dtype) as one batched GPU
+    execution (nrfft_realft_batch_*), where the Rust spreads items over rayon
+    threads (Real_FT.rs:365-369)."""
 
     def __init__(self) -> None:
         self.use_optimized = True
@@ -228,8 +235,22 @@ class RealFTProcessor:
         realft(data, n, isign)
 
     def process_batch(self, batches: Iterable[tuple]) -> None:
-        for data, n, isign in batches:
-            realft(data, n, isign)
+        items = list(batches)
+        groups: dict = {}
+        for data, n, isign in items:
+            n = int(n)
+            assert n % 2 == 0, "n must be even"                      # Real_FT.rs:5
+            assert data.size >= n, "data length must be at least n"  # Real_FT.rs:6
+            v = _inplace_view(data, n)
+            if v.dtype not in (np.float32, np.float64):
+                raise TypeError(f"unsupported dtype {v.dtype} (float32 or float64)")
+            groups.setdefault((n, 1 if isign == 1 else -1, v.dtype.str), []).append(v)
+        for (n, isign, _), views in groups.items():
+            f32 = views[0].dtype == np.float32
+            ct = ctypes.c_float if f32 else ctypes.c_double
+            ptrs = (ctypes.POINTER(ct) * len(views))(*[v.ctypes.data_as(ctypes.POINTER(ct)) for v in views])
+            fn = _lib.lib().nrfft_realft_batch_f32 if f32 else _lib.lib().nrfft_realft_batch_f64
+            _check(fn(ptrs, n, len(views), isign))
 
 
 # ---------------------------------------------------------------------------

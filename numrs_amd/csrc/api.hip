@@ -290,6 +290,15 @@ int plan_c2c_1d(Plan& p, int logn, int64_t batch, int src, int dst, size_t* tmp_
     return NRFFT_OK;
 }
 
+// Strided axes of >= 4096 points take the two-pass four-step along the axis
+// when at least this many columns sit side by side (inner), so pass B gets
+// 16-wide tiles; narrower axes run as one column pass.
+#ifndef NRFFT_STRIDED_4STEP_MIN_INNER
+#define NRFFT_STRIDED_4STEP_MIN_INNER 16
+#endif
+constexpr int64_t STRIDED_4STEP_MIN_INNER = NRFFT_STRIDED_4STEP_MIN_INNER;
+constexpr int64_t STRIDED_4STEP_CHUNK_BYTES = int64_t(128) << 20;
+
 // n-D (C order) batched C2C; axes processed fastest first.
 int plan_c2c_nd(Plan& p, const std::vector<size_t>& nn, int64_t batch) {
     size_t tmp = 0;
@@ -327,7 +336,7 @@ int plan_c2c_nd(Plan& p, const std::vector<size_t>& nn, int64_t batch) {
             d.nb2 = 1;
             int rc = plan_add_c2c_pass(p, MODE_JJ, logl, 0, d, src, B_OUT);
             if (rc) return rc;
-        } else if (logl >= 12) {
+        } else if (logl >= 12 && inner >= STRIDED_4STEP_MIN_INNER && !p.strided_one_pass) {
             // A long strided axis as a four-step along the axis, L = L1 L2:
             // pass A transforms over l1 (element stride L2 inner) and applies
             // W_L^{k1 l2}, pass B over l2 (stride inner) into natural order
@@ -336,42 +345,53 @@ int plan_c2c_nd(Plan& p, const std::vector<size_t>& nn, int64_t batch) {
             // FP32 4096 x 4096 column pass 1.9 TB/s, 8192 x 2048 0.9 TB/s).
             // The pass-A tile index is b0 = l2 inner + column, so the kernel's
             // four-step twiddle W^{k1 (b0 >> log2 inner)} is W_L^{k1 l2}.
+            // The outer slabs run in chunks (pass A then pass B per chunk) so
+            // the workspace stays near STRIDED_4STEP_CHUNK_BYTES (one slab at
+            // least) and a chunk's intermediate can stay in the MALL.
             if (L * inner > 0x7fffffffLL) return fail(NRFFT_E_UNSUPPORTED, "axis span too large");
             const int l1 = logl / 2, l2 = logl - l1;
             const int64_t L1 = int64_t(1) << l1, L2 = int64_t(1) << l2;
             const int want = p.tune.tilog_a >= 0 ? p.tune.tilog_a : col_tilog(p.dtype);
-            PassDesc a{};
-            a.nb0 = L2 * inner;
-            a.si0 = 1;
-            a.so0 = 1;
-            a.esi = (int32_t)(L2 * inner);
-            a.eso = (int32_t)(L2 * inner);
-            a.nb1 = outer;
-            a.si1 = L * inner;
-            a.so1 = L * inner;
-            a.nb2 = 1;
-            a.tw_logn = logl;
-            a.tw_b0_shift = ilog2((size_t)inner);
-            const int ta = pick_tilog(p.dtype, MODE_TT, l1, a.nb0, want);
-            if (ta < 0) return fail(NRFFT_E_UNSUPPORTED, "no column kernel");
-            int rc = plan_add_c2c_pass(p, MODE_TT, l1, ta, a, src, B_TMP);
-            if (rc) return rc;
-            PassDesc b{};
-            b.nb0 = inner;
-            b.si0 = 1;
-            b.so0 = 1;
-            b.esi = (int32_t)inner;
-            b.eso = (int32_t)(L1 * inner);
-            b.nb1 = L1;
-            b.si1 = L2 * inner;
-            b.so1 = inner;
-            b.nb2 = outer;
-            b.si2 = L * inner;
-            b.so2 = L * inner;
-            const int tb = pick_tilog(p.dtype, MODE_TT, l2, inner, want);
-            if (tb < 0) return fail(NRFFT_E_UNSUPPORTED, "no column kernel");
-            if ((rc = plan_add_c2c_pass(p, MODE_TT, l2, tb, b, B_TMP, B_OUT))) return rc;
-            tmp = std::max<size_t>(tmp, size_t(outer) * size_t(L) * size_t(inner));
+            const int64_t slab = L * inner;
+            const int64_t chunk =
+                std::max<int64_t>(1, std::min<int64_t>(outer, STRIDED_4STEP_CHUNK_BYTES / (slab * (int64_t)csize(p.dtype))));
+            for (int64_t c0 = 0; c0 < outer; c0 += chunk) {
+                const int64_t cb = std::min<int64_t>(chunk, outer - c0);
+                PassDesc a{};
+                a.nb0 = L2 * inner;
+                a.si0 = 1;
+                a.so0 = 1;
+                a.esi = (int32_t)(L2 * inner);
+                a.eso = (int32_t)(L2 * inner);
+                a.nb1 = cb;
+                a.si1 = slab;
+                a.so1 = slab;
+                a.nb2 = 1;
+                a.tw_logn = logl;
+                a.tw_b0_shift = ilog2((size_t)inner);
+                const int ta = pick_tilog(p.dtype, MODE_TT, l1, a.nb0, want);
+                if (ta < 0) return fail(NRFFT_E_UNSUPPORTED, "no column kernel");
+                int rc = plan_add_c2c_pass(p, MODE_TT, l1, ta, a, src, B_TMP);
+                if (rc) return rc;
+                p.launches.back().off_src = c0 * slab;
+                PassDesc b{};
+                b.nb0 = inner;
+                b.si0 = 1;
+                b.so0 = 1;
+                b.esi = (int32_t)inner;
+                b.eso = (int32_t)(L1 * inner);
+                b.nb1 = L1;
+                b.si1 = L2 * inner;
+                b.so1 = inner;
+                b.nb2 = cb;
+                b.si2 = slab;
+                b.so2 = slab;
+                const int tb = pick_tilog(p.dtype, MODE_TT, l2, inner, want);
+                if (tb < 0) return fail(NRFFT_E_UNSUPPORTED, "no column kernel");
+                if ((rc = plan_add_c2c_pass(p, MODE_TT, l2, tb, b, B_TMP, B_OUT))) return rc;
+                p.launches.back().off_dst = c0 * slab;
+            }
+            tmp = std::max<size_t>(tmp, size_t(chunk) * size_t(slab));
         } else {
             if (L * inner > 0x7fffffffLL) return fail(NRFFT_E_UNSUPPORTED, "axis span too large");
             d.nb0 = inner;
@@ -523,7 +543,8 @@ int host_c2c(int dtype, void* data, const std::vector<size_t>& nn, size_t batch,
     return NRFFT_OK;
 }
 
-int make_plan(nrfft_plan* plan, int kind, int dtype, std::vector<size_t> nn, size_t batch, size_t extra) {
+int make_plan(nrfft_plan* plan, int kind, int dtype, std::vector<size_t> nn, size_t batch, size_t extra,
+              const Tune* explicit_tune) {
     if (!plan) return fail(NRFFT_E_INVALID, "null plan pointer");
     *plan = nullptr;
     if (dtype != NRFFT_F32 && dtype != NRFFT_F64) return fail(NRFFT_E_INVALID, "dtype");
@@ -542,13 +563,34 @@ int make_plan(nrfft_plan* plan, int kind, int dtype, std::vector<size_t> nn, siz
     p->plan.batch = batch;
     p->plan.extra = extra;
     p->plan.device = dev;
-    tune_lookup(p->plan);
+    if (explicit_tune) p->plan.tune = *explicit_tune;
+    else tune_lookup(p->plan);
     int rc = build_plan(p->plan);
     if (!rc && p->plan.tmp_elems) {
-        const size_t bytes = p->plan.tmp_elems * csize(dtype);
+        size_t bytes = p->plan.tmp_elems * csize(dtype);
         e = hipMalloc(&p->tmp, bytes);
-        if (e != hipSuccess) rc = fail(NRFFT_E_NOMEM, std::string("workspace: ") + hipGetErrorString(e));
-        else p->tmp_bytes = bytes;
+        if (e != hipSuccess && kind == PK_C2C && p->plan.nn.size() > 1 && !p->plan.strided_one_pass) {
+            // The four-step of a long strided axis needs a workspace; without
+            // one, such axes run as single column passes.
+            (void)hipGetLastError();
+            Plan q = p->plan;
+            q.launches.clear();
+            q.tmp_elems = 0;
+            q.strided_one_pass = true;
+            rc = build_plan(q);
+            if (!rc) {
+                p->plan = std::move(q);
+                bytes = p->plan.tmp_elems * csize(dtype);
+                e = bytes ? hipMalloc(&p->tmp, bytes) : hipSuccess;
+            }
+        }
+        if (rc) {
+        } else if (e != hipSuccess) {
+            (void)hipGetLastError();
+            rc = fail(NRFFT_E_NOMEM, std::string("workspace: ") + hipGetErrorString(e));
+        } else {
+            p->tmp_bytes = bytes;
+        }
     }
     if (rc) {
         delete p;

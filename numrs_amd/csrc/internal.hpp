@@ -113,6 +113,10 @@ struct Plan {
     std::vector<Launch> launches;
     size_t tmp_elems = 0;    // complex elements of workspace
     size_t tmp2_offset = 0;  // B_TMP2 = workspace + tmp2_offset complex elements
+    // Strided n-D axes of >= 4096 points run as one column pass (no
+    // workspace) instead of the two-pass four-step along the axis: set when
+    // the four-step's workspace could not be allocated.
+    bool strided_one_pass = false;
 };
 
 extern thread_local std::string g_err;
@@ -180,5 +184,8 @@ struct nrfft_plan_s {
 };
 
 namespace nrfft {
-int make_plan(nrfft_plan_s** plan, int kind, int dtype, std::vector<size_t> nn, size_t batch, size_t extra);
+// explicit_tune: tunables for this plan only (the tune cache is neither read
+// nor written); nullptr = the cached tuning for the key, else the defaults.
+int make_plan(nrfft_plan_s** plan, int kind, int dtype, std::vector<size_t> nn, size_t batch, size_t extra,
+              const Tune* explicit_tune = nullptr);
 }

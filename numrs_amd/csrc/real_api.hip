@@ -2,6 +2,7 @@
 // execution and the extern "C" entry points of include/nrfft.h.
 #include <cstring>
 #include <memory>
+#include <new>
 #include <utility>
 #include <vector>
 
@@ -467,6 +468,42 @@ int nrfft_realft_f32(float* data, size_t n, int isign) {
     g_err.clear();
     int rc = check_realft(data, n, isign);
     return rc ? rc : host_real(PK_REALFT, NRFFT_F32, data, nullptr, 0, data, n, 0, 1, isign);
+}
+
+}  // extern "C"
+
+namespace nrfft {
+// RealFTProcessor::process_batch (Real_FT.rs:365-369) for `count` host
+// buffers of n reals each: gathered into one contiguous batch, transformed
+// by one batched plan execution, scattered back.
+template <typename T>
+static int realft_batch(T* const* data, size_t n, size_t count, int isign, int dtype) {
+    g_err.clear();
+    if (count == 0) return NRFFT_OK;
+    if (!data) return fail(NRFFT_E_INVALID, "null buffer list");
+    for (size_t i = 0; i < count; ++i)
+        if (int rc = check_realft(data[i], n, isign)) return rc;
+    if (count == 1) return host_real(PK_REALFT, dtype, data[0], nullptr, 0, data[0], n, 0, 1, isign);
+    std::vector<T> buf;
+    try {
+        buf.resize(n * count);
+    } catch (const std::bad_alloc&) {
+        return fail(NRFFT_E_NOMEM, "host staging");
+    }
+    for (size_t i = 0; i < count; ++i) std::memcpy(buf.data() + i * n, data[i], n * sizeof(T));
+    if (int rc = host_real(PK_REALFT, dtype, buf.data(), nullptr, 0, buf.data(), n, 0, count, isign)) return rc;
+    for (size_t i = 0; i < count; ++i) std::memcpy(data[i], buf.data() + i * n, n * sizeof(T));
+    return NRFFT_OK;
+}
+}  // namespace nrfft
+
+extern "C" {
+
+int nrfft_realft_batch_f64(double* const* data, size_t n, size_t count, int isign) {
+    return realft_batch<double>(data, n, count, isign, NRFFT_F64);
+}
+int nrfft_realft_batch_f32(float* const* data, size_t n, size_t count, int isign) {
+    return realft_batch<float>(data, n, count, isign, NRFFT_F32);
 }
 
 int nrfft_convlv_f64(const double* data, size_t n, const double* respns, size_t m, int isign, double* ans) {

@@ -124,7 +124,14 @@ void use_c2c_tune(Plan& p, int logn) {
     p.tune = q.tune;
 }
 
+// The cache holds plain-precision C2C tunings (and uneven size classes);
+// split-precision C2C plans (extra != 0) need a split with an MFMA-capable
+// pass and always take the defaults.
 void tune_lookup(Plan& p) {
+    if (p.kind == PK_C2C && p.extra != 0) {
+        p.tune = default_tune(p);
+        return;
+    }
     std::lock_guard<std::mutex> lk(g_tune_mu);
     if (!g_tune_env_loaded) {
         g_tune_env_loaded = true;
@@ -217,6 +224,7 @@ int nrfft_plan_c2c_tuned(nrfft_plan* plan, int dtype, const size_t* nn, size_t n
                          size_t ntune) {
     g_err.clear();
     if (!tune && ntune) return fail(NRFFT_E_INVALID, "null tune");
+    if (ntune > 7) return fail(NRFFT_E_INVALID, "ntune > 7");
     Tune t;
     if (ntune > 0) t.split_log1 = tune[0];
     if (ntune > 1) t.tilog_a = tune[1];
@@ -224,15 +232,16 @@ int nrfft_plan_c2c_tuned(nrfft_plan* plan, int dtype, const size_t* nn, size_t n
     if (ntune > 3) t.chunk = tune[3];
     if (ntune > 4) t.wg_per_cu = tune[4];
     if (ntune > 5) t.xcd_remap = tune[5];
-    std::vector<size_t> d;
+    const int prec = ntune > 6 ? tune[6] : 0;
+    if (prec != 0 && prec != NRFFT_SPLIT_F16 && prec != NRFFT_SPLIT_BF16) return fail(NRFFT_E_INVALID, "precision");
+    if (prec != 0 && dtype != NRFFT_F32) return fail(NRFFT_E_INVALID, "split precision needs dtype NRFFT_F32");
     if (!nn || ndim == 0) return fail(NRFFT_E_DIMS, "Invalid dimensions");
-    d.assign(nn, nn + ndim);
-    {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tune_env_loaded = true;
-        g_tune[TuneKey(PK_C2C, dtype, d, batch)] = t;
-    }
-    return nrfft_plan_c2c(plan, dtype, nn, ndim, batch);
+    for (size_t i = 0; i < ndim; ++i)
+        if (nn[i] <= 1) return fail(NRFFT_E_DIMS, "Invalid dimension size");
+    for (size_t i = 0; i < ndim; ++i)
+        if (!is_pow2(nn[i])) return fail(NRFFT_E_NOT_POW2, "dimension must be a power of two");
+    // The tune applies to this plan only: the cache is left as it is.
+    return make_plan(plan, PK_C2C, dtype, std::vector<size_t>(nn, nn + ndim), batch, (size_t)prec, &t);
 }
 
 int nrfft_autotune_c2c(int dtype, const size_t* nn, size_t ndim, size_t batch, int iters, int* best_tune,
@@ -300,6 +309,15 @@ int nrfft_autotune_c2c(int dtype, const size_t* nn, size_t ndim, size_t batch, i
         best_tune[5] = best.xcd_remap;
     }
     if (best_ms) *best_ms = best_t;
+    return NRFFT_OK;
+}
+
+int nrfft_plan_tune(nrfft_plan plan, int* tune, size_t ntune) {
+    g_err.clear();
+    if (!plan || (!tune && ntune)) return fail(NRFFT_E_INVALID, "bad argument");
+    const Tune& t = plan->plan.tune;
+    const int v[6] = {t.split_log1, t.tilog_a, t.tilog_b, t.chunk, t.wg_per_cu, t.xcd_remap};
+    for (size_t i = 0; i < ntune && i < 6; ++i) tune[i] = v[i];
     return NRFFT_OK;
 }
 

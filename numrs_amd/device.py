@@ -1,5 +1,11 @@
 """Device-pointer plans (data resident in HBM), for torch tensors or raw
-device addresses.  Execution is enqueued on the caller's HIP stream."""
+device addresses.  Execution is enqueued on the caller's HIP stream.
+
+Every execute checks each torch tensor it is given before anything is
+launched: contiguous, on the plan's GPU, of the plan's dtype and holding at
+least the values the plan reads or writes (a short or wrongly typed buffer
+would make the kernels access memory past its end).  Raw integer addresses
+are passed through unchecked: the caller vouches for them."""
 from __future__ import annotations
 
 import ctypes
@@ -20,9 +26,23 @@ def _dtype_code(dtype) -> int:
     raise TypeError(f"unsupported dtype {dtype}")
 
 
+_DT_NAME = {_lib.F32: "float32", _lib.F64: "float64"}
+
+
 def _check(code: int) -> None:
     if code != _lib.OK:
         raise NrfftError(code)
+
+
+def _current_device():
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            return torch.cuda.current_device()
+    except Exception:  # noqa: BLE001 - no torch or no GPU: device unknown
+        pass
+    return None
 
 
 def _ptr(x) -> int:
@@ -52,6 +72,8 @@ def _stream(stream) -> int:
 class _Plan:
     def __init__(self):
         self._h = ctypes.c_void_p()
+        self._dtype = None
+        self._device = _current_device()
 
     def __del__(self):
         try:
@@ -63,6 +85,23 @@ class _Plan:
         if getattr(self, "_h", None) and self._h.value:
             _lib.lib().nrfft_plan_destroy(self._h)
             self._h = ctypes.c_void_p()
+
+    def _buf(self, x, values: int, name: str) -> int:
+        """Address of buffer x after checking that it can hold `values`
+        scalars of the plan's dtype on the plan's device."""
+        if not self._h.value:
+            raise ValueError("plan has been destroyed")
+        if isinstance(x, int):
+            return x
+        addr = _ptr(x)
+        want = _DT_NAME[self._dtype]
+        if str(x.dtype) != f"torch.{want}":
+            raise ValueError(f"{name}: dtype {x.dtype} does not match the plan's {want}")
+        if self._device is not None and x.device.index is not None and x.device.index != self._device:
+            raise ValueError(f"{name}: tensor on cuda:{x.device.index}, plan on cuda:{self._device}")
+        if x.numel() < values:
+            raise ValueError(f"{name}: {x.numel()} values, the plan needs {values}")
+        return addr
 
     @property
     def workspace_bytes(self) -> int:
@@ -77,40 +116,50 @@ class C2CPlan(_Plan):
     """Batched C2C of shape nn (C order); ndim 1 = four1, ndim >= 2 = fourn.
     ``split`` = "f16" / "bf16" selects the split-precision MFMA form (float32).
     ``tune`` = (split_log1, tilog_a, tilog_b, chunk, wg_per_cu, xcd_remap)
-    (a prefix is enough; -1 = default) overrides the cached or default
-    tunables for this key."""
+    (a prefix is enough; -1 = default) sets the tunables of this plan only."""
 
     def __init__(self, dtype, nn: Sequence[int], batch: int = 1, tune: Sequence[int] | None = None,
                  split: str | None = None):
         super().__init__()
         nn = [int(v) for v in nn]
         arr = (ctypes.c_size_t * len(nn))(*nn)
+        self._dtype = _dtype_code(dtype)
+        prec = 0
         if split is not None:
-            if _dtype_code(dtype) != _lib.F32:
+            if self._dtype != _lib.F32:
                 raise TypeError("split precision applies to float32 plans")
             prec = {"f16": _lib.SPLIT_F16, "bf16": _lib.SPLIT_BF16}[split]
-            if tune is not None:
-                t = (ctypes.c_int * len(tune))(*[int(v) for v in tune])
-                _check(_lib.lib().nrfft_plan_c2c_tuned(ctypes.byref(self._h), _lib.F32, arr, len(nn), batch, t,
-                                                       len(tune)))
-                self.destroy()
-                self._h = ctypes.c_void_p()
+        if tune is not None:
+            tv = [int(v) for v in tune][:6]
+            tv += [-1] * (6 - len(tv))
+            tv.append(prec)
+            t = (ctypes.c_int * 7)(*tv)
+            _check(_lib.lib().nrfft_plan_c2c_tuned(ctypes.byref(self._h), self._dtype, arr, len(nn), batch, t, 7))
+        elif prec:
             _check(_lib.lib().nrfft_plan_c2c_split(ctypes.byref(self._h), arr, len(nn), batch, prec))
-        elif tune is None:
-            _check(_lib.lib().nrfft_plan_c2c(ctypes.byref(self._h), _dtype_code(dtype), arr, len(nn), batch))
         else:
-            t = (ctypes.c_int * len(tune))(*[int(v) for v in tune])
-            _check(_lib.lib().nrfft_plan_c2c_tuned(ctypes.byref(self._h), _dtype_code(dtype), arr, len(nn), batch,
-                                                   t, len(tune)))
+            _check(_lib.lib().nrfft_plan_c2c(ctypes.byref(self._h), self._dtype, arr, len(nn), batch))
         self.nn, self.batch = nn, batch
+        self._values = 2 * batch * int(np.prod(nn))
 
     def execute(self, x_in, x_out, isign: int, stream=None) -> None:
-        _check(_lib.lib().nrfft_execute_c2c(self._h, _ptr(x_in), _ptr(x_out), isign, _stream(stream)))
+        a = self._buf(x_in, self._values, "x_in")
+        b = self._buf(x_out, self._values, "x_out")
+        _check(_lib.lib().nrfft_execute_c2c(self._h, a, b, isign, _stream(stream)))
+
+    @property
+    def tune(self) -> tuple:
+        """The tunables this plan was built with: (split_log1, tilog_a,
+        tilog_b, chunk, wg_per_cu, xcd_remap), -1 = planner default."""
+        t = (ctypes.c_int * 6)()
+        _check(_lib.lib().nrfft_plan_tune(self._h, t, 6))
+        return tuple(t)
 
     def execute_launches(self, x_in, x_out, isign: int, first: int, count: int, stream=None) -> None:
         """Enqueue only launches [first, first+count) (e.g. one pass)."""
-        _check(_lib.lib().nrfft_execute_c2c_launches(self._h, _ptr(x_in), _ptr(x_out), isign, _stream(stream),
-                                                      first, count))
+        a = self._buf(x_in, self._values, "x_in")
+        b = self._buf(x_out, self._values, "x_out")
+        _check(_lib.lib().nrfft_execute_c2c_launches(self._h, a, b, isign, _stream(stream), first, count))
 
 
 def autotune_c2c(dtype, nn: Sequence[int], batch: int = 1, iters: int = 5):
@@ -136,31 +185,39 @@ def tune_cache_load(path: str) -> None:
 class RealFTPlan(_Plan):
     def __init__(self, dtype, n: int, batch: int = 1):
         super().__init__()
-        _check(_lib.lib().nrfft_plan_realft(ctypes.byref(self._h), _dtype_code(dtype), n, batch))
+        self._dtype = _dtype_code(dtype)
+        _check(_lib.lib().nrfft_plan_realft(ctypes.byref(self._h), self._dtype, n, batch))
         self.n, self.batch = n, batch
 
     def execute(self, x, isign: int, stream=None) -> None:
-        _check(_lib.lib().nrfft_execute_realft(self._h, _ptr(x), isign, _stream(stream)))
+        _check(_lib.lib().nrfft_execute_realft(self._h, self._buf(x, self.n * self.batch, "x"), isign,
+                                               _stream(stream)))
 
 
 class ConvlvPlan(_Plan):
     def __init__(self, dtype, n: int, m: int, batch: int = 1):
         super().__init__()
-        _check(_lib.lib().nrfft_plan_convlv(ctypes.byref(self._h), _dtype_code(dtype), n, m, batch))
+        self._dtype = _dtype_code(dtype)
+        _check(_lib.lib().nrfft_plan_convlv(ctypes.byref(self._h), self._dtype, n, m, batch))
         self.n, self.m, self.batch = n, m, batch
 
     def execute(self, data, respns, ans, isign: int = 1, stream=None) -> None:
-        _check(_lib.lib().nrfft_execute_convlv(self._h, _ptr(data), _ptr(respns), _ptr(ans), isign, _stream(stream)))
+        nb, mb = self.n * self.batch, self.m * self.batch
+        _check(_lib.lib().nrfft_execute_convlv(self._h, self._buf(data, nb, "data"), self._buf(respns, mb, "respns"),
+                                               self._buf(ans, nb, "ans"), isign, _stream(stream)))
 
 
 class CorrelPlan(_Plan):
     def __init__(self, dtype, n: int, batch: int = 1):
         super().__init__()
-        _check(_lib.lib().nrfft_plan_correl(ctypes.byref(self._h), _dtype_code(dtype), n, batch))
+        self._dtype = _dtype_code(dtype)
+        _check(_lib.lib().nrfft_plan_correl(ctypes.byref(self._h), self._dtype, n, batch))
         self.n, self.batch = n, batch
 
     def execute(self, d1, d2, ans, stream=None) -> None:
-        _check(_lib.lib().nrfft_execute_correl(self._h, _ptr(d1), _ptr(d2), _ptr(ans), _stream(stream)))
+        nb = self.n * self.batch
+        _check(_lib.lib().nrfft_execute_correl(self._h, self._buf(d1, nb, "d1"), self._buf(d2, nb, "d2"),
+                                               self._buf(ans, nb, "ans"), _stream(stream)))
 
 
 class TwofftPlan(_Plan):
@@ -169,11 +226,14 @@ class TwofftPlan(_Plan):
 
     def __init__(self, dtype, n: int, batch: int = 1):
         super().__init__()
-        _check(_lib.lib().nrfft_plan_twofft(ctypes.byref(self._h), _dtype_code(dtype), n, batch))
+        self._dtype = _dtype_code(dtype)
+        _check(_lib.lib().nrfft_plan_twofft(ctypes.byref(self._h), self._dtype, n, batch))
         self.n, self.batch = n, batch
 
     def execute(self, d1, d2, fft1, fft2, stream=None) -> None:
-        _check(_lib.lib().nrfft_execute_twofft(self._h, _ptr(d1), _ptr(d2), _ptr(fft1), _ptr(fft2),
+        nb = self.n * self.batch
+        _check(_lib.lib().nrfft_execute_twofft(self._h, self._buf(d1, nb, "d1"), self._buf(d2, nb, "d2"),
+                                               self._buf(fft1, 2 * nb, "fft1"), self._buf(fft2, 2 * nb, "fft2"),
                                                _stream(stream)))
 
 
@@ -182,11 +242,14 @@ class Cosft1Plan(_Plan):
 
     def __init__(self, dtype, n: int, batch: int = 1):
         super().__init__()
-        _check(_lib.lib().nrfft_plan_cosft1(ctypes.byref(self._h), _dtype_code(dtype), n, batch))
+        self._dtype = _dtype_code(dtype)
+        _check(_lib.lib().nrfft_plan_cosft1(ctypes.byref(self._h), self._dtype, n, batch))
         self.n, self.batch = n, batch
 
     def execute(self, x, y, stream=None) -> None:
-        _check(_lib.lib().nrfft_execute_cosft1(self._h, _ptr(x), _ptr(y), _stream(stream)))
+        nb = (self.n + 1) * self.batch
+        _check(_lib.lib().nrfft_execute_cosft1(self._h, self._buf(x, nb, "x"), self._buf(y, nb, "y"),
+                                               _stream(stream)))
 
 
 class Cosft2Plan(_Plan):
@@ -194,11 +257,14 @@ class Cosft2Plan(_Plan):
 
     def __init__(self, dtype, n: int, batch: int = 1):
         super().__init__()
-        _check(_lib.lib().nrfft_plan_cosft2(ctypes.byref(self._h), _dtype_code(dtype), n, batch))
+        self._dtype = _dtype_code(dtype)
+        _check(_lib.lib().nrfft_plan_cosft2(ctypes.byref(self._h), self._dtype, n, batch))
         self.n, self.batch = n, batch
 
     def execute(self, x, y, isign: int, stream=None) -> None:
-        _check(_lib.lib().nrfft_execute_cosft2(self._h, _ptr(x), _ptr(y), isign, _stream(stream)))
+        nb = self.n * self.batch
+        _check(_lib.lib().nrfft_execute_cosft2(self._h, self._buf(x, nb, "x"), self._buf(y, nb, "y"), isign,
+                                               _stream(stream)))
 
 
 class Rlft3Plan(_Plan):
@@ -206,21 +272,26 @@ class Rlft3Plan(_Plan):
 
     def __init__(self, dtype, nn1: int, nn2: int, nn3: int, batch: int = 1):
         super().__init__()
-        _check(_lib.lib().nrfft_plan_rlft3(ctypes.byref(self._h), _dtype_code(dtype), nn1, nn2, nn3, batch))
+        self._dtype = _dtype_code(dtype)
+        _check(_lib.lib().nrfft_plan_rlft3(ctypes.byref(self._h), self._dtype, nn1, nn2, nn3, batch))
         self.shape, self.batch = (nn1, nn2, nn3), batch
 
     def execute(self, data, speq, isign: int, stream=None) -> None:
-        _check(_lib.lib().nrfft_execute_rlft3(self._h, _ptr(data), _ptr(speq), isign, _stream(stream)))
+        nn1, nn2, nn3 = self.shape
+        _check(_lib.lib().nrfft_execute_rlft3(self._h, self._buf(data, nn1 * nn2 * nn3 * self.batch, "data"),
+                                              self._buf(speq, 2 * nn1 * nn2 * self.batch, "speq"), isign,
+                                              _stream(stream)))
 
 
-class UnevenPlan:
+class UnevenPlan(_Plan):
     """Uneven batch: items of different lengths / dimensionality in one
     persistent work-queue launch (replayed as a captured hipGraph).
     ``items`` is a list of shapes, (N,) or (n0, n1); ``offsets`` are the items'
     complex-element offsets in the buffers (default: packed back to back)."""
 
     def __init__(self, dtype, items, offsets=None):
-        self._h = ctypes.c_void_p()
+        super().__init__()
+        self._dtype = _dtype_code(dtype)
         items = [tuple(int(v) for v in it) for it in items]
         n = len(items)
         nd = (ctypes.c_int * n)(*[len(it) for it in items])
@@ -233,23 +304,30 @@ class UnevenPlan:
         self.offsets = [int(o) for o in offsets]
         self.items = items
         offs = (ctypes.c_size_t * n)(*self.offsets)
-        _check(_lib.lib().nrfft_plan_uneven(ctypes.byref(self._h), _dtype_code(dtype), n, nd, dims, offs))
+        _check(_lib.lib().nrfft_plan_uneven(ctypes.byref(self._h), self._dtype, n, nd, dims, offs))
         self.total = max(o + int(np.prod(it)) for o, it in zip(self.offsets, items))
 
-    def __del__(self):
-        try:
-            if self._h.value:
-                _lib.lib().nrfft_uneven_destroy(self._h)
-                self._h = ctypes.c_void_p()
-        except Exception:  # noqa: BLE001 - interpreter shutdown
-            pass
+    def destroy(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.lib().nrfft_uneven_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    @property
+    def workspace_bytes(self) -> int:
+        return self.info()["workspace_bytes"]
+
+    @property
+    def num_launches(self) -> int:
+        return 3  # counter reset, phase 1, phase 2
 
     def execute(self, x_in, x_out, isign: int, stream=None, graph: bool = True) -> None:
         fn = _lib.lib().nrfft_execute_uneven if graph else _lib.lib().nrfft_execute_uneven_nograph
-        _check(fn(self._h, _ptr(x_in), _ptr(x_out), isign, _stream(stream)))
+        a = self._buf(x_in, 2 * self.total, "x_in")
+        b = self._buf(x_out, 2 * self.total, "x_out")
+        _check(fn(self._h, a, b, isign, _stream(stream)))
 
     def check(self, stream=None) -> None:
-        """Wait for the stream; raise if the queue's grid barrier timed out."""
+        """Wait for the stream; raise if the last execution left tasks untaken."""
         _check(_lib.lib().nrfft_uneven_check(self._h, _stream(stream)))
 
     def info(self) -> dict:
@@ -274,6 +352,8 @@ def autotune_uneven(dtype, log2n_min: int = 14, log2n_max: int = 20, iters: int 
 def gen_uniform(x, seed: int, start: int = 0, stream=None) -> None:
     """Fill tensor x (float32/float64, contiguous, on the GPU) with the seeded
     uniform[-1,1) stream of numrs_amd.synth, generated in HBM."""
+    if str(x.dtype) not in ("torch.float32", "torch.float64"):
+        raise ValueError(f"gen_uniform: dtype {x.dtype} is not float32 / float64")
     n = x.numel()
     fn = _lib.lib().nrfft_gen_uniform_f32 if str(x.dtype) == "torch.float32" else _lib.lib().nrfft_gen_uniform_f64
     _check(fn(_ptr(x), n, seed, start, _stream(stream)))
