@@ -109,6 +109,18 @@ int get_ctx(DeviceCtx** out) {
     return NRFFT_OK;
 }
 
+// The current device's context if nrfft_init (or any earlier call) built
+// it; never allocates.
+int find_ctx(DeviceCtx** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    auto it = g_ctx.find(dev);
+    if (it == g_ctx.end()) return fail(NRFFT_E_INVALID, "device " + std::to_string(dev) + " not initialised (nrfft_init)");
+    *out = it->second;
+    return NRFFT_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Kernel lookup.
 // ---------------------------------------------------------------------------
@@ -132,6 +144,15 @@ int pick_tilog(int dtype, int mode, int logl, int64_t nb0, int want_log) {
 // Plans.
 // ---------------------------------------------------------------------------
 int plan_add_c2c_pass(Plan& p, int mode, int logl, int tilog, const PassDesc& d, int src, int dst) {
+    Launch l;
+    if (int rc = c2c_pass_launch(p, mode, logl, tilog, d, src, dst, &l)) return rc;
+    p.launches.push_back(l);
+    return NRFFT_OK;
+}
+
+// The launch of one C2C pass (kernel, persistent grid, descriptor); reads
+// p.kind, p.dtype, p.extra and p.tune only.
+int c2c_pass_launch(const Plan& p, int mode, int logl, int tilog, const PassDesc& d, int src, int dst, Launch* out) {
     const int prec = (p.kind == PK_C2C && p.dtype == NRFFT_F32) ? (int)p.extra : 0;
     const bool use_mfma = prec != 0 && mfma_lookup(prec, +1, mode, logl).fn != nullptr;
     const KernelRef& kp = use_mfma ? mfma_lookup(prec, +1, mode, logl) : c2c_lookup(p.dtype, +1, mode, logl, tilog);
@@ -188,7 +209,7 @@ int plan_add_c2c_pass(Plan& p, int mode, int logl, int tilog, const PassDesc& d,
     l.logl = logl;
     l.ti = kp.ti;
     l.mode = mode;
-    p.launches.push_back(l);
+    *out = l;
     return NRFFT_OK;
 }
 
@@ -226,7 +247,27 @@ int plan_c2c_1d(Plan& p, int logn, int64_t batch, int src, int dst, size_t* tmp_
         return plan_add_c2c_pass(p, MODE_JJ, logn, 0, d, src, dst);
     }
     if (logn > 2 * C2C_MAX_LOGL_) return fail(NRFFT_E_UNSUPPORTED, "transform longer than 2^26");
-    // Four-step: N = N1 * N2, x viewed as [n1][n2].
+    FourStep fs;
+    if (int rc = fourstep_setup(p, logn, &fs)) return rc;
+    // Batch chunking: pass A and pass B of one chunk run back to back so the
+    // chunk's intermediate can stay in the last-level cache (256 MB MALL).
+    const int64_t chunk = (p.tune.chunk > 0 && p.tune.chunk < batch) ? p.tune.chunk : batch;
+    *tmp_elems = std::max<size_t>(*tmp_elems, size_t(N) * size_t(chunk));
+    for (int64_t c0 = 0; c0 < batch; c0 += chunk) {
+        const int64_t cb = std::min<int64_t>(chunk, batch - c0);
+        Launch la, lb;
+        if (int rc = fourstep_launches(p, fs, cb, src, dst, &la, &lb)) return rc;
+        la.off_src = c0 * N;
+        lb.off_dst = c0 * N;
+        p.launches.push_back(la);
+        p.launches.push_back(lb);
+    }
+    return NRFFT_OK;
+}
+
+// Four-step split N = N1 * N2 (x viewed as [n1][n2]) and the two passes'
+// tile widths.
+int fourstep_setup(const Plan& p, int logn, FourStep* fs) {
     int l1 = logn / 2, l2 = logn - l1;
     if (p.tune.split_log1 > 0 && p.tune.split_log1 < logn && logn - p.tune.split_log1 <= C2C_MAX_LOGL_ &&
         p.tune.split_log1 <= C2C_MAX_LOGL_) {
@@ -238,56 +279,53 @@ int plan_c2c_1d(Plan& p, int logn, int64_t batch, int src, int dst, size_t* tmp_
         l1 = logn / 2;
         l2 = logn - l1;
     }
-    const int64_t N1 = int64_t(1) << l1, N2 = int64_t(1) << l2;
-    // Batch chunking: pass A and pass B of one chunk run back to back so the
-    // chunk's intermediate can stay in the last-level cache (256 MB MALL).
-    const int64_t chunk = (p.tune.chunk > 0 && p.tune.chunk < batch) ? p.tune.chunk : batch;
-    *tmp_elems = std::max<size_t>(*tmp_elems, size_t(N) * size_t(chunk));
+    if (l2 > C2C_MAX_LOGL_ || l1 > C2C_MAX_LOGL_) return fail(NRFFT_E_UNSUPPORTED, "no four-step split");
     const int want_a = p.tune.tilog_a >= 0 ? p.tune.tilog_a : col_tilog(p.dtype);
     const int want_b = p.tune.tilog_b >= 0 ? p.tune.tilog_b : col_tilog(p.dtype);
-    const int ta = pick_tilog(p.dtype, MODE_TT, l1, N2, want_a);
-    const int tb = pick_tilog(p.dtype, MODE_JT, l2, N1, want_b);
-    if (ta < 0 || tb < 0) return fail(NRFFT_E_UNSUPPORTED, "no four-step kernel");
-    for (int64_t c0 = 0; c0 < batch; c0 += chunk) {
-        const int64_t cb = std::min<int64_t>(chunk, batch - c0);
-        // Pass A: FFT over n1 (stride N2), tile over n2, twiddle W_N^{k1 n2}; -> tmp[k1][n2].
-        PassDesc a{};
-        a.nb0 = N2;
-        a.si0 = 1;
-        a.so0 = 1;
-        a.esi = (int32_t)N2;
-        a.eso = (int32_t)N2;
-        a.nb1 = cb;
-        a.si1 = N;
-        a.so1 = N;
-        a.nb2 = 1;
-        a.tw_logn = logn;
-        a.tw_b0_shift = 0;
-        int rc = plan_add_c2c_pass(p, MODE_TT, l1, ta, a, src, B_TMP);
-        if (rc) return rc;
-        p.launches.back().off_src = c0 * N;
-        // Pass B: FFT over n2 (contiguous rows of tmp), rows k1 tiled; X[k2 N1 + k1].
-        PassDesc b{};
-        b.nb0 = N1;
-        b.si0 = (int32_t)N2;
-        b.esi = 1;
-        b.so0 = 1;
-        b.eso = (int32_t)N1;
-        b.nb1 = cb;
-        b.si1 = N;
-        b.so1 = N;
-        b.nb2 = 1;
-        // The transform's result: stored non-temporally, so the writes do not
-        // evict the next chunk's input and intermediate from L2 / MALL
-        // (64 x 2^20 FP32: 0.487 -> 0.454 ms on MI355X).  Whatever reads it
-        // next (the caller, or a finishing kernel of a real or spectral plan)
-        // runs after the last chunk.
-        b.nt_out = 1;
-        rc = plan_add_c2c_pass(p, MODE_JT, l2, tb, b, B_TMP, dst);
-        if (rc) return rc;
-        p.launches.back().off_dst = c0 * N;
-    }
+    fs->logn = logn;
+    fs->l1 = l1;
+    fs->l2 = l2;
+    fs->ta = pick_tilog(p.dtype, MODE_TT, l1, int64_t(1) << l2, want_a);
+    fs->tb = pick_tilog(p.dtype, MODE_JT, l2, int64_t(1) << l1, want_b);
+    if (fs->ta < 0 || fs->tb < 0) return fail(NRFFT_E_UNSUPPORTED, "no four-step kernel");
     return NRFFT_OK;
+}
+
+// Pass A and pass B of one chunk of cb transforms (offsets left at 0).
+int fourstep_launches(const Plan& p, const FourStep& fs, int64_t cb, int src, int dst, Launch* la, Launch* lb) {
+    const int64_t N = int64_t(1) << fs.logn, N1 = int64_t(1) << fs.l1, N2 = int64_t(1) << fs.l2;
+    // Pass A: FFT over n1 (stride N2), tile over n2, twiddle W_N^{k1 n2}; -> tmp[k1][n2].
+    PassDesc a{};
+    a.nb0 = N2;
+    a.si0 = 1;
+    a.so0 = 1;
+    a.esi = (int32_t)N2;
+    a.eso = (int32_t)N2;
+    a.nb1 = cb;
+    a.si1 = N;
+    a.so1 = N;
+    a.nb2 = 1;
+    a.tw_logn = fs.logn;
+    a.tw_b0_shift = 0;
+    if (int rc = c2c_pass_launch(p, MODE_TT, fs.l1, fs.ta, a, src, B_TMP, la)) return rc;
+    // Pass B: FFT over n2 (contiguous rows of tmp), rows k1 tiled; X[k2 N1 + k1].
+    PassDesc b{};
+    b.nb0 = N1;
+    b.si0 = (int32_t)N2;
+    b.esi = 1;
+    b.so0 = 1;
+    b.eso = (int32_t)N1;
+    b.nb1 = cb;
+    b.si1 = N;
+    b.so1 = N;
+    b.nb2 = 1;
+    // The transform's result: stored non-temporally, so the writes do not
+    // evict the next chunk's input and intermediate from L2 / MALL
+    // (64 x 2^20 FP32: 0.487 -> 0.454 ms on MI355X).  Whatever reads it
+    // next (the caller, or a finishing kernel of a real or spectral plan)
+    // runs after the last chunk.
+    b.nt_out = 1;
+    return c2c_pass_launch(p, MODE_JT, fs.l2, fs.tb, b, B_TMP, dst, lb);
 }
 
 // Strided axes of >= 4096 points take the two-pass four-step along the axis
@@ -429,40 +467,44 @@ int exec_plan(const Plan& p, const void* in, void* out, void* tmp, const void* a
     for (const Launch& l : p.launches) {
         ++idx;
         if (idx < first || (count >= 0 && idx >= first + count)) continue;
-        if (l.args[7] != 0 && ((l.args[7] > 0) != (isign > 0))) continue;
-        const int eff = l.args[6] != 0 ? (int)l.args[6] : isign;
-        const int si = eff > 0 ? 0 : 1;
-        void* a = static_cast<char*>(bufs[l.src]) + l.off_src * (int64_t)csize(p.dtype);
-        void* b = static_cast<char*>(bufs[l.dst]) + l.off_dst * (int64_t)csize(p.dtype);
-        if (l.kind == LK_COPY) {
-            if (a != b) HIPCHK(hipMemcpyAsync(b, a, l.bytes, hipMemcpyDeviceToDevice, stream));
-            continue;
-        }
-        if (l.kind == LK_C2C) {
-            PassDesc d = l.desc;
-            if (p.dtype == NRFFT_F32 && l.prec) {
-                Twiddles<float> tw = ctx_twiddles<float>(ctx);
-                const void* amat = l.prec == NRFFT_SPLIT_F16 ? ctx->mfma_f16 : ctx->mfma_bf16;
-                void* args[] = {&a, &b, &d, &tw, &amat};
-                HIPCHK(hipLaunchKernel(l.fn[si], dim3(l.grid), dim3(l.nt), args, 0, stream));
-            } else if (p.dtype == NRFFT_F32) {
-                Twiddles<float> tw = ctx_twiddles<float>(ctx);
-                void* args[] = {&a, &b, &d, &tw};
-                if (l.fn_dma[si] && (reinterpret_cast<uintptr_t>(a) & 15) == 0)
-                    HIPCHK(hipLaunchKernel(l.fn_dma[si], dim3(l.grid_dma), dim3(l.nt_dma), args, 0, stream));
-                else
-                    HIPCHK(hipLaunchKernel(l.fn[si], dim3(l.grid), dim3(l.nt), args, 0, stream));
-            } else {
-                Twiddles<double> tw = ctx_twiddles<double>(ctx);
-                void* args[] = {&a, &b, &d, &tw};
-                HIPCHK(hipLaunchKernel(l.fn[si], dim3(l.grid), dim3(l.nt), args, 0, stream));
-            }
-            continue;
-        }
-        rc = l.kind == LK_SPEC ? exec_spectral(p, l, bufs, eff, stream, ctx) : exec_special(p, l, bufs, eff, stream, ctx);
-        if (rc) return rc;
+        if ((rc = exec_launch(p, l, bufs, isign, stream, ctx))) return rc;
     }
     return NRFFT_OK;
+}
+
+// Enqueue one launch of plan p on the buffers bufs (indexed by BufId).
+int exec_launch(const Plan& p, const Launch& l, void* const* bufs, int isign, hipStream_t stream, DeviceCtx* ctx) {
+    if (l.args[7] != 0 && ((l.args[7] > 0) != (isign > 0))) return NRFFT_OK;
+    const int eff = l.args[6] != 0 ? (int)l.args[6] : isign;
+    const int si = eff > 0 ? 0 : 1;
+    void* a = static_cast<char*>(bufs[l.src]) + l.off_src * (int64_t)csize(p.dtype);
+    void* b = static_cast<char*>(bufs[l.dst]) + l.off_dst * (int64_t)csize(p.dtype);
+    if (l.kind == LK_COPY) {
+        if (a != b) HIPCHK(hipMemcpyAsync(b, a, l.bytes, hipMemcpyDeviceToDevice, stream));
+        return NRFFT_OK;
+    }
+    if (l.kind == LK_C2C) {
+        PassDesc d = l.desc;
+        if (p.dtype == NRFFT_F32 && l.prec) {
+            Twiddles<float> tw = ctx_twiddles<float>(ctx);
+            const void* amat = l.prec == NRFFT_SPLIT_F16 ? ctx->mfma_f16 : ctx->mfma_bf16;
+            void* args[] = {&a, &b, &d, &tw, &amat};
+            HIPCHK(hipLaunchKernel(l.fn[si], dim3(l.grid), dim3(l.nt), args, 0, stream));
+        } else if (p.dtype == NRFFT_F32) {
+            Twiddles<float> tw = ctx_twiddles<float>(ctx);
+            void* args[] = {&a, &b, &d, &tw};
+            if (l.fn_dma[si] && (reinterpret_cast<uintptr_t>(a) & 15) == 0)
+                HIPCHK(hipLaunchKernel(l.fn_dma[si], dim3(l.grid_dma), dim3(l.nt_dma), args, 0, stream));
+            else
+                HIPCHK(hipLaunchKernel(l.fn[si], dim3(l.grid), dim3(l.nt), args, 0, stream));
+        } else {
+            Twiddles<double> tw = ctx_twiddles<double>(ctx);
+            void* args[] = {&a, &b, &d, &tw};
+            HIPCHK(hipLaunchKernel(l.fn[si], dim3(l.grid), dim3(l.nt), args, 0, stream));
+        }
+        return NRFFT_OK;
+    }
+    return l.kind == LK_SPEC ? exec_spectral(p, l, bufs, eff, stream, ctx) : exec_special(p, l, bufs, eff, stream, ctx);
 }
 
 // Plan cache for the host-pointer calls (plans without workspace).

@@ -41,6 +41,7 @@ inline Twiddles<T> ctx_twiddles(const DeviceCtx* c) {
     else return Twiddles<T>{c->c64c, c->c64f, c->c64s};
 }
 int get_ctx(DeviceCtx** out);
+int find_ctx(DeviceCtx** out);
 
 // Buffers a launch reads or writes.
 // B_AUX2: a second caller buffer (twofft's second spectrum, rlft3's speq).
@@ -137,8 +138,19 @@ void use_c2c_tune(Plan& p, int logn);
 bool tune_find(int kind, int dtype, const std::vector<size_t>& nn, size_t batch, Tune* out);
 void tune_store(int kind, int dtype, const std::vector<size_t>& nn, size_t batch, const Tune& t);
 Tune default_tune(const Plan& p);
+// Defaults of a 1-D C2C plan of length n (kind PK_C2C; extra = split precision).
+Tune default_tune_1d(int dtype, size_t n, size_t batch, size_t extra);
 int plan_c2c_1d(Plan& p, int logn, int64_t batch, int src, int dst, size_t* tmp_elems);
 int plan_add_c2c_pass(Plan& p, int mode, int logl, int tilog, const PassDesc& d, int src, int dst);
+int c2c_pass_launch(const Plan& p, int mode, int logl, int tilog, const PassDesc& d, int src, int dst, Launch* out);
+struct FourStep {
+    int logn = 0, l1 = 0, l2 = 0;  // N = 2^l1 * 2^l2
+    int ta = 0, tb = 0;            // log2 tile widths of pass A / pass B
+};
+int fourstep_setup(const Plan& p, int logn, FourStep* fs);
+int fourstep_launches(const Plan& p, const FourStep& fs, int64_t cb, int src, int dst, Launch* la, Launch* lb);
+int exec_launch(const Plan& p, const Launch& l, void* const* bufs, int isign, hipStream_t stream, DeviceCtx* ctx);
+int col_tilog(int dtype);
 int exec_plan(const Plan& p, const void* in, void* out, void* tmp, const void* aux, int isign, hipStream_t stream,
               int first = 0, int count = -1, void* aux2 = nullptr);
 int cached_plan(int kind, int dtype, const std::vector<size_t>& nn, size_t batch, size_t extra,

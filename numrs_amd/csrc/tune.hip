@@ -67,9 +67,16 @@ static int load_file_locked(const char* path) {
 // Defaults for shapes the tuner has not seen, from sweeps on MI355X
 // (64 x 2^20 FP32: split 2^9 x 2^11, 16-wide tiles, 16-transform chunks).
 Tune default_tune(const Plan& p) {
+    if (p.kind == PK_C2C && p.nn.size() == 1) return default_tune_1d(p.dtype, p.nn[0], p.batch, p.extra);
     Tune t;
-    if (p.kind == PK_C2C && p.nn.size() == 1) {
-        const int logn = ilog2(p.nn[0]);
+    if (p.kind == PK_C2C) t.tilog_a = 4;
+    return t;
+}
+
+Tune default_tune_1d(int dtype, size_t n, size_t batch, size_t extra) {
+    Tune t;
+    {
+        const int logn = ilog2(n);
         if (logn > C2C_MAX_LOGL_) {
             // Four-step split and batch chunking per length, from a sweep on
             // MI355X (256-512 MiB batches, pass B with non-temporal result
@@ -83,13 +90,13 @@ Tune default_tune(const Plan& p) {
             // 0.938 -> 0.626 (512 MiB); FP64 (256 MiB) 2^14 0.218 -> 0.206,
             // 2^20 0.259 -> 0.211, 2^21 0.529 -> 0.229, 2^22 0.642 -> 0.313.
             static const int short_f32[5] = {7, 8, 7, 8, 9}, short_f64[5] = {7, 8, 8, 9, 9};  // 2^14..2^18
-            const bool f32 = p.dtype == NRFFT_F32;
+            const bool f32 = dtype == NRFFT_F32;
             t.split_log1 = logn >= 19 ? std::max(logn - C2C_MAX_LOGL_, std::min(logn - 11, 11))
                                       : (f32 ? short_f32 : short_f64)[logn - 14];
             // A split-precision plan needs a pass of 2^8..2^10 points for its
             // MFMA stages; if neither pass has one, the split near sqrt N.
             auto mfma_len = [](int l) { return l >= 8 && l <= 10; };
-            if (p.extra != 0 && !mfma_len(t.split_log1) && !mfma_len(logn - t.split_log1))
+            if (extra != 0 && !mfma_len(t.split_log1) && !mfma_len(logn - t.split_log1))
                 t.split_log1 = std::max(logn - C2C_MAX_LOGL_, logn / 2 - 1);
             // Tile widths (log2 transforms per tile; capped by LDS per length):
             // 16 for FP32; FP64 8-wide pass-B tiles and 8-wide pass-A tiles from
@@ -100,14 +107,12 @@ Tune default_tune(const Plan& p) {
             // A small problem (a few transforms) is launch-latency bound: 4-wide
             // tiles spread each pass over more CUs (one FP64 2^16 transform:
             // 10.6 -> 9.8 us).
-            if (p.batch * p.nn[0] <= (size_t(1) << 18)) t.tilog_a = t.tilog_b = 2;
-            const size_t per = p.nn[0] * (f32 ? 8 : 16);
+            if (batch * n <= (size_t(1) << 18)) t.tilog_a = t.tilog_b = 2;
+            const size_t per = n * (f32 ? 8 : 16);
             const size_t mib = logn == 14 ? 32 : (f32 ? logn <= 18 : logn <= 17) ? 64 : 128;
             const size_t chunk = std::max<size_t>(1, (mib << 20) / per);
-            if (chunk < p.batch) t.chunk = (int)chunk;
+            if (chunk < batch) t.chunk = (int)chunk;
         }
-    } else if (p.kind == PK_C2C) {
-        t.tilog_a = 4;
     }
     return t;
 }
