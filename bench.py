@@ -100,9 +100,23 @@ def _rel_l2_gpu(a, ref) -> float:
     return float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(ref))
 
 
-def _time_plan(plan, x, y, steps: int, warmup: int) -> float:
+def _settle(fn, seconds: float) -> None:
+    """Run fn back to back for `seconds` before a timed region, so the GPU
+    clocks have settled whatever the warm-up step count (a 20-step run right
+    after start-up measured ~7-10% slower per step than a settled one)."""
     import torch
 
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(10):
+            fn()
+        torch.cuda.synchronize()
+
+
+def _time_plan(plan, x, y, steps: int, warmup: int, settle: float = 1.0) -> float:
+    import torch
+
+    _settle(lambda: plan.execute(x, y, 1), settle)
     for _ in range(warmup):
         plan.execute(x, y, 1)
     torch.cuda.synchronize()
@@ -206,8 +220,16 @@ def headline(args) -> dict:
     p64 = numrs_amd.C2CPlan("float64", [n], batch)
     p64.execute(x64, y64, 1)
     torch.cuda.synchronize()
-    del x64, p64
     out["rel_l2_vs_fp64"] = _rel_l2_gpu(y, y64)
+    # The FP64 plan on the same batch (the reference computes in f64 only).
+    y64b = torch.empty_like(x64)
+    ms64 = _time_plan(p64, x64, y64b, args.steps, args.warmup)
+    comp64 = 2.0 * pts * 16
+    out["fp64"] = {"ms": round(ms64, 4), "GFLOPs": round(flops / (ms64 * 1e-3) / 1e9, 1),
+                   "GBps_compulsory": round(comp64 / (ms64 * 1e-3) / 1e9, 1),
+                   "frac_hbm": round(comp64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                   "launches": p64.num_launches}
+    del x64, y64b, p64
     split = {}
     for name in ("f16", "bf16"):
         sp = numrs_amd.C2CPlan("float32", [n], batch, split=name)
@@ -222,44 +244,166 @@ def headline(args) -> dict:
     return out
 
 
+def _host_threads() -> dict:
+    """Host cores this process may use.  OpenMP pools are capped at 16
+    threads: the GPU host's share of CPU for one GPU is 16 workers, whatever
+    nproc reports there."""
+    nproc = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = nproc
+    return {"nproc": nproc, "affinity": aff, "threads": max(1, min(aff, 16))}
+
+
+def _median_time(run, reps: int):
+    """(median seconds, value returned by the last run) over reps timed runs
+    after one untimed warm-up run."""
+    out = run()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = run()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), out
+
+
 def cpu_baseline(args) -> dict:
-    """The oracle's literal four1 (FFT_1.rs:5-107, FP64) over a bounded sample
-    of the headline workload: `sample` transforms of 2^20 on all host cores
-    (OpenMP over transforms, as FFTProcessor::fft_batch spreads them with
-    rayon), then a smaller sample on one core.  One warm-up run, then the
-    median of `reps` runs."""
+    """The reference's CPU path, restated by the oracle (FP64, the reference's
+    only precision), timed on the host cores beside the GPU run, on bounded
+    samples of each configuration:
+      headline (64 x 2^20): the literal four1 of FFT_1.rs:5-107 over
+        transforms on all usable cores (OpenMP, as FFTProcessor::fft_batch
+        spreads them with rayon, FFT_1.rs:185-189), and on one core;
+      config 1: one four1 of 2^16 (serial, as the reference runs one call);
+      config 2: NR fourn 256^3 (serial: the reference has no parallel fourn);
+      config 3: convlv over signals (Convolve.rs:246-249) on all cores and
+        one core;
+      config 5: fft_batch over the uneven items (1-D literal four1, 2-D
+        fourn) on all cores and one core.
+    The oracle is built with -march=native on this host when gcc is here
+    (oracle/liborc_native.so), else the portable build is used."""
     import oracle
     from numrs_amd import synth
 
+    host = _host_threads()
+    L = oracle.native_lib()
+    build = "gcc -O3 -march=native -ffp-contract=off -fopenmp (built on this host)"
+    if L is None:
+        L = oracle.lib()
+        build = "gcc -O3 -ffp-contract=off -fopenmp (portable build; -march=native build failed)"
+    threads = host["threads"]
+    res = {}
+
+    # headline shape: 64 transforms of 2^20 on all cores, 8 on one core
     n = 1 << 20
-    threads = min(os.cpu_count() or 1, 16)
     sample = args.cpu_sample
     src = np.empty(2 * n * sample, np.float64)
     for i in range(sample):
         src[2 * n * i:2 * n * (i + 1)] = synth.uniform(3, 2 * n, start=2 * n * i).astype(np.float32)
 
-    def timed(count: int, nthreads: int, reps: int):
+    def four1_batch(count, nthreads):
         data = src[:2 * n * count].copy()
-        used = oracle.four1_batch_inplace(data, n, count, 1, nthreads)  # warm-up
-        times = []
-        for _ in range(reps):
-            data[:] = src[:2 * n * count]
-            t0 = time.perf_counter()
-            used = oracle.four1_batch_inplace(data, n, count, 1, nthreads)
-            times.append(time.perf_counter() - t0)
-        return float(np.median(times)), used
+        return oracle.four1_batch_inplace(data, n, count, 1, nthreads, L)
 
-    med, used = timed(sample, threads, args.cpu_reps)
-    one_count = max(1, sample // 8)
-    med1, _ = timed(one_count, 1, 3)
-    gflops = 5.0 * n * 20 * sample / med / 1e9
-    gflops1 = 5.0 * n * 20 * one_count / med1 / 1e9
-    return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": used, "kind": "port",
-            "sample": f"{sample} of the 64 transforms of N=2^20 (FP64 literal four1 of FFT_1.rs), "
-                      f"median of {args.cpu_reps} runs after one warm-up: {med:.2f} s wall on {used} threads "
-                      f"({med * used:.0f} core-s per run)",
-            "single_core": {"value": round(gflops1, 3), "unit": "GFLOP/s", "cores": 1,
-                            "sample": f"{one_count} transforms of N=2^20, median of 3 runs: {med1:.2f} s"}}
+    med, used = _median_time(lambda: four1_batch(sample, threads), args.cpu_reps)
+    one = max(1, sample // 8)
+    med1, _ = _median_time(lambda: four1_batch(one, 1), 3)
+    del src
+    fl = 5.0 * n * 20
+    out = {"value": round(fl * sample / med / 1e9, 3), "unit": "GFLOP/s", "cores": used, "kind": "port",
+           "sample": f"{sample} of the 64 transforms of N=2^20 (FP64 literal four1 of FFT_1.rs), median of "
+                     f"{args.cpu_reps} runs after one warm-up: {med:.2f} s wall on {used} threads",
+           "single_core": {"value": round(fl * one / med1 / 1e9, 3), "unit": "GFLOP/s", "cores": 1,
+                           "sample": f"{one} transforms of N=2^20, median of 3 runs: {med1:.2f} s"},
+           "host": dict(host, build=build)}
+
+    # config 1: four1 2^16 FP64, uniform seed 0, one transform
+    n1 = 1 << 16
+    x1 = synth.uniform(0, 2 * n1)
+
+    def c1():
+        d = x1.copy()
+        oracle.four1_literal_with(L, d, 1)
+        return d
+
+    t1, _ = _median_time(c1, 21)
+    res["config1_four1_f64_2p16"] = {"us": round(t1 * 1e6, 1), "GFLOPs": round(5.0 * n1 * 16 / t1 / 1e9, 3),
+                                     "cores": 1, "sample": "one transform, median of 21 runs"}
+
+    # config 3: convlv 4096 x 2^14, m = 257 (FP64 on the FP32-rounded inputs)
+    b3, n3, m3 = 4096, 1 << 14, 257
+    d3 = synth.normal(2, b3 * n3).astype(np.float32).astype(np.float64)
+    r3 = synth.gaussian_windowed_responses(2, b3, m3, 40.0, start=b3 * n3).astype(np.float32).astype(np.float64)
+    a3 = np.empty_like(d3)
+    t3, used3 = _median_time(lambda: oracle.convlv_batch(d3, n3, r3.reshape(-1), m3, b3, 1, a3, threads, L), 3)
+    one3 = 256
+    t31, _ = _median_time(lambda: oracle.convlv_batch(d3, n3, r3.reshape(-1), m3, one3, 1, a3, 1, L), 3)
+    fl3 = 3 * 2.5 * n3 * 14
+    res["config3_convlv_4096x2p14_m257"] = {
+        "ms": round(t3 * 1e3, 2), "signals_per_s": round(b3 / t3, 1), "GFLOPs": round(fl3 * b3 / t3 / 1e9, 3),
+        "GBps": round((2.0 * n3 + m3) * 8 * b3 / t3 / 1e9, 2), "cores": used3,
+        "sample": f"all 4096 signals, median of 3 runs on {used3} threads",
+        "single_core": {"signals_per_s": round(one3 / t31, 1), "GFLOPs": round(fl3 * one3 / t31 / 1e9, 3),
+                        "sample": f"{one3} signals, median of 3 runs"}}
+    del d3, r3, a3
+
+    # config 5: the uneven batch's first items (the list is in random order)
+    items = synth.uneven_shapes(4, 10000)
+    offs, acc = [], 0
+    for it in items:
+        offs.append(acc)
+        acc += int(np.prod(it))
+
+    def uneven(count, nthreads):
+        tot = offs[count - 1] + int(np.prod(items[count - 1]))
+        data = synth.uniform(4, 2 * tot).astype(np.float32).astype(np.float64)
+        t0 = time.perf_counter()
+        used_ = oracle.uneven_batch_inplace(data, items[:count], offs[:count], 1, nthreads, L)
+        return time.perf_counter() - t0, used_
+
+    def uneven_flops(count):
+        return sum(5.0 * int(np.prod(it)) * math.log2(int(np.prod(it))) for it in items[:count])
+
+    k5, k51 = args.cpu_uneven_items, max(1, args.cpu_uneven_items // 8)
+    uneven(k5, threads)  # warm-up
+    r5 = sorted(uneven(k5, threads) for _ in range(3))[1]
+    r51 = sorted(uneven(k51, 1) for _ in range(3))[1]
+    res["config5_uneven_first_items"] = {
+        "GFLOPs": round(uneven_flops(k5) / r5[0] / 1e9, 3), "items_per_s": round(k5 / r5[0], 1),
+        "cores": r5[1], "sample": f"items 0..{k5 - 1} of the 10,000 (shapes drawn in random order), "
+                                  f"median of 3 runs: {r5[0]:.2f} s",
+        "single_core": {"GFLOPs": round(uneven_flops(k51) / r51[0] / 1e9, 3),
+                        "sample": f"items 0..{k51 - 1}, median of 3 runs: {r51[0]:.2f} s"}}
+
+    # config 2: NR fourn 256^3 FP64 (serial), one run after none: ~10 s
+    x2 = synth.normal(1, 2 * 256 ** 3).astype(np.float32).astype(np.float64)
+    t0 = time.perf_counter()
+    oracle.fourn_with(L, x2, [256, 256, 256], 1)
+    t2 = time.perf_counter() - t0
+    res["config2_fourn_256cubed_f64"] = {"s": round(t2, 3), "GFLOPs": round(5.0 * 256 ** 3 * 24 / t2 / 1e9, 3),
+                                         "cores": 1, "sample": "one forward transform (FP32-rounded N(0,1) "
+                                                               "seed 1, the GPU bench's input)"}
+    out["configs"] = res
+    out["_fourn_oracle_output"] = x2
+    return out
+
+
+def _fourn_gpu_vs_oracle(ref: np.ndarray) -> float:
+    """Relative L2 of the FP64 GPU fourn 256^3 (the bench's input) against the
+    oracle's output for the same input, computed by the CPU-baseline leg."""
+    import torch
+
+    import numrs_amd
+
+    pts = 256 ** 3
+    x = torch.empty(2 * pts, dtype=torch.float64, device="cuda")
+    fill_normal(x, 1, 2 * pts, np.float32)
+    numrs_amd.C2CPlan("float64", [256, 256, 256], 1).execute(x, x, 1)
+    got = x.cpu().numpy()
+    del x
+    torch.cuda.empty_cache()
+    return float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
 
 
 def copy_calibration() -> dict:
@@ -353,23 +497,42 @@ def _extra_fourn() -> dict:
     import numrs_amd
 
     res = {}
-    # fourn 256^3 FP32 and FP64 (config 2 shape): per pass and total.
-    for dt, tdt, es in (("float32", torch.float32, 8), ("float64", torch.float64, 16)):
-        shape = [256, 256, 256]
-        pts = 256 ** 3
+    # fourn 256^3 FP32 and FP64 (config 2 shape): total, and each axis pass
+    # timed between HIP events inside the plan's own sequence (a pass timed
+    # repeated on its own finds its 128-256 MiB array warm in the MALL).
+    # The FP32 result's relative L2 is against the FP64 path on the same
+    # (FP32-rounded) input; the FP64 path is checked against the oracle in
+    # the tests and in the CPU-baseline leg.
+    shape = [256, 256, 256]
+    pts = 256 ** 3
+    ref = None
+    for dt, tdt, es in (("float64", torch.float64, 16), ("float32", torch.float32, 8)):
+        # N(0,1) from seed 1 rounded to FP32 (the FP64 run holds the same values)
         x = torch.empty(2 * pts, dtype=tdt, device="cuda")
-        fill_normal(x, 1, 2 * pts, np.float32 if es == 8 else np.float64)
+        fill_normal(x, 1, 2 * pts, np.float32)
+        y = torch.empty_like(x)
         plan = numrs_amd.C2CPlan(dt, shape, 1)
-        for _ in range(3):
-            plan.execute(x, x, 1)
-        total = time_events(lambda: plan.execute(x, x, 1), 10)
-        per = [time_events(lambda i=i: plan.execute_launches(x, x, 1, i, 1), 10) for i in range(plan.num_launches)]
+        plan.execute(x, y, 1)
+        torch.cuda.synchronize()
+        if ref is None:
+            ref = y.clone()
+            err = None
+        else:
+            err = _rel_l2_gpu(y, ref)
+        _settle(lambda: plan.execute(x, y, 1), 0.5)
+        total = time_events(lambda: plan.execute(x, y, 1), 20)
+        per = _time_launches_in_sequence(plan, x, y, 20)
         byt = 2.0 * pts * es
         res[f"fourn_256cubed_{dt}"] = {
             "ms": round(total, 4), "GFLOPs": round(5 * pts * 24 / (total * 1e-3) / 1e9, 1),
             "GBps_total_compulsory": round(byt / (total * 1e-3) / 1e9, 1),
-            "pass_GBps": [round(byt / (p * 1e-3) / 1e9, 1) for p in per]}
-        del plan, x
+            "frac_hbm": round(byt / (total * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "pass_ms_in_sequence": [round(p, 4) for p in per],
+            "pass_GBps_in_sequence": [round(byt / (p * 1e-3) / 1e9, 1) for p in per]}
+        if err is not None:
+            res[f"fourn_256cubed_{dt}"]["rel_l2_vs_fp64"] = err
+        del plan, x, y
+    del ref
     torch.cuda.empty_cache()
     return res
 
@@ -595,6 +758,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-sample", type=int, default=64, help="transforms in the CPU baseline sample")
     ap.add_argument("--cpu-reps", type=int, default=5, help="timed runs of the CPU sample (median kept)")
+    ap.add_argument("--cpu-uneven-items", type=int, default=400, help="uneven items in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--only", choices=["headline", "four1", "fourn", "convlv", "uneven", "spectral", "fourfs"], default=None,
@@ -631,7 +795,11 @@ def main() -> int:
         out["extra"] = extra_configs(args)
     if not args.no_cpu and args.only is None:
         log("[bench] cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(args)
+        cb = cpu_baseline(args)
+        ref = cb.pop("_fourn_oracle_output")
+        cb["configs"]["config2_fourn_256cubed_f64"]["gpu_fp64_rel_l2_vs_oracle"] = _fourn_gpu_vs_oracle(ref)
+        del ref
+        out["cpu_baseline"] = cb
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -11,8 +11,8 @@ const KernelRef& c2c_lookup_f32p(int mode, int logl, int tilog) {
 // DMA-staged column passes, log2 L = 8, 9 (else fn = nullptr).
 const DmaRef& dma_lookup_f32p(int logl) {
     static const DmaRef none{nullptr, 0, 0};
-    static const DmaRef t8{reinterpret_cast<const void*>(&k_c2c_dma<1, 256>), 256, dma_lds_bytes<256>()};
-    static const DmaRef t9{reinterpret_cast<const void*>(&k_c2c_dma<1, 512>), 512, dma_lds_bytes<512>()};
+    static const DmaRef t8{dma_kernel<1, 256>(), 256, dma_lds_bytes<256>()};
+    static const DmaRef t9{dma_kernel<1, 512>(), 512, dma_lds_bytes<512>()};
     return logl == 8 ? t8 : logl == 9 ? t9 : none;
 }
 }  // namespace nrfft

@@ -68,8 +68,15 @@ constexpr int kc2c_lds_bytes() {
     constexpr int ltm = kc2c_ltm<T, S, L, TI, NT, ML, MS>();
     return E::NST > 1 ? (E::LDS_ELEMS + (ltm > 0 ? E::tw_elems(ltm) : 0)) * (int)sizeof(cpx<T>) : 0;
 }
+// Register prefetch (NRFFT_PREFETCH).  Not for the 1024-thread FP32 pass B
+// of 2048 points alone either: its 4 waves per SIMD leave 128 VGPRs, and
+// the next tile's 32 would spill 136 bytes per lane.
+template <typename T, int NT, int ML, int MS>
+constexpr bool kc2c_pf() {
+    return NRFFT_PREFETCH != 0;
+}
 template <typename T, int S, int L, int TI, int NT, int ML, int MS>
-constexpr int kc2c_waves = waves_per_eu_target<kc2c_lds_bytes<T, S, L, TI, NT, ML, MS>(), NT, NRFFT_PREFETCH != 0>();
+constexpr int kc2c_waves = waves_per_eu_target<kc2c_lds_bytes<T, S, L, TI, NT, ML, MS>(), NT, kc2c_pf<T, NT, ML, MS>()>();
 
 // Thread index laundered through a register move the compiler cannot hoist,
 // so per-tile address arithmetic is recomputed inside the persistent loop
@@ -125,11 +132,13 @@ k_c2c(const cpx<T>* in, cpx<T>* out, PassDesc d, Twiddles<T> tw) {
 
     // Published by the first exchange barrier, before stage 1 reads it.
     if constexpr (LTM > 0) E::template stage_tw_to_lds<LTM>(ltw, tw.stage, tid);
-#if NRFFT_PREFETCH
-    Ctx cn = ctx_of(tile);
-    cpx<T> vn[EPT];
-    load(vn, cn, fresh_tid(tid));
-#endif
+    constexpr bool PF = kc2c_pf<T, NT, ML, MS>();
+    Ctx cn{};
+    cpx<T> vn[PF ? EPT : 1];
+    if constexpr (PF) {
+        cn = ctx_of(tile);
+        load(vn, cn, fresh_tid(tid));
+    }
     // The tile loop, with plain or non-temporal result stores (NTS): a
     // runtime choice between two stores of the same value would be merged
     // into one plain store by the compiler.
@@ -138,24 +147,22 @@ k_c2c(const cpx<T>* in, cpx<T>* out, PassDesc d, Twiddles<T> tw) {
         bool first = true;
         for (;;) {
             cpx<T> v[EPT];
-#if NRFFT_PREFETCH
-#pragma unroll
-            for (int e = 0; e < EPT; ++e) v[e] = vn[e];
-            const Ctx c = cn;
             const int64_t next = tile + G;
             const bool has_next = next < ntiles;
-            if (has_next) {
-                cn = ctx_of(next);
-                load(vn, cn, fresh_tid(tid));
+            Ctx c;
+            if constexpr (PF) {
+#pragma unroll
+                for (int e = 0; e < EPT; ++e) v[e] = vn[e];
+                c = cn;
+                if (has_next) {
+                    cn = ctx_of(next);
+                    load(vn, cn, fresh_tid(tid));
+                }
+            } else {
+                c = ctx_of(tile);
+                load(v, c, fresh_tid(tid));
             }
             const int ltid = fresh_tid(tid);
-#else
-            const Ctx c = ctx_of(tile);
-            const int64_t next = tile + G;
-            const bool has_next = next < ntiles;
-            const int ltid = fresh_tid(tid);
-            load(v, c, ltid);
-#endif
             E::template compute<0>(v, ltid, tw.stage);
             E::template middle<1, LTM>(v, ltid, lds, tw.stage, !first, ltw);
             if (d.tw_logn) {

@@ -119,6 +119,97 @@ __device__ __forceinline__ T placed_response(const T* r, int64_t e, int64_t n, i
     return r[e - (n - mid)];
 }
 
+// Window of the placed response's support in complex indices (z_i = x_{2i}
+// + i x_{2i+1}, taken circularly): nonzero reals lie at e < m and e >= n -
+// mid, so z_i can be nonzero only for lo <= i < hi with lo = floor((n -
+// mid)/2) - L <= 0 and hi = ceil(m/2).  win_start is lo rounded down to a
+// multiple of 64; the window [win_start, win_start + 256) covers the support
+// when win_start + 256 >= hi.
+__host__ __device__ inline int resp_win_lo(int64_t L, int64_t m) { return int((2 * L - (m - 1) / 2) / 2 - L); }
+__host__ __device__ inline int resp_win_hi(int64_t m) { return int((m + 1) / 2); }
+__host__ __device__ inline int resp_win_start(int64_t L, int64_t m) {
+    const int lo = resp_win_lo(L, m);
+    return -(((-lo) + 63) / 64) * 64;
+}
+__host__ __device__ inline bool resp_win_fits(int64_t L, int64_t m) {
+    return resp_win_start(L, m) + 256 >= resp_win_hi(m) && -resp_win_lo(L, m) < L;
+}
+
+// Spectrum Z_k, k < L, of the placed response (Convolve.rs:41-63) of one
+// signal of n = 2L reals read as L complex values, left in LDS in natural
+// order (lds[E::lidx(0, k)], then a barrier), for a response whose support
+// fits the window [js, js + 256) (resp_win_fits; js a multiple of 64):
+//   Z_{k1 + Q k2} = W_256^{js k2} sum_{j < 256} (z_{js+j} W_L^{(js+j) k1}) W_256^{j k2},
+// Q = L / 256, W_M = exp(+2 pi i / M) (the forward transform).  The 256
+// window values are gathered once into zw (LDS); each k1 < Q is then one
+// 256-point DFT on the 16 lanes k1*16 .. k1*16+15 (one wave): lane g takes
+// j = 16 j2 + g, j2 < 16, with the twiddles W_L^{(js+g) k1 + 16 k1 j2} built
+// from five table values (twiddle_run's scheme); radix 16 over j2, twiddle
+// W_256^{g ka}, a 16 x 16 transpose through LDS inside the wave, radix 16
+// over j1, and the window factor W_256^{js k2} = i^{(js/64) k2}, which
+// depends on k2 mod 4 = ka mod 4 only.  This replaces log2 L stages over an
+// input that is zero outside 2 m of its 2 L reals.  Needs one transform per
+// group, NT = L / 16 threads, 4096 <= L <= 8192 (NT >= 256 gathers the
+// window); the LDS buffer holds Q * 272 values (it holds L + L/16 + 1).
+template <class E, typename T, int L>
+__device__ __forceinline__ void window_response_spectrum(const T* r, int64_t m, int js, cpx<T>* lds, cpx<T>* zw,
+                                                         int tid, const Twiddles<T>& tw) {
+    constexpr int Q = L / 256;
+    constexpr int64_t N = 2 * int64_t(L);
+    constexpr uint32_t CS = TW_COARSE / L;  // W_L^e = coarse[e CS]
+    static_assert(L % 256 == 0 && L >= 4096 && L <= TW_COARSE && E::LDS_ELEMS >= Q * 272, "window spectrum layout");
+    {
+        const int lo = resp_win_lo(L, m), hi = resp_win_hi(m);
+        const int i = js + tid;
+        cpx<T> z{(T)0, (T)0};
+        if (tid < 256 && i >= lo && i < hi) {
+            const int ii = i < 0 ? i + L : i;
+            z = cpx<T>{placed_response(r, 2 * int64_t(ii), N, m), placed_response(r, 2 * int64_t(ii) + 1, N, m)};
+        }
+        if (tid < 256) zw[tid] = z;
+    }
+    const int k1 = tid >> 4, g = tid & 15;
+    // W_L^{m0 + j2 m1}, j2 < 16: m0 = (js + g) k1, m1 = 16 k1 (mod L).
+    const uint32_t mask = uint32_t(L - 1);
+    const uint32_t m0 = uint32_t((js + g + L) * k1) & mask, m1 = uint32_t(16 * k1) & mask;
+    cpx<T> p[4], w[16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) p[q] = tw.coarse[((m1 << q) & mask) * CS];
+    w[0] = tw.coarse[m0 * CS];
+#pragma unroll
+    for (int j2 = 1; j2 < 16; ++j2) {
+        const int lb = j2 & (-j2);
+        w[j2] = cmul(w[j2 - lb], p[ilog2c(lb)]);
+    }
+    lds_barrier();  // zw published
+    cpx<T> v[16];
+#pragma unroll
+    for (int j2 = 0; j2 < 16; ++j2) v[j2] = cmul(zw[16 * j2 + g], w[j2]);
+    dft16<1>(v);  // v[ka] = sum_j2 a_{16 j2 + g} W_16^{j2 ka}
+    // Row g of the 256-point stage-1 table: W_256^{g ka}, ka < 16.
+    const cpx<T>* row = tw.stage + 2 * 256 + StageTable<256>::off(1) + g * 16;
+#pragma unroll
+    for (int ka = 1; ka < 16; ++ka) v[ka] = cmul(v[ka], row[ka]);
+    // Transpose [j1 = g][ka] -> [ka][j1] in this group's 16 x 17 region; the
+    // 16 lanes are in one wave, whose LDS operations complete in order.
+    cpx<T>* tr = lds + k1 * 272;
+#pragma unroll
+    for (int ka = 0; ka < 16; ++ka) tr[g * 17 + ka] = v[ka];
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#pragma unroll
+    for (int j1 = 0; j1 < 16; ++j1) v[j1] = tr[j1 * 17 + g];
+    dft16<1>(v);  // v[kb] = Y[g + 16 kb]
+    const int rot = ((js / 64) * g) & 3;  // i^rot
+    lds_barrier();  // every group has read its transpose region
+#pragma unroll
+    for (int kb = 0; kb < 16; ++kb) {
+        const cpx<T> y = v[kb];
+        const cpx<T> o = rot == 0 ? y : rot == 1 ? cpx<T>{-y.y, y.x} : rot == 2 ? cpx<T>{-y.x, -y.y} : cpx<T>{y.y, -y.x};
+        lds[E::lidx(0, k1 + Q * (g + 16 * kb))] = o;
+    }
+    lds_barrier();
+}
+
 }  // namespace nrfft
 
 namespace nrfft {
@@ -227,6 +318,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(conv_waves<T
     constexpr int LTM = EF::NST >= 2 ? EF::NST - 2 : 0;
     __shared__ cpx<T> lds[EF::LDS_ELEMS];
     __shared__ __align__(16) cpx<T> ltw[EF::tw_elems(LTM) > 0 ? EF::tw_elems(LTM) : 1];
+    // The response's spectrum from its 256-value window (a short response).
+    constexpr bool WINDOW = TI == 1 && NT * 16 == L && L >= 4096 && L <= TW_COARSE;
+    __shared__ cpx<T> zw[WINDOW ? 256 : 1];
     const int tid = threadIdx.x;
     const int64_t s0 = int64_t(blockIdx.x) * TI;
     const int nvalid = (nsig - s0) < TI ? int(nsig - s0) : TI;
@@ -270,11 +364,21 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(conv_waves<T
         }, false, ltw);
     } else {
         const T* rb = second + s0 * m;
-        EF::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
-            if (TI > 1 && t >= nvalid) return cpx<T>{(T)0, (T)0};
-            const T* r = rb + t * m;
-            return cpx<T>{placed_response(r, 2 * int64_t(i), N, m), placed_response(r, 2 * int64_t(i) + 1, N, m)};
-        }, false, ltw);
+        bool done = false;
+        if constexpr (WINDOW) {
+            // A short response: its spectrum from the window it occupies.
+            if (resp_win_fits(L, m)) {
+                window_response_spectrum<EF, T, L>(rb, m, resp_win_start(L, m), lds, zw, tid, tw);
+                done = true;
+            }
+        }
+        if (!done) {
+            EF::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
+                if (TI > 1 && t >= nvalid) return cpx<T>{(T)0, (T)0};
+                const T* r = rb + t * m;
+                return cpx<T>{placed_response(r, 2 * int64_t(i), N, m), placed_response(r, 2 * int64_t(i) + 1, N, m)};
+            }, false, ltw);
+        }
     }
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {

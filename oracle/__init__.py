@@ -18,7 +18,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _SO = os.path.join(_HERE, "liborc.so")
+_SO_NATIVE = os.path.join(_HERE, "liborc_native.so")
 _lib = None
+_lib_native = None
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _szp = ctypes.POINTER(ctypes.c_size_t)
@@ -29,41 +31,60 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def native_lib():
+    """The oracle built with -march=native on this host (make -C oracle
+    native), for the CPU-baseline timings; None if it cannot be built here."""
+    global _lib_native
+    if _lib_native is None:
+        try:
+            subprocess.run(["make", "-s", "-C", _HERE, "native"], check=True, timeout=120,
+                           capture_output=True)
+            _lib_native = _load(_SO_NATIVE)
+        except (OSError, subprocess.SubprocessError):
+            return None
+    return _lib_native
+
+
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
         if not os.path.exists(_SO):
             raise RuntimeError(f"oracle library missing: {_SO} (run `make -C oracle`)")
-        L = ctypes.CDLL(_SO)
-        sz, i32, u64 = ctypes.c_size_t, ctypes.c_int, ctypes.c_uint64
-        sig = {
-            "orc_gen_uniform": (None, [u64, sz, _dp]),
-            "orc_gen_normal": (None, [u64, sz, _dp]),
-            "orc_gen_skip": (u64, [u64, sz]),
-            "orc_four1_literal": (None, [_dp, sz, i32]),
-            "orc_four1_nr": (None, [_dp, sz, i32]),
-            "orc_dft_brute": (None, [_dp, sz, i32, _dp]),
-            "orc_realft": (None, [_dp, sz, i32]),
-            "orc_realft_literal": (None, [_dp, sz, i32]),
-            "orc_fourn": (i32, [_dp, _szp, sz, i32]),
-            "orc_prepare_response": (None, [_dp, _dp, sz, sz]),
-            "orc_convlv": (i32, [_dp, sz, _dp, sz, i32, _dp]),
-            "orc_convlv_direct": (i32, [_dp, sz, _dp, sz, _dp]),
-            "orc_correl": (i32, [_dp, _dp, sz, _dp]),
-            "orc_correl_direct_circular": (None, [_dp, _dp, sz, _dp]),
-            "orc_four1_batch": (i32, [_dp, sz, sz, i32, i32]),
-            "orc_twofft": (None, [_dp, _dp, sz, _dp, _dp]),
-            "orc_cosft1": (None, [_dp, sz]),
-            "orc_cosft2": (None, [_dp, sz, i32]),
-            "orc_rlft3": (i32, [_dp, _dp, sz, sz, sz, i32]),
-            "orc_convlv_batch": (i32, [_dp, sz, _dp, sz, sz, i32, _dp, i32]),
-        }
-        for name, (res, args) in sig.items():
-            f = getattr(L, name)
-            f.restype = res
-            f.argtypes = args
-        _lib = L
+        _lib = _load(_SO)
     return _lib
+
+
+def _load(path: str) -> ctypes.CDLL:
+    L = ctypes.CDLL(path)
+    sz, i32, u64 = ctypes.c_size_t, ctypes.c_int, ctypes.c_uint64
+    sig = {
+        "orc_gen_uniform": (None, [u64, sz, _dp]),
+        "orc_gen_normal": (None, [u64, sz, _dp]),
+        "orc_gen_skip": (u64, [u64, sz]),
+        "orc_four1_literal": (None, [_dp, sz, i32]),
+        "orc_four1_nr": (None, [_dp, sz, i32]),
+        "orc_dft_brute": (None, [_dp, sz, i32, _dp]),
+        "orc_realft": (None, [_dp, sz, i32]),
+        "orc_realft_literal": (None, [_dp, sz, i32]),
+        "orc_fourn": (i32, [_dp, _szp, sz, i32]),
+        "orc_prepare_response": (None, [_dp, _dp, sz, sz]),
+        "orc_convlv": (i32, [_dp, sz, _dp, sz, i32, _dp]),
+        "orc_convlv_direct": (i32, [_dp, sz, _dp, sz, _dp]),
+        "orc_correl": (i32, [_dp, _dp, sz, _dp]),
+        "orc_correl_direct_circular": (None, [_dp, _dp, sz, _dp]),
+        "orc_four1_batch": (i32, [_dp, sz, sz, i32, i32]),
+        "orc_twofft": (None, [_dp, _dp, sz, _dp, _dp]),
+        "orc_cosft1": (None, [_dp, sz]),
+        "orc_cosft2": (None, [_dp, sz, i32]),
+        "orc_rlft3": (i32, [_dp, _dp, sz, sz, sz, i32]),
+        "orc_convlv_batch": (i32, [_dp, sz, _dp, sz, sz, i32, _dp, i32]),
+        "orc_uneven_batch": (i32, [_dp, ctypes.POINTER(i32), _szp, _szp, sz, i32, i32]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
 
 
 def _f64(a) -> np.ndarray:
@@ -195,14 +216,38 @@ def correl_direct_circular(d1, d2) -> np.ndarray:
 
 
 # ---- CPU baseline drivers ---------------------------------------------------
-def four1_batch_inplace(data: np.ndarray, nn: int, batch: int, isign: int, nthreads: int = 0) -> int:
+# `L` selects the library (default: the portable build; native_lib() for
+# timings).  Each returns the OpenMP threads used.
+def four1_batch_inplace(data: np.ndarray, nn: int, batch: int, isign: int, nthreads: int = 0, L=None) -> int:
     assert data.dtype == np.float64 and data.flags.c_contiguous and data.size == 2 * nn * batch
-    return int(lib().orc_four1_batch(_p(data), nn, batch, isign, nthreads))
+    return int((L or lib()).orc_four1_batch(_p(data), nn, batch, isign, nthreads))
 
 
 def convlv_batch(data: np.ndarray, n: int, respns: np.ndarray, m: int, batch: int, isign: int,
-                 out: np.ndarray, nthreads: int = 0) -> int:
-    return int(lib().orc_convlv_batch(_p(data), n, _p(respns), m, batch, isign, _p(out), nthreads))
+                 out: np.ndarray, nthreads: int = 0, L=None) -> int:
+    return int((L or lib()).orc_convlv_batch(_p(data), n, _p(respns), m, batch, isign, _p(out), nthreads))
+
+
+def uneven_batch_inplace(data: np.ndarray, items, offsets, isign: int, nthreads: int = 0, L=None) -> int:
+    """Items (N,) through the literal four1, (n0, n1) through fourn, in place
+    at complex offsets `offsets` of data; parallel over items."""
+    assert data.dtype == np.float64 and data.flags.c_contiguous
+    n = len(items)
+    nd = (ctypes.c_int * n)(*[len(it) for it in items])
+    dims = (ctypes.c_size_t * (2 * n))(*[v for it in items for v in (it[0], it[1] if len(it) > 1 else 1)])
+    offs = (ctypes.c_size_t * n)(*[int(o) for o in offsets])
+    return int((L or lib()).orc_uneven_batch(_p(data), nd, dims, offs, n, isign, nthreads))
+
+
+def four1_literal_with(L, data: np.ndarray, isign: int) -> None:
+    """In-place literal four1 through library L (timing driver)."""
+    (L or lib()).orc_four1_literal(_p(data), data.size // 2, isign)
+
+
+def fourn_with(L, data: np.ndarray, nn, isign: int) -> None:
+    """In-place NR fourn through library L (timing driver)."""
+    arr = (ctypes.c_size_t * len(nn))(*[int(v) for v in nn])
+    _chk((L or lib()).orc_fourn(_p(data), arr, len(nn), isign))
 
 
 def twofft(d1, d2):

@@ -806,6 +806,37 @@ int orc_convlv_batch(const double *data, size_t n, const double *respns, size_t 
     return used;
 }
 
+/* CPU baseline of an uneven batch (FFTProcessor::fft_batch, FFT_1.rs:185-189,
+ * parallel over items): item i (ndim[i] = 1 or 2, dims dims[2i], dims[2i+1])
+ * at complex offset offs[i] of data, 1-D items through the literal four1,
+ * 2-D items through fourn.  Returns the threads used. */
+int orc_uneven_batch(double *data, const int *ndim, const size_t *dims, const size_t *offs, size_t count, int isign,
+                     int nthreads) {
+    int used = 1;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+    {
+#pragma omp single
+        used = omp_get_num_threads();
+#pragma omp for schedule(dynamic, 1)
+        for (long i = 0; i < (long)count; ++i) {
+            double *d = data + 2 * offs[i];
+            if (ndim[i] == 1) orc_four1_literal(d, dims[2 * i], isign);
+            else (void)orc_fourn(d, dims + 2 * i, 2, isign);
+        }
+    }
+#else
+    (void)nthreads;
+    for (size_t i = 0; i < count; ++i) {
+        double *d = data + 2 * offs[i];
+        if (ndim[i] == 1) orc_four1_literal(d, dims[2 * i], isign);
+        else (void)orc_fourn(d, dims + 2 * i, 2, isign);
+    }
+#endif
+    return used;
+}
+
 int orc_fourn_threads(double *data, const size_t *nn, size_t ndim, int isign, int nthreads) {
     /* fourn itself is serial (NR); threads only matter for the batch drivers. */
     (void)nthreads;

@@ -386,8 +386,8 @@ def test_convlv_golden(golden):
     assert rel_l2(dec, oracle.convlv(golden["convlv_1024_257_out"], r, -1)) < 1e-10
 
 
-@pytest.mark.parametrize("n,m", [(2, 1), (2, 2), (4, 3), (8, 8), (64, 7), (1024, 257), (16384, 257), (32768, 101),
-                                 (65536, 1000)])
+@pytest.mark.parametrize("n,m", [(2, 1), (2, 2), (4, 3), (8, 8), (64, 7), (1024, 257), (4096, 257), (8192, 1),
+                                 (8192, 100), (8192, 383), (16384, 257), (16384, 511), (32768, 101), (65536, 1000)])
 @pytest.mark.parametrize("isign", [1, -1])
 def test_convlv_f64(n, m, isign):
     d = synth.normal(n + m, n)
