@@ -140,6 +140,21 @@ int nrfft_execute_convlv(nrfft_plan plan, const void* d_data, const void* d_resp
 int nrfft_plan_correl(nrfft_plan* plan, int dtype, size_t n, size_t batch);
 int nrfft_execute_correl(nrfft_plan plan, const void* d_data1, const void* d_data2, void* d_ans, void* stream);
 
+/* Allocation-free batched four1 on device pointers.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same transform, kernels and default tunings as nrfft_plan_c2c(dtype, {nn},
+ * 1, batch) + nrfft_execute_c2c, but no plan: the four-step intermediate of
+ * nn > 8192 lives in a 64 MiB device array built into the library (one per
+ * GPU, held until the process ends), so nn * sizeof(complex) must not exceed
+ * 64 MiB (FP32 nn <= 2^23, FP64 nn <= 2^22; larger -> NRFFT_E_UNSUPPORTED).
+ * nrfft_init(device) must have run on the current device (else
+ * NRFFT_E_INVALID); the call then calls no allocating routine.  Calls that
+ * overlap in time on one GPU share that array and race; calls ordered on one
+ * stream do not.  In place allowed (d_out == d_in). */
+int nrfft_execute_four1_batch_noalloc(int dtype, const void* d_in, void* d_out, size_t nn, size_t batch, int isign,
+                                      void* stream);
+
 /* Profiling aid (no counterpart in the reference): enqueue only launches
  * [first, first + count) of a C2C plan, e.g. one pass of a four-step plan. */
 int nrfft_execute_c2c_launches(nrfft_plan plan, const void* d_in, void* d_out, int isign, void* stream, int first,

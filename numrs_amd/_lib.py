@@ -104,6 +104,7 @@ _SIGS = {
     "nrfft_fft_batch_f32": (_i, [ctypes.POINTER(_fp), _szp, _sz, _i]),
     "nrfft_gen_uniform_f32": (_i, [_vp, _sz, ctypes.c_uint64, ctypes.c_uint64, _vp]),
     "nrfft_gen_uniform_f64": (_i, [_vp, _sz, ctypes.c_uint64, ctypes.c_uint64, _vp]),
+    "nrfft_execute_four1_batch_noalloc": (_i, [_i, _vp, _vp, _sz, _sz, _i, _vp]),
     "nrfft_plan_destroy": (_i, [_vp]),
     "nrfft_plan_workspace_bytes": (_sz, [_vp]),
     "nrfft_plan_num_launches": (_i, [_vp]),

@@ -675,7 +675,8 @@ int nrfft_init(int device) {
     g_err.clear();
     HIPCHK(hipSetDevice(device));
     DeviceCtx* c = nullptr;
-    return get_ctx(&c);
+    if (int rc = get_ctx(&c)) return rc;
+    return noalloc_setup(device);
 }
 
 static int check_four1(const void* data, size_t nn, int isign) {

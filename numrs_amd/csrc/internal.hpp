@@ -42,6 +42,9 @@ inline Twiddles<T> ctx_twiddles(const DeviceCtx* c) {
 }
 int get_ctx(DeviceCtx** out);
 int find_ctx(DeviceCtx** out);
+// Per-device set-up of the allocation-free entry point's scratch (its
+// device address); called by nrfft_init with the device current.
+int noalloc_setup(int device);
 
 // Buffers a launch reads or writes.
 // B_AUX2: a second caller buffer (twofft's second spectrum, rlft3's speq).
