@@ -79,17 +79,17 @@ def time_events(fn, reps: int) -> float:
 
 
 def measured_traffic_ratio():
-    """HBM bytes / algorithmic bytes of the FP32 four-step passes, from the
-    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated)."""
+    """Bytes past L2 / algorithmic bytes of the FP32 four-step passes, from the
+    committed rocprofv3 summary of raw TCC request counts (last column)."""
     path = os.path.join(ROOT, "profiles", "c2c_fp32_batch64_n2p20_traffic.txt")
     try:
         ratios = []
         for line in open(path):
             parts = line.split()
-            if len(parts) >= 5 and ("nrfft::k_c2c<float," in line or "nrfft::k_c2c_dma<" in line) and parts[0].isdigit():
-                ratios.append(float(parts[3]) / 256.0)
+            if line.startswith(" nrfft::k_c2c_dma<1, 512>") or line.startswith(" nrfft::k_c2c<float, 1, 2048"):
+                ratios.append(float(parts[-1]))
         return (max(ratios), os.path.relpath(path, ROOT)) if ratios else (None, None)
-    except OSError:
+    except (OSError, ValueError):
         return None, None
 
 
@@ -207,9 +207,9 @@ def headline(args) -> dict:
                           "note": "compulsory bytes (one read + one write of the batch) / whole step"},
         "traffic_rate": ({"GBps": round(ratio * launch_bytes * nl / (ms * 1e-3) / 1e9, 1),
                           "frac": round(ratio * launch_bytes * nl / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                          "note": "counter-measured bytes past L2 per step (rocprofv3 FETCH_SIZE + WRITE_SIZE of "
-                                  "every launch, from traffic_source) / whole step; each of the two passes "
-                                  "reads and writes every point"} if ratio else None),
+                          "note": "counter-measured bytes past L2 per step (rocprofv3 TCC read / write request "
+                                  "counts of every launch, from traffic_source) / whole step; each of the two "
+                                  "passes reads and writes every point"} if ratio else None),
         "launch_ms": [round(p, 4) for p in per],
     }
     # FP64 reference of the same FP32 inputs from the FP64 path (parity-tested
@@ -457,9 +457,8 @@ def _extra_four1() -> dict:
     x = x.double()
     y = torch.empty_like(x)
     plan = numrs_amd.C2CPlan("float64", [n], 1)
-    for _ in range(5):
-        plan.execute(x, y, 1)
-    ms = time_events(lambda: plan.execute(x, y, 1), 50)
+    _settle(lambda: plan.execute(x, y, 1), 0.3)
+    ms = time_events(lambda: plan.execute(x, y, 1), 200)
     res["four1_f64_2p16"] = {"us": round(ms * 1e3, 2), "GFLOPs": round(5 * n * 16 / (ms * 1e-3) / 1e9, 2),
                              "GBps": round(2 * n * 16 / (ms * 1e-3) / 1e9, 1)}
     del plan, x, y
@@ -552,8 +551,7 @@ def _extra_convlv() -> dict:
     r = torch.from_numpy(synth.gaussian_windowed_responses(2, b, m, 40.0, start=b * n).astype(np.float32)).cuda()
     a = torch.empty_like(d)
     plan = numrs_amd.ConvlvPlan("float32", n, m, b)
-    for _ in range(3):
-        plan.execute(d, r, a, 1)
+    _settle(lambda: plan.execute(d, r, a, 1), 0.5)
     ms = time_events(lambda: plan.execute(d, r, a, 1), 20)
     byt = (2.0 * n + m) * 4 * b
     res["convlv_f32_4096x2p14_m257"] = {
@@ -569,8 +567,7 @@ def _extra_convlv() -> dict:
     x = torch.empty(b * n, dtype=torch.float32, device="cuda")
     fill_normal(x, 2, b * n)
     plan = numrs_amd.RealFTPlan("float32", n, b)
-    for _ in range(3):
-        plan.execute(x, 1)
+    _settle(lambda: plan.execute(x, 1), 0.5)
     ms = time_events(lambda: plan.execute(x, 1), 20)
     byt = 2.0 * n * 4 * b
     res["realft_f32_64x2p21_fwd"] = {
@@ -607,7 +604,7 @@ def _extra_spectral() -> dict:
         f1 = torch.empty(2 * b * n, dtype=f32, device="cuda")
         f2 = torch.empty_like(f1)
         p = numrs_amd.TwofftPlan("float32", n, b)
-        p.execute(d1, d2, f1, f2)
+        _settle(lambda: p.execute(d1, d2, f1, f2), 0.3)
         ms = time_events(lambda: p.execute(d1, d2, f1, f2), 10)
         res[f"twofft_f32_{b}x2p{n.bit_length() - 1}"] = entry(
             ms, (2 * n + 4 * n) * 4.0 * b, 5.0 * n * math.log2(n) * b)
@@ -617,7 +614,7 @@ def _extra_spectral() -> dict:
         numrs_amd.gen_uniform(x, 42)
         y = torch.empty_like(x)
         p = numrs_amd.Cosft1Plan("float32", n, b)
-        p.execute(x, y)
+        _settle(lambda: p.execute(x, y), 0.3)
         ms = time_events(lambda: p.execute(x, y), 10)
         res[f"cosft1_f32_{b}x2p{n.bit_length() - 1}"] = entry(ms, 2.0 * (n + 1) * 4 * b)
         del p, x, y
@@ -627,7 +624,7 @@ def _extra_spectral() -> dict:
         y = torch.empty_like(x)
         p = numrs_amd.Cosft2Plan("float32", n, b)
         for isign, tag in ((1, "fwd"), (-1, "inv")):
-            p.execute(x, y, isign)
+            _settle(lambda: p.execute(x, y, isign), 0.3)
             ms = time_events(lambda: p.execute(x, y, isign), 10)
             res[f"cosft2_{tag}_f32_{b}x2p{n.bit_length() - 1}"] = entry(ms, 2.0 * n * 4 * b)
         del p, x, y
@@ -637,7 +634,7 @@ def _extra_spectral() -> dict:
     numrs_amd.gen_uniform(x, 44)
     sp = torch.empty(2 * 256 * 256, dtype=f32, device="cuda")
     p = numrs_amd.Rlft3Plan("float32", *shape, 1)
-    p.execute(x, sp, 1)
+    _settle(lambda: p.execute(x, sp, 1), 0.3)
     ms = time_events(lambda: p.execute(x, sp, 1), 10)
     res["rlft3_f32_256cubed_fwd"] = entry(ms, 2.0 * vol * 4 + sp.numel() * 4, 2.5 * vol * 24)
     del p, x, sp
@@ -703,10 +700,9 @@ def _extra_uneven() -> dict:
     x = torch.empty(2 * total, dtype=torch.float32, device="cuda")
     numrs_amd.gen_uniform(x, 4)
     y = torch.empty_like(x)
-    for _ in range(2):
-        plan.execute(x, y, 1)
+    _settle(lambda: plan.execute(x, y, 1), 0.5)
     plan.check()
-    ms = time_events(lambda: plan.execute(x, y, 1), 5)
+    ms = time_events(lambda: plan.execute(x, y, 1), 10)
     plan.check()
     flops = sum(5.0 * int(np.prod(it)) * math.log2(int(np.prod(it))) for it in items)
     byt = 2.0 * total * 8
