@@ -203,7 +203,7 @@ __host__ __device__ constexpr int ilog2c(long long v) { return v <= 1 ? 0 : 1 + 
 #endif
 // One pad slot per NRFFT_LDS_PAD complex values of a padded LDS tile.
 #ifndef NRFFT_LDS_PAD
-#define NRFFT_LDS_PAD 16
+#define NRFFT_LDS_PAD 32
 #endif
 
 __host__ __device__ constexpr int plan_radix(int L, int LOG, int NST, int s) {
@@ -316,11 +316,14 @@ struct Engine {
     static constexpr int LAYOUT = (NST == 1) ? MAPL : MAPS;
     static constexpr int map_of(int s) { return s == 0 ? MAPL : MAPS; }
 
-    // Padded LDS index of element i of tile transform t: one pad slot per 16
-    // complex values, so strides that are multiples of 16 land in distinct
-    // banks (MAP_T layout [i][TI]: lanes across t are contiguous, lanes along
-    // i are 17 * TI / 16 apart; MAP_J layout [t][L]: lanes along i are
-    // contiguous, stride-16 runs are 17 apart).
+    // Padded LDS index of element i of tile transform t: one pad slot per PAD
+    // complex values (NRFFT_LDS_PAD, 32 by default), so strides that are
+    // multiples of PAD land in distinct banks (MAP_T layout [i][TI]: lanes
+    // across t are contiguous, lanes along i are (PAD + 1) TI / PAD apart;
+    // MAP_J layout [t][L]: lanes along i are contiguous, stride-PAD runs are
+    // PAD + 1 apart).  Measured on MI355X (64 x 2^20 FP32): PAD 32 0.441 ms,
+    // 16 0.452 ms, 8 0.491 ms (the 2048-point pass B 53.8 / 56.4 / 63.1 us);
+    // convlv 4096 x 2^14 0.306 / 0.318 / 0.318 ms.
     // With NRFFT_LDS_SWIZZLE the MAP_T layout is unpadded and XOR-swizzles the
     // column with a hash of the row instead (no affine offsets).
     static constexpr int LSI = (LAYOUT == MAP_T) ? TI : 1;  // raw distance between i and i + 1

@@ -150,14 +150,14 @@ __host__ __device__ inline bool resp_win_fits(int64_t L, int64_t m) {
 // depends on k2 mod 4 = ka mod 4 only.  This replaces log2 L stages over an
 // input that is zero outside 2 m of its 2 L reals.  Needs one transform per
 // group, NT = L / 16 threads, 4096 <= L <= 8192 (NT >= 256 gathers the
-// window); the LDS buffer holds Q * 272 values (it holds L + L/16 + 1).
+// window); the transposes use Q * 256 = L values of the LDS buffer.
 template <class E, typename T, int L>
 __device__ __forceinline__ void window_response_spectrum(const T* r, int64_t m, int js, cpx<T>* lds, cpx<T>* zw,
                                                          int tid, const Twiddles<T>& tw) {
     constexpr int Q = L / 256;
     constexpr int64_t N = 2 * int64_t(L);
     constexpr uint32_t CS = TW_COARSE / L;  // W_L^e = coarse[e CS]
-    static_assert(L % 256 == 0 && L >= 4096 && L <= TW_COARSE && E::LDS_ELEMS >= Q * 272, "window spectrum layout");
+    static_assert(L % 256 == 0 && L >= 4096 && L <= TW_COARSE && E::LDS_ELEMS >= Q * 256, "window spectrum layout");
     {
         const int lo = resp_win_lo(L, m), hi = resp_win_hi(m);
         const int i = js + tid;
@@ -190,14 +190,16 @@ __device__ __forceinline__ void window_response_spectrum(const T* r, int64_t m, 
     const cpx<T>* row = tw.stage + 2 * 256 + StageTable<256>::off(1) + g * 16;
 #pragma unroll
     for (int ka = 1; ka < 16; ++ka) v[ka] = cmul(v[ka], row[ka]);
-    // Transpose [j1 = g][ka] -> [ka][j1] in this group's 16 x 17 region; the
-    // 16 lanes are in one wave, whose LDS operations complete in order.
-    cpx<T>* tr = lds + k1 * 272;
+    // Transpose [j1 = g][ka] -> [ka][j1] in this group's 16 x 16 region, the
+    // column index XOR-ed with the row so that 16 lanes writing one column or
+    // reading one row touch 16 distinct banks; the 16 lanes are in one wave,
+    // whose LDS operations complete in order.
+    cpx<T>* tr = lds + k1 * 256;
 #pragma unroll
-    for (int ka = 0; ka < 16; ++ka) tr[g * 17 + ka] = v[ka];
+    for (int ka = 0; ka < 16; ++ka) tr[g * 16 + (ka ^ g)] = v[ka];
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
 #pragma unroll
-    for (int j1 = 0; j1 < 16; ++j1) v[j1] = tr[j1 * 17 + g];
+    for (int j1 = 0; j1 < 16; ++j1) v[j1] = tr[j1 * 16 + (g ^ j1)];
     dft16<1>(v);  // v[kb] = Y[g + 16 kb]
     const int rot = ((js / 64) * g) & 3;  // i^rot
     lds_barrier();  // every group has read its transpose region
