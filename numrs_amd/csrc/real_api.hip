@@ -172,6 +172,7 @@ struct RealRef {
     const void* fwd;
     const void* inv;
     const void* conv;
+    const void* conv_win;  // windowed-response convlv, or nullptr
     int nt, ti;
 };
 
@@ -181,9 +182,11 @@ RealRef real_ref_one() {
     constexpr int TI = ti_row<T, L>();
     constexpr int EPT = L < 16 ? L : 16;
     constexpr int NT = TI * L / EPT;
+    const void* win = nullptr;
+    if constexpr (conv_has_window<T, L, TI, NT>()) win = reinterpret_cast<const void*>(&k_convlv<T, L, TI, NT, 1>);
     return RealRef{reinterpret_cast<const void*>(&k_realft<T, L, TI, NT, 1>),
                    reinterpret_cast<const void*>(&k_realft<T, L, TI, NT, -1>),
-                   reinterpret_cast<const void*>(&k_convlv<T, L, TI, NT>), NT, TI};
+                   reinterpret_cast<const void*>(&k_convlv<T, L, TI, NT, 0>), win, NT, TI};
 }
 
 template <typename T, int... LS>
@@ -195,7 +198,7 @@ template <typename T>
 struct RealTable {
     RealRef t[REAL_MAX_LOGL + 1];
     RealTable() {
-        t[0] = RealRef{nullptr, nullptr, nullptr, 0, 0};
+        t[0] = RealRef{nullptr, nullptr, nullptr, nullptr, 0, 0};
         real_fill<T>(t, std::make_integer_sequence<int, REAL_MAX_LOGL>{});
     }
 };
@@ -342,7 +345,8 @@ static int exec_real_t(const Plan& p, const Launch& l, void* const* bufs, int is
             void* aux = bufs[B_AUX];
             void* args[] = {&src, &aux, &dst, &nsig, &m, &op, &tw};
             unsigned grid = (unsigned)((nsig + r.ti - 1) / r.ti);
-            HIPCHK(hipLaunchKernel(r.conv, dim3(grid), dim3(r.nt), args, 0, stream));
+            const bool win = r.conv_win != nullptr && op != OP_CORREL && resp_win_fits(a[1], m);
+            HIPCHK(hipLaunchKernel(win ? r.conv_win : r.conv, dim3(grid), dim3(r.nt), args, 0, stream));
             return NRFFT_OK;
         }
         case RS_UNTANGLE: {

@@ -176,20 +176,31 @@ __device__ __forceinline__ void window_response_spectrum(const T* r, int64_t m, 
 #pragma unroll
     for (int q = 0; q < 4; ++q) p[q] = tw.coarse[((m1 << q) & mask) * CS];
     w[0] = tw.coarse[m0 * CS];
+    lds_barrier();  // zw published
+    // Each w[j2] is used as soon as it is formed; only the parents of later
+    // entries stay live, which keeps this step's registers low while the
+    // caller holds the signal spectrum.
+    cpx<T> v[16];
+    v[0] = cmul(zw[g], w[0]);
 #pragma unroll
     for (int j2 = 1; j2 < 16; ++j2) {
         const int lb = j2 & (-j2);
         w[j2] = cmul(w[j2 - lb], p[ilog2c(lb)]);
+        v[j2] = cmul(zw[16 * j2 + g], w[j2]);
     }
-    lds_barrier();  // zw published
-    cpx<T> v[16];
-#pragma unroll
-    for (int j2 = 0; j2 < 16; ++j2) v[j2] = cmul(zw[16 * j2 + g], w[j2]);
     dft16<1>(v);  // v[ka] = sum_j2 a_{16 j2 + g} W_16^{j2 ka}
     // Row g of the 256-point stage-1 table: W_256^{g ka}, ka < 16.
+    // The window factor i^{(js/64) k2} of output k2 = ka + 16 kb goes with
+    // column ka here: it is the same for the whole group (js is), so it costs
+    // a uniform select instead of a per-lane choice after the transpose.
     const cpx<T>* row = tw.stage + 2 * 256 + StageTable<256>::off(1) + g * 16;
+    const int jsq = js / 64;
 #pragma unroll
-    for (int ka = 1; ka < 16; ++ka) v[ka] = cmul(v[ka], row[ka]);
+    for (int ka = 1; ka < 16; ++ka) {
+        const cpx<T> y = cmul(v[ka], row[ka]);
+        const int rk = (jsq * ka) & 3;
+        v[ka] = rk == 0 ? y : rk == 1 ? cpx<T>{-y.y, y.x} : rk == 2 ? cpx<T>{-y.x, -y.y} : cpx<T>{y.y, -y.x};
+    }
     // Transpose [j1 = g][ka] -> [ka][j1] in this group's 16 x 16 region, the
     // column index XOR-ed with the row so that 16 lanes writing one column or
     // reading one row touch 16 distinct banks; the 16 lanes are in one wave,
@@ -200,15 +211,10 @@ __device__ __forceinline__ void window_response_spectrum(const T* r, int64_t m, 
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
 #pragma unroll
     for (int j1 = 0; j1 < 16; ++j1) v[j1] = tr[j1 * 16 + (g ^ j1)];
-    dft16<1>(v);  // v[kb] = Y[g + 16 kb]
-    const int rot = ((js / 64) * g) & 3;  // i^rot
+    dft16<1>(v);  // v[kb] = Z[k1 + Q (g + 16 kb)]
     lds_barrier();  // every group has read its transpose region
 #pragma unroll
-    for (int kb = 0; kb < 16; ++kb) {
-        const cpx<T> y = v[kb];
-        const cpx<T> o = rot == 0 ? y : rot == 1 ? cpx<T>{-y.y, y.x} : rot == 2 ? cpx<T>{-y.x, -y.y} : cpx<T>{y.y, -y.x};
-        lds[E::lidx(0, k1 + Q * (g + 16 * kb))] = o;
-    }
+    for (int kb = 0; kb < 16; ++kb) lds[E::lidx(0, k1 + Q * (g + 16 * kb))] = v[kb];
     lds_barrier();
 }
 
@@ -305,7 +311,19 @@ constexpr int conv_waves() {
     return (sizeof(T) == 4 && (L == 512 || L == 8192)) ? 4 : 1;
 }
 
+// Whether a (T, L, TI, NT) k_convlv has a windowed-response form (WIN = 1):
+// one transform per group, NT = L / 16 threads, 4096 <= L <= TW_COARSE.
 template <typename T, int L, int TI, int NT>
+constexpr bool conv_has_window() {
+    return TI == 1 && NT * 16 == L && L >= 4096 && L <= TW_COARSE;
+}
+
+// WIN = 1: OP_CONV / OP_DECONV with a response whose placed support fits the
+// 256-value window (resp_win_fits), the response spectrum from that window;
+// WIN = 0: every other case, the response (or data2) through a full FFT.  Two
+// instantiations, so the full FFT's registers do not count against the
+// windowed form, which holds the signal spectrum across its response step.
+template <typename T, int L, int TI, int NT, int WIN>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(conv_waves<T, L>()))) void k_convlv(const cpx<T>* data, const T* second, cpx<T>* ans, int64_t nsig,
                                                int64_t m, int op, Twiddles<T> tw) {
     using EF = Engine<T, 1, L, TI, NT, MAP_J, MAP_J>;
@@ -321,7 +339,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(conv_waves<T
     __shared__ cpx<T> lds[EF::LDS_ELEMS];
     __shared__ __align__(16) cpx<T> ltw[EF::tw_elems(LTM) > 0 ? EF::tw_elems(LTM) : 1];
     // The response's spectrum from its 256-value window (a short response).
-    constexpr bool WINDOW = TI == 1 && NT * 16 == L && L >= 4096 && L <= TW_COARSE;
+    constexpr bool WINDOW = WIN != 0;
+    static_assert(!WINDOW || conv_has_window<T, L, TI, NT>(), "windowed form needs one 8192/4096-point signal per group");
     __shared__ cpx<T> zw[WINDOW ? 256 : 1];
     const int tid = threadIdx.x;
     const int64_t s0 = int64_t(blockIdx.x) * TI;
@@ -359,28 +378,22 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(conv_waves<T
     lds_barrier();
 
     // 2. Spectrum of the response (or of data2).
-    if (op == OP_CORREL) {
+    if constexpr (WINDOW) {
+        // A short response: its spectrum from the window it occupies (the
+        // launch checked op != OP_CORREL and resp_win_fits).
+        window_response_spectrum<EF, T, L>(second + s0 * m, m, resp_win_start(L, m), lds, zw, tid, tw);
+    } else if (op == OP_CORREL) {
         const cpx<T>* sb = reinterpret_cast<const cpx<T>*>(second) + s0 * L;
         EF::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
             return (TI == 1 || t < nvalid) ? sb[t * L + i] : cpx<T>{(T)0, (T)0};
         }, false, ltw);
     } else {
         const T* rb = second + s0 * m;
-        bool done = false;
-        if constexpr (WINDOW) {
-            // A short response: its spectrum from the window it occupies.
-            if (resp_win_fits(L, m)) {
-                window_response_spectrum<EF, T, L>(rb, m, resp_win_start(L, m), lds, zw, tid, tw);
-                done = true;
-            }
-        }
-        if (!done) {
-            EF::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
-                if (TI > 1 && t >= nvalid) return cpx<T>{(T)0, (T)0};
-                const T* r = rb + t * m;
-                return cpx<T>{placed_response(r, 2 * int64_t(i), N, m), placed_response(r, 2 * int64_t(i) + 1, N, m)};
-            }, false, ltw);
-        }
+        EF::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
+            if (TI > 1 && t >= nvalid) return cpx<T>{(T)0, (T)0};
+            const T* r = rb + t * m;
+            return cpx<T>{placed_response(r, 2 * int64_t(i), N, m), placed_response(r, 2 * int64_t(i) + 1, N, m)};
+        }, false, ltw);
     }
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {
