@@ -8,9 +8,9 @@
 //   F_k = sum_j x_j e^{+2 pi i jk/n}                           (NR realft)
 // The untangle pairs bins k and L-k; it is fused into the kernel that runs
 // the FFT (the FFT result is left in LDS and one thread handles each pair).
-// convlv/correl keep the signal spectrum in registers while the response
-// spectrum is computed in the same LDS buffer, apply the spectrum operation,
-// untangle back and run the inverse FFT: R2C -> op -> C2R in one launch.
+// convlv/correl hold one operand's spectrum in registers while the other's
+// is computed in the same LDS buffer, apply the spectrum operation, untangle
+// back and run the inverse FFT: R2C -> op -> C2R in one launch.
 #pragma once
 #include "device_fft.hpp"
 
@@ -302,10 +302,11 @@ __global__ __launch_bounds__(NT) void k_realft(const cpx<T>* in, cpx<T>* out, in
 // ---------------------------------------------------------------------------
 // Waves per SIMD k_convlv is compiled for.  Capping the FP32 kernels at 4
 // (128 VGPRs) lets two 512-thread groups share a CU at L = 8192 (they need
-// 168 VGPRs otherwise): 0.42 -> 0.33 ms for 4096 x 2^14 on MI355X, with the
-// signal spectrum partly spilled across the response FFT; also faster at
-// L = 512.  At L = 32..128 the cap costs 33-41 spilled VGPRs and 25-35%, at
-// L = 1024..4096 it measured equal, so those keep the compiler's choice.
+// 153-168 VGPRs otherwise): with the windowed response formed first, 4096 x
+// 2^14 takes 0.258 ms on MI355X with the cap (3 VGPRs spilled) against
+// 0.346 ms without; also faster at L = 512.  At L = 32..128 the cap costs
+// 33-41 spilled VGPRs and 25-35%, at L = 1024..4096 it measured equal, so
+// those keep the compiler's choice.
 template <typename T, int L>
 constexpr int conv_waves() {
     return (sizeof(T) == 4 && (L == 512 || L == 8192)) ? 4 : 1;
@@ -358,82 +359,92 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(conv_waves<T
     const cpx<T> wt = twiddle_big<1>(tw, uint32_t(tid % NP), LOG2N);
     auto wpair = [&](int u) -> cpx<T> { return NP <= NT ? wt : cmul(wt, uw[u]); };
 
-    // 1. Spectrum of the signal, kept in registers pair by pair.
-    EF::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
-        return (TI == 1 || t < nvalid) ? db[t * L + i] : cpx<T>{(T)0, (T)0};
-    }, false, ltw);
-    cpx<T> xa[PPT], xb[PPT];
+    // The two forward transforms share the tile: the first one's spectrum is
+    // untangled pair by pair into registers and held while the second runs.
+    // With one signal per group the second operand (the response, or data2)
+    // goes first: its FFT or window step needs more registers than the
+    // signal's plain loads and holds nothing meanwhile (FP32 L = 8192: 8
+    // bytes of spill per lane instead of 88).  With several signals per group
+    // the placed response's loader for TI transforms weighs more, and the
+    // signal goes first.
+    constexpr bool SECOND_FIRST = TI == 1;
+    auto second_spectrum = [&]() {
+        if constexpr (WINDOW) {
+            // A short response: its spectrum from the window it occupies
+            // (the launch checked op != OP_CORREL and resp_win_fits).
+            window_response_spectrum<EF, T, L>(second + s0 * m, m, resp_win_start(L, m), lds, zw, tid, tw);
+        } else if (op == OP_CORREL) {
+            const cpx<T>* sb = reinterpret_cast<const cpx<T>*>(second) + s0 * L;
+            EF::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
+                return (TI == 1 || t < nvalid) ? sb[t * L + i] : cpx<T>{(T)0, (T)0};
+            }, false, ltw);
+        } else {
+            const T* rsp = second + s0 * m;
+            EF::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
+                if (TI > 1 && t >= nvalid) return cpx<T>{(T)0, (T)0};
+                const T* r = rsp + t * m;
+                return cpx<T>{placed_response(r, 2 * int64_t(i), N, m), placed_response(r, 2 * int64_t(i) + 1, N, m)};
+            }, false, ltw);
+        }
+    };
+    auto signal_spectrum = [&]() {
+        EF::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
+            return (TI == 1 || t < nvalid) ? db[t * L + i] : cpx<T>{(T)0, (T)0};
+        }, false, ltw);
+    };
+
+    // 1. The first spectrum, held untangled (bins p and L - p of pair p).
+    if constexpr (SECOND_FIRST) second_spectrum(); else signal_spectrum();
+    cpx<T> ha[PPT], hb[PPT];
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {
         const int q = tid + u * NT;
         if ((TI * NP) % NT != 0 && q >= TI * NP) continue;
-        const int t = q / NP, p = q % NP;
+        const int t = TI == 1 ? 0 : q / NP, p = TI == 1 ? q : q % NP;
         if (p == 0) {
-            xa[u] = dc_forward(lds[EF::lidx(t, 0)]);
-            xb[u] = lds[EF::lidx(t, NP)];
+            ha[u] = dc_forward(lds[EF::lidx(t, 0)]);
+            hb[u] = lds[EF::lidx(t, NP)];
         } else {
-            untangle_pair(lds[EF::lidx(t, p)], lds[EF::lidx(t, L - p)], wpair(u), (T)-0.5, xa[u], xb[u]);
+            untangle_pair(lds[EF::lidx(t, p)], lds[EF::lidx(t, L - p)], wpair(u), (T)-0.5, ha[u], hb[u]);
         }
     }
-    lds_barrier();
+    lds_barrier();  // every value is read before the next FFT overwrites the tile
 
-    // 2. Spectrum of the response (or of data2).
-    if constexpr (WINDOW) {
-        // A short response: its spectrum from the window it occupies (the
-        // launch checked op != OP_CORREL and resp_win_fits).
-        window_response_spectrum<EF, T, L>(second + s0 * m, m, resp_win_start(L, m), lds, zw, tid, tw);
-    } else if (op == OP_CORREL) {
-        const cpx<T>* sb = reinterpret_cast<const cpx<T>*>(second) + s0 * L;
-        EF::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
-            return (TI == 1 || t < nvalid) ? sb[t * L + i] : cpx<T>{(T)0, (T)0};
-        }, false, ltw);
-    } else {
-        const T* rb = second + s0 * m;
-        EF::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
-            if (TI > 1 && t >= nvalid) return cpx<T>{(T)0, (T)0};
-            const T* r = rb + t * m;
-            return cpx<T>{placed_response(r, 2 * int64_t(i), N, m), placed_response(r, 2 * int64_t(i) + 1, N, m)};
-        }, false, ltw);
-    }
+    // 2. The other spectrum into the tile.
+    if constexpr (SECOND_FIRST) signal_spectrum(); else second_spectrum();
+
+    // 3. Each pair (p, L - p): untangle, spectrum operation (Convolve.rs:90-138,
+    // Correlation.rs:68-110) with the signal's bins first, inverse untangle,
+    // back in place.  A pair is read and written by one thread only.
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {
         const int q = tid + u * NT;
         if ((TI * NP) % NT != 0 && q >= TI * NP) continue;
-        const int t = q / NP, p = q % NP;
+        const int t = TI == 1 ? 0 : q / NP, p = TI == 1 ? q : q % NP;
         if (p == 0) {
-            const cpx<T> r0 = dc_forward(lds[EF::lidx(t, 0)]);
-            const cpx<T> rh = lds[EF::lidx(t, NP)];
-            cpx<T> y0{spec_apply_real(op, xa[u].x, r0.x, inv_no2), spec_apply_real(op, xa[u].y, r0.y, inv_no2)};
-            xa[u] = dc_inverse(y0);
+            const cpx<T> t0 = dc_forward(lds[EF::lidx(t, 0)]);
+            const cpx<T> th = lds[EF::lidx(t, NP)];
+            const cpx<T> x0 = SECOND_FIRST ? t0 : ha[u], r0 = SECOND_FIRST ? ha[u] : t0;
+            const cpx<T> xh = SECOND_FIRST ? th : hb[u], rh = SECOND_FIRST ? hb[u] : th;
+            const cpx<T> y0{spec_apply_real(op, x0.x, r0.x, inv_no2), spec_apply_real(op, x0.y, r0.y, inv_no2)};
+            lds[EF::lidx(t, 0)] = dc_inverse(y0);
             // Bin L/2 pairs with itself: untangle leaves it unchanged both ways.
-            xb[u] = spec_apply(op, xb[u], rh, inv_no2);
+            lds[EF::lidx(t, NP)] = spec_apply(op, xh, rh, inv_no2);
         } else {
-            cpx<T> ra, rb2;
+            cpx<T> ta, tb, oa, ob2;
             const cpx<T> w = wpair(u);
-            untangle_pair(lds[EF::lidx(t, p)], lds[EF::lidx(t, L - p)], w, (T)-0.5, ra, rb2);
-            const cpx<T> ya = spec_apply(op, xa[u], ra, inv_no2);
-            const cpx<T> yb = spec_apply(op, xb[u], rb2, inv_no2);
+            untangle_pair(lds[EF::lidx(t, p)], lds[EF::lidx(t, L - p)], w, (T)-0.5, ta, tb);
+            const cpx<T> xa = SECOND_FIRST ? ta : ha[u], ra = SECOND_FIRST ? ha[u] : ta;
+            const cpx<T> xb = SECOND_FIRST ? tb : hb[u], rb = SECOND_FIRST ? hb[u] : tb;
             // Inverse untangle: conj(w) = exp(-i pi k / L).
-            untangle_pair(ya, yb, conj(w), (T)0.5, xa[u], xb[u]);
-        }
-    }
-    lds_barrier();
-#pragma unroll
-    for (int u = 0; u < PPT; ++u) {
-        const int q = tid + u * NT;
-        if ((TI * NP) % NT != 0 && q >= TI * NP) continue;
-        const int t = q / NP, p = q % NP;
-        if (p == 0) {
-            lds[EF::lidx(t, 0)] = xa[u];
-            lds[EF::lidx(t, NP)] = xb[u];
-        } else {
-            lds[EF::lidx(t, p)] = xa[u];
-            lds[EF::lidx(t, L - p)] = xb[u];
+            untangle_pair(spec_apply(op, xa, ra, inv_no2), spec_apply(op, xb, rb, inv_no2), conj(w), (T)0.5, oa, ob2);
+            lds[EF::lidx(t, p)] = oa;
+            lds[EF::lidx(t, L - p)] = ob2;
         }
     }
     lds_barrier();
 
-    // 3. Inverse FFT: (n/2) * (result / no2) = result.
+    // 4. Inverse FFT: (n/2) * (result / no2) = result.
     EI::template run_from_lds<LTM>(tid, lds, tw.stage, [&](int t, int k, cpx<T> v) {
         if (TI == 1 || t < nvalid) ob[t * L + k] = v;
     }, ltw);

@@ -408,13 +408,16 @@ def test_convlv_batch_per_signal_f32():
         assert rel_l2(y[i], ref) < 1e-5
 
 
-@pytest.mark.parametrize("n,m", [(64, 7), (1024, 257), (2048, 101), (8192, 257), (16384, 257)])
+@pytest.mark.parametrize("n,m", [(64, 7), (512, 17), (1024, 257), (2048, 101), (4096, 33), (8192, 257), (16384, 257),
+                                 (16384, 1001), (32768, 257)])
 @pytest.mark.parametrize("isign", [1, -1])
 def test_convlv_batch_f32_lengths(n, m, isign):
     # FP32 one-launch convlv / deconvolution across tile shapes (TI > 1 up
-    # to one signal per group; L = 512 and 8192 are the 4-waves-per-SIMD
-    # builds), per-signal responses, against the FP64 oracle on the
-    # FP32-rounded inputs.
+    # to one signal per group, where the response's spectrum is formed
+    # first; L = 512 and 8192 are the 4-waves-per-SIMD builds; m = 257 at
+    # n = 8192 / 16384 takes the windowed response, m = 1001 the full FFT;
+    # n = 32768 is the multi-pass form), per-signal responses, against the
+    # FP64 oracle on the FP32-rounded inputs.
     b = 6
     d = f32round(synth.normal(n + 3, n * b)).reshape(b, n)
     if isign == 1:
@@ -432,7 +435,7 @@ def test_convlv_batch_f32_lengths(n, m, isign):
         assert rel_l2(y[i], ref) < 1e-5, (n, m, isign, i)
 
 
-@pytest.mark.parametrize("n", [1024, 16384])
+@pytest.mark.parametrize("n", [64, 512, 1024, 4096, 8192, 16384, 32768])
 def test_correl_f32(n):
     d1 = f32round(synth.normal(n + 11, n))
     d2 = f32round(synth.normal(n + 12, n))
