@@ -135,9 +135,17 @@ __host__ __device__ inline bool resp_win_fits(int64_t L, int64_t m) {
     return resp_win_start(L, m) + 256 >= resp_win_hi(m) && -resp_win_lo(L, m) < L;
 }
 
+// Slot of bin k of the windowed response's spectrum in the LDS tile (Q =
+// L / 256): bits 1-4 of k XOR-ed with (k / Q) mod 16.  The window step's
+// writes (lanes across k1 and g at k = k1 + Q (g + 16 kb)) and the pair
+// loop's reads (consecutive k) then both touch distinct banks; with the
+// tile's padded layout the writes were 4-way conflicts.
+template <int Q>
+__device__ __forceinline__ int resp_slot(int k) { return k ^ (((k / Q) & 15) << 1); }
+
 // Spectrum Z_k, k < L, of the placed response (Convolve.rs:41-63) of one
-// signal of n = 2L reals read as L complex values, left in LDS in natural
-// order (lds[E::lidx(0, k)], then a barrier), for a response whose support
+// signal of n = 2L reals read as L complex values, left in LDS at
+// lds[resp_slot<L / 256>(k)] (then a barrier), for a response whose support
 // fits the window [js, js + 256) (resp_win_fits; js a multiple of 64):
 //   Z_{k1 + Q k2} = W_256^{js k2} sum_{j < 256} (z_{js+j} W_L^{(js+j) k1}) W_256^{j k2},
 // Q = L / 256, W_M = exp(+2 pi i / M) (the forward transform).  The 256
@@ -214,7 +222,7 @@ __device__ __forceinline__ void window_response_spectrum(const T* r, int64_t m, 
     dft16<1>(v);  // v[kb] = Z[k1 + Q (g + 16 kb)]
     lds_barrier();  // every group has read its transpose region
 #pragma unroll
-    for (int kb = 0; kb < 16; ++kb) lds[E::lidx(0, k1 + Q * (g + 16 * kb))] = v[kb];
+    for (int kb = 0; kb < 16; ++kb) lds[resp_slot<Q>(k1 + Q * (g + 16 * kb))] = v[kb];
     lds_barrier();
 }
 
@@ -395,6 +403,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(conv_waves<T
 
     // 1. The first spectrum, held untangled (bins p and L - p of pair p).
     if constexpr (SECOND_FIRST) second_spectrum(); else signal_spectrum();
+    auto hidx = [](int t, int i) -> int {
+        if constexpr (WINDOW) return resp_slot<L / 256>(i);
+        else return EF::lidx(t, i);
+    };
     cpx<T> ha[PPT], hb[PPT];
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {
@@ -402,10 +414,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(conv_waves<T
         if ((TI * NP) % NT != 0 && q >= TI * NP) continue;
         const int t = TI == 1 ? 0 : q / NP, p = TI == 1 ? q : q % NP;
         if (p == 0) {
-            ha[u] = dc_forward(lds[EF::lidx(t, 0)]);
-            hb[u] = lds[EF::lidx(t, NP)];
+            ha[u] = dc_forward(lds[hidx(t, 0)]);
+            hb[u] = lds[hidx(t, NP)];
         } else {
-            untangle_pair(lds[EF::lidx(t, p)], lds[EF::lidx(t, L - p)], wpair(u), (T)-0.5, ha[u], hb[u]);
+            untangle_pair(lds[hidx(t, p)], lds[hidx(t, L - p)], wpair(u), (T)-0.5, ha[u], hb[u]);
         }
     }
     lds_barrier();  // every value is read before the next FFT overwrites the tile
