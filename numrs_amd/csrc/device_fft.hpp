@@ -529,6 +529,19 @@ struct Engine {
         lds_barrier();
     }
 
+    // run_to_lds on stage-0 inputs the caller loaded with load0 earlier (so
+    // their latency can overlap other work).
+    template <int LTM = 0>
+    static __device__ __forceinline__ void run_to_lds_loaded(cpx<T>* v, int tid, cpx<T>* lds, const cpx<T>* stage_tw,
+                                                             bool lead_barrier = false, const cpx<T>* ltw = nullptr) {
+        compute<0>(v, tid, stage_tw);
+        middle<1, LTM>(v, tid, lds, stage_tw, lead_barrier, ltw);
+        if constexpr (NST > 1) lds_barrier();
+        else if (lead_barrier) lds_barrier();
+        store_lds<NST - 1>(v, tid, lds);
+        lds_barrier();
+    }
+
     // Transform whose natural-order input is already in LDS (a barrier must
     // separate the writes of that input from this call).
     template <int LTM = 0, class ST>

@@ -395,6 +395,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(conv_waves<T
             }, false, ltw);
         }
     };
+    auto signal_load = [&](int t, int i) -> cpx<T> {
+        return (TI == 1 || t < nvalid) ? db[t * L + i] : cpx<T>{(T)0, (T)0};
+    };
     auto signal_spectrum = [&]() {
         EF::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
             return (TI == 1 || t < nvalid) ? db[t * L + i] : cpx<T>{(T)0, (T)0};
@@ -402,7 +405,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(conv_waves<T
     };
 
     // 1. The first spectrum, held untangled (bins p and L - p of pair p).
+    // When the signal comes second its stage-0 loads are issued here, so
+    // their HBM latency overlaps the untangle of the held spectrum.
     if constexpr (SECOND_FIRST) second_spectrum(); else signal_spectrum();
+    cpx<T> sv[SECOND_FIRST ? EF::EPT : 1];
+    if constexpr (SECOND_FIRST) EF::load0(sv, tid, signal_load);
     auto hidx = [](int t, int i) -> int {
         if constexpr (WINDOW) return resp_slot<L / 256>(i);
         else return EF::lidx(t, i);
@@ -423,7 +430,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(conv_waves<T
     lds_barrier();  // every value is read before the next FFT overwrites the tile
 
     // 2. The other spectrum into the tile.
-    if constexpr (SECOND_FIRST) signal_spectrum(); else second_spectrum();
+    if constexpr (SECOND_FIRST) EF::template run_to_lds_loaded<LTM>(sv, tid, lds, tw.stage, false, ltw);
+    else second_spectrum();
 
     // 3. Each pair (p, L - p): untangle, spectrum operation (Convolve.rs:90-138,
     // Correlation.rs:68-110) with the signal's bins first, inverse untangle,
