@@ -161,7 +161,7 @@ def test_four1_batch_host_pipelined_threads():
 
 
 # ---- realft ------------------------------------------------------------------
-@pytest.mark.parametrize("logn", list(range(1, 16)) + [17, 19])
+@pytest.mark.parametrize("logn", list(range(1, 16)) + [16, 17, 18, 19, 20, 21, 22, 23])
 def test_realft_f64(logn):
     n = 1 << logn
     x = synth.normal(logn, n)
@@ -179,7 +179,7 @@ def test_realft_golden(golden):
         assert rel_l2(y, golden[f"realft_{n}_out"]) < TOL64
 
 
-@pytest.mark.parametrize("logn", [3, 10, 14, 16])
+@pytest.mark.parametrize("logn", [3, 10, 14, 15, 16, 17, 18, 20, 22, 24])
 def test_realft_f32(logn):
     n = 1 << logn
     x = f32round(synth.normal(logn, n))
@@ -293,6 +293,26 @@ def test_realft_plan_multipass_chunked_f32():
     back = t.cpu().numpy()
     for i in (0, 17, 35):
         assert rel_l2(back[i * n:(i + 1) * n], (n / 2) * x[i * n:(i + 1) * n].astype(np.float64)) < TOL32, i
+
+
+def test_realft_plan_multipass_chunked_f64():
+    # 40 FP64 signals of 2^18 reals: the 2^17-point transform's second pass
+    # with the untangle fused in, over several chunks and a partial one;
+    # every signal against the oracle, then the inverse returns (n/2) x.
+    import torch
+
+    n, b = 1 << 18, 40
+    x = synth.normal(31, b * n)
+    t = torch.from_numpy(x.copy()).cuda()
+    plan = numrs_amd.RealFTPlan("float64", n, b)
+    plan.execute(t, 1)
+    got = t.cpu().numpy()
+    for i in range(b):
+        assert rel_l2(got[i * n:(i + 1) * n], oracle.realft(x[i * n:(i + 1) * n], 1)) < TOL64, i
+    plan.execute(t, -1)
+    back = t.cpu().numpy()
+    for i in (0, 21, 39):
+        assert rel_l2(back[i * n:(i + 1) * n], (n / 2) * x[i * n:(i + 1) * n]) < TOL64, i
 
 
 # ---- fourn -------------------------------------------------------------------
