@@ -201,6 +201,10 @@ __host__ __device__ constexpr int ilog2c(long long v) { return v <= 1 ? 0 : 1 + 
 #ifndef NRFFT_LDS_SWIZZLE
 #define NRFFT_LDS_SWIZZLE 0
 #endif
+// Cross-lane (DPP) exchange for 256-point FP32 row-mapped passes.
+#ifndef NRFFT_DPP_EXCHANGE
+#define NRFFT_DPP_EXCHANGE 1
+#endif
 // One pad slot per NRFFT_LDS_PAD complex values of a padded LDS tile.
 #ifndef NRFFT_LDS_PAD
 #define NRFFT_LDS_PAD 32
@@ -226,6 +230,62 @@ struct RadixPlan {
 };
 
 enum { MAP_J = 0, MAP_T = 1 };
+
+// ---------------------------------------------------------------------------
+// Cross-lane exchange: a 16 x 16 transpose across the 16 lanes of each DPP
+// row (lanes 16k .. 16k+15 of a wave), in registers -- lane l's v[r] and lane
+// r's v[l] trade places.  Four steps, one per bit m = 8, 4, 2, 1 of the lane
+// and register indices: an element moves to (lane ^ m, register ^ m) when
+// the two indices differ in bit m (after all four every element has swapped
+// its lane and register index).  m = 8 and 4 use DPP row_ror:8 and row_shl /
+// row_shr:4 with bank masks (the receiving lanes are whole 4-lane banks, so
+// each step is one DPP move per float and register); m = 2 and 1 use
+// quad_perm plus a per-lane select.
+// ---------------------------------------------------------------------------
+template <int CTRL, int BANKS>
+__device__ __forceinline__ float dpp_update(float old, float src) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), CTRL, 0xF, BANKS, false));
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float src) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(src), CTRL, 0xF, 0xF, false));
+}
+constexpr int DPP_ROW_SHL4 = 0x104, DPP_ROW_SHR4 = 0x114, DPP_ROW_ROR8 = 0x128;
+constexpr int DPP_QUAD_XOR1 = 1 | (0 << 2) | (3 << 4) | (2 << 6);  // quad_perm [1, 0, 3, 2]
+constexpr int DPP_QUAD_XOR2 = 2 | (3 << 2) | (0 << 4) | (1 << 6);  // quad_perm [2, 3, 0, 1]
+
+template <int M>
+__device__ __forceinline__ void row_transpose_step(cpx<float>* v, int lane) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        if (r & M) continue;
+        const cpx<float> a = v[r], b = v[r | M];
+        if constexpr (M == 8) {
+            // lanes 8-15 (banks 2, 3) take the partner's v[r | 8] into v[r];
+            // lanes 0-7 (banks 0, 1) its v[r] into v[r | 8]
+            v[r] = {dpp_update<DPP_ROW_ROR8, 0xC>(a.x, b.x), dpp_update<DPP_ROW_ROR8, 0xC>(a.y, b.y)};
+            v[r | M] = {dpp_update<DPP_ROW_ROR8, 0x3>(b.x, a.x), dpp_update<DPP_ROW_ROR8, 0x3>(b.y, a.y)};
+        } else if constexpr (M == 4) {
+            // banks 1, 3 (lane bit 2 set) read lane - 4, banks 0, 2 lane + 4
+            v[r] = {dpp_update<DPP_ROW_SHR4, 0xA>(a.x, b.x), dpp_update<DPP_ROW_SHR4, 0xA>(a.y, b.y)};
+            v[r | M] = {dpp_update<DPP_ROW_SHL4, 0x5>(b.x, a.x), dpp_update<DPP_ROW_SHL4, 0x5>(b.y, a.y)};
+        } else {
+            constexpr int Q = M == 2 ? DPP_QUAD_XOR2 : DPP_QUAD_XOR1;
+            const bool hi = (lane & M) != 0;
+            const cpx<float> pb{dpp_move<Q>(b.x), dpp_move<Q>(b.y)};  // partner's v[r | M]
+            const cpx<float> pa{dpp_move<Q>(a.x), dpp_move<Q>(a.y)};  // partner's v[r]
+            v[r] = hi ? pb : a;
+            v[r | M] = hi ? b : pa;
+        }
+    }
+}
+__device__ __forceinline__ void row_transpose16(cpx<float>* v) {
+    const int lane = int(threadIdx.x) & 15;
+    row_transpose_step<8>(v, lane);
+    row_transpose_step<4>(v, lane);
+    row_transpose_step<2>(v, lane);
+    row_transpose_step<1>(v, lane);
+}
 
 // Twiddle tables (device memory, built in FP64 on the GPU):
 //   coarse[m] = exp(2 pi i m / 2^13),       m < 2^13
@@ -312,6 +372,12 @@ struct Engine {
     static_assert(TI * L % NT == 0, "tile must split evenly over threads");
     static constexpr int NST = P::NST;
     static_assert(EPT % P::EPT_MIN == 0 || L < 16, "EPT must cover the largest radix");
+    // A 256-point FP32 transform on 16 lanes with the row mapping on both
+    // sides: its one exchange is a 16 x 16 transpose inside a DPP row
+    // (row_transpose16), no LDS (NRFFT_DPP_EXCHANGE).
+    static constexpr bool DPP_X = NRFFT_DPP_EXCHANGE && sizeof(T) == 4 && L == 256 && NST == 2 &&
+                                  P::radix(0) == 16 && P::radix(1) == 16 && EPT == 16 && MAPL == MAP_J &&
+                                  MAPS == MAP_J;
     // LDS layout follows the mapping used after the first stage.
     static constexpr int LAYOUT = (NST == 1) ? MAPL : MAPS;
     static constexpr int map_of(int s) { return s == 0 ? MAPL : MAPS; }
@@ -478,16 +544,23 @@ struct Engine {
     // Stages 1..NST-1 (stage 0 has been computed on v).  The leading barrier
     // of each exchange protects the LDS buffer against the previous stage's
     // readers (WAR); the trailing one publishes the writes (RAW).
-    template <int s = 1, int LTM = 0>
+    // DPP_OK: the caller lets the exchange run across lanes where DPP_X
+    // allows it (then this call touches no LDS and has no barrier, so the
+    // caller must not rely on it to publish other LDS data).
+    template <int s = 1, int LTM = 0, bool DPP_OK = false>
     static __device__ __forceinline__ void middle(cpx<T>* v, int tid, cpx<T>* lds, const cpx<T>* stage_tw,
                                                   bool lead_barrier, const cpx<T>* ltw = nullptr) {
         if constexpr (s < NST) {
-            if (lead_barrier) lds_barrier();
-            store_lds<s - 1>(v, tid, lds);
-            lds_barrier();
-            load_lds<s>(v, tid, lds);
+            if constexpr (DPP_OK && DPP_X && s == 1) {
+                row_transpose16(reinterpret_cast<cpx<float>*>(v));
+            } else {
+                if (lead_barrier) lds_barrier();
+                store_lds<s - 1>(v, tid, lds);
+                lds_barrier();
+                load_lds<s>(v, tid, lds);
+            }
             compute<s, LTM>(v, tid, stage_tw, ltw);
-            middle<s + 1, LTM>(v, tid, lds, stage_tw, true, ltw);
+            middle<s + 1, LTM, DPP_OK>(v, tid, lds, stage_tw, true, ltw);
         }
     }
 

@@ -57,16 +57,23 @@ constexpr int waves_per_eu_target() {
 #ifndef NRFFT_PASS_LDS_TW
 #define NRFFT_PASS_LDS_TW 1
 #endif
+// Kernels whose one exchange runs across lanes (Engine::DPP_X) use no LDS:
+// no exchange tile and no LDS twiddle copy (stage 1 reads the global table).
+template <typename T, int S, int L, int TI, int NT, int ML, int MS>
+constexpr bool kc2c_uses_lds() {
+    using E = Engine<T, S, L, TI, NT, ML, MS>;
+    return E::NST > 1 && !E::DPP_X;
+}
 template <typename T, int S, int L, int TI, int NT, int ML, int MS>
 constexpr int kc2c_ltm() {
     using E = Engine<T, S, L, TI, NT, ML, MS>;
-    return (NRFFT_PASS_LDS_TW && E::NST > 1) ? engine_ltm<E>(E::LDS_ELEMS * E::CSIZE) : 0;
+    return (NRFFT_PASS_LDS_TW && kc2c_uses_lds<T, S, L, TI, NT, ML, MS>()) ? engine_ltm<E>(E::LDS_ELEMS * E::CSIZE) : 0;
 }
 template <typename T, int S, int L, int TI, int NT, int ML, int MS>
 constexpr int kc2c_lds_bytes() {
     using E = Engine<T, S, L, TI, NT, ML, MS>;
     constexpr int ltm = kc2c_ltm<T, S, L, TI, NT, ML, MS>();
-    return E::NST > 1 ? (E::LDS_ELEMS + (ltm > 0 ? E::tw_elems(ltm) : 0)) * (int)sizeof(cpx<T>) : 0;
+    return kc2c_uses_lds<T, S, L, TI, NT, ML, MS>() ? (E::LDS_ELEMS + (ltm > 0 ? E::tw_elems(ltm) : 0)) * (int)sizeof(cpx<T>) : 0;
 }
 // Register prefetch (NRFFT_PREFETCH).  Not for the 1024-thread FP32 pass B
 // of 2048 points alone either: its 4 waves per SIMD leave 128 VGPRs, and
@@ -75,8 +82,12 @@ template <typename T, int NT, int ML, int MS>
 constexpr bool kc2c_pf() {
     return NRFFT_PREFETCH != 0;
 }
+// The cross-lane (no-LDS) kernels are compiled for 4 waves per SIMD: at 8
+// their 16 values and the exchange's temporaries spill.
 template <typename T, int S, int L, int TI, int NT, int ML, int MS>
-constexpr int kc2c_waves = waves_per_eu_target<kc2c_lds_bytes<T, S, L, TI, NT, ML, MS>(), NT, kc2c_pf<T, NT, ML, MS>()>();
+constexpr int kc2c_waves = (Engine<T, S, L, TI, NT, ML, MS>::NST > 1 && !kc2c_uses_lds<T, S, L, TI, NT, ML, MS>())
+                               ? 4
+                               : waves_per_eu_target<kc2c_lds_bytes<T, S, L, TI, NT, ML, MS>(), NT, kc2c_pf<T, NT, ML, MS>()>();
 
 // Thread index laundered through a register move the compiler cannot hoist,
 // so per-tile address arithmetic is recomputed inside the persistent loop
@@ -94,7 +105,7 @@ k_c2c(const cpx<T>* in, cpx<T>* out, PassDesc d, Twiddles<T> tw) {
     using V2 = T __attribute__((ext_vector_type(2)));
     constexpr int EPT = E::EPT;
     constexpr int LTM = kc2c_ltm<T, S, L, TI, NT, ML, MS>();
-    __shared__ cpx<T> lds[E::NST > 1 ? E::LDS_ELEMS : 1];
+    __shared__ cpx<T> lds[kc2c_uses_lds<T, S, L, TI, NT, ML, MS>() ? E::LDS_ELEMS : 1];
     __shared__ __align__(16) cpx<T> ltw[LTM > 0 ? E::tw_elems(LTM) : 1];
     const int tid = threadIdx.x;
     const int64_t tiles0 = (d.nb0 + TI - 1) / TI;
@@ -164,7 +175,7 @@ k_c2c(const cpx<T>* in, cpx<T>* out, PassDesc d, Twiddles<T> tw) {
             }
             const int ltid = fresh_tid(tid);
             E::template compute<0>(v, ltid, tw.stage);
-            E::template middle<1, LTM>(v, ltid, lds, tw.stage, !first, ltw);
+            E::template middle<1, LTM, true>(v, ltid, lds, tw.stage, !first, ltw);
             if (d.tw_logn) {
                 const int logn = d.tw_logn, sh = d.tw_b0_shift;
                 const uint32_t mask = (1u << logn) - 1u;
