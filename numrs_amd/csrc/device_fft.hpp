@@ -201,7 +201,7 @@ __host__ __device__ constexpr int ilog2c(long long v) { return v <= 1 ? 0 : 1 + 
 #ifndef NRFFT_LDS_SWIZZLE
 #define NRFFT_LDS_SWIZZLE 0
 #endif
-// Cross-lane (DPP) exchange for 256-point FP32 row-mapped passes.
+// Cross-lane (DPP) exchange for 256-point row-mapped passes.
 #ifndef NRFFT_DPP_EXCHANGE
 #define NRFFT_DPP_EXCHANGE 1
 #endif
@@ -242,24 +242,32 @@ enum { MAP_J = 0, MAP_T = 1 };
 // each step is one DPP move per float and register); m = 2 and 1 use
 // quad_perm plus a per-lane select.
 // ---------------------------------------------------------------------------
+// DPP moves of one 32-bit lane value (FP64 values move as two halves).
 template <int CTRL, int BANKS>
 __device__ __forceinline__ float dpp_update(float old, float src) {
     return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), CTRL, 0xF, BANKS, false));
 }
-template <int CTRL>
-__device__ __forceinline__ float dpp_move(float src) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(src), CTRL, 0xF, 0xF, false));
+template <int CTRL, int BANKS>
+__device__ __forceinline__ double dpp_update(double old, double src) {
+    const long long o = __double_as_longlong(old), v = __double_as_longlong(src);
+    const int lo = __builtin_amdgcn_update_dpp(int(o), int(v), CTRL, 0xF, BANKS, false);
+    const int hi = __builtin_amdgcn_update_dpp(int(o >> 32), int(v >> 32), CTRL, 0xF, BANKS, false);
+    return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+template <int CTRL, typename T>
+__device__ __forceinline__ T dpp_move(T src) {
+    return dpp_update<CTRL, 0xF>(src, src);
 }
 constexpr int DPP_ROW_SHL4 = 0x104, DPP_ROW_SHR4 = 0x114, DPP_ROW_ROR8 = 0x128;
 constexpr int DPP_QUAD_XOR1 = 1 | (0 << 2) | (3 << 4) | (2 << 6);  // quad_perm [1, 0, 3, 2]
 constexpr int DPP_QUAD_XOR2 = 2 | (3 << 2) | (0 << 4) | (1 << 6);  // quad_perm [2, 3, 0, 1]
 
-template <int M>
-__device__ __forceinline__ void row_transpose_step(cpx<float>* v, int lane) {
+template <int M, typename T>
+__device__ __forceinline__ void row_transpose_step(cpx<T>* v, int lane) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         if (r & M) continue;
-        const cpx<float> a = v[r], b = v[r | M];
+        const cpx<T> a = v[r], b = v[r | M];
         if constexpr (M == 8) {
             // lanes 8-15 (banks 2, 3) take the partner's v[r | 8] into v[r];
             // lanes 0-7 (banks 0, 1) its v[r] into v[r | 8]
@@ -272,14 +280,15 @@ __device__ __forceinline__ void row_transpose_step(cpx<float>* v, int lane) {
         } else {
             constexpr int Q = M == 2 ? DPP_QUAD_XOR2 : DPP_QUAD_XOR1;
             const bool hi = (lane & M) != 0;
-            const cpx<float> pb{dpp_move<Q>(b.x), dpp_move<Q>(b.y)};  // partner's v[r | M]
-            const cpx<float> pa{dpp_move<Q>(a.x), dpp_move<Q>(a.y)};  // partner's v[r]
+            const cpx<T> pb{dpp_move<Q>(b.x), dpp_move<Q>(b.y)};  // partner's v[r | M]
+            const cpx<T> pa{dpp_move<Q>(a.x), dpp_move<Q>(a.y)};  // partner's v[r]
             v[r] = hi ? pb : a;
             v[r | M] = hi ? b : pa;
         }
     }
 }
-__device__ __forceinline__ void row_transpose16(cpx<float>* v) {
+template <typename T>
+__device__ __forceinline__ void row_transpose16(cpx<T>* v) {
     const int lane = int(threadIdx.x) & 15;
     row_transpose_step<8>(v, lane);
     row_transpose_step<4>(v, lane);
@@ -372,10 +381,10 @@ struct Engine {
     static_assert(TI * L % NT == 0, "tile must split evenly over threads");
     static constexpr int NST = P::NST;
     static_assert(EPT % P::EPT_MIN == 0 || L < 16, "EPT must cover the largest radix");
-    // A 256-point FP32 transform on 16 lanes with the row mapping on both
+    // A 256-point transform on 16 lanes with the row mapping on both
     // sides: its one exchange is a 16 x 16 transpose inside a DPP row
     // (row_transpose16), no LDS (NRFFT_DPP_EXCHANGE).
-    static constexpr bool DPP_X = NRFFT_DPP_EXCHANGE && sizeof(T) == 4 && L == 256 && NST == 2 &&
+    static constexpr bool DPP_X = NRFFT_DPP_EXCHANGE && L == 256 && NST == 2 &&
                                   P::radix(0) == 16 && P::radix(1) == 16 && EPT == 16 && MAPL == MAP_J &&
                                   MAPS == MAP_J;
     // LDS layout follows the mapping used after the first stage.
@@ -552,7 +561,7 @@ struct Engine {
                                                   bool lead_barrier, const cpx<T>* ltw = nullptr) {
         if constexpr (s < NST) {
             if constexpr (DPP_OK && DPP_X && s == 1) {
-                row_transpose16(reinterpret_cast<cpx<float>*>(v));
+                row_transpose16(v);
             } else {
                 if (lead_barrier) lds_barrier();
                 store_lds<s - 1>(v, tid, lds);
