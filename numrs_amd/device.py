@@ -56,14 +56,24 @@ def _ptr(x) -> int:
     return x.data_ptr()
 
 
-def _stream(stream) -> int:
-    if stream is None:
-        try:
-            import torch
+_RAW_STREAM = None  # torch's current-stream lookup (the cheapest one this torch has)
 
-            return torch.cuda.current_stream().cuda_stream
-        except Exception:  # noqa: BLE001 - no torch: default stream
-            return 0
+
+def _stream(stream) -> int:
+    global _RAW_STREAM
+    if stream is None:
+        if _RAW_STREAM is None:
+            try:
+                import torch
+
+                raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+                if raw is not None:
+                    _RAW_STREAM = lambda: raw(torch.cuda.current_device())  # noqa: E731
+                else:
+                    _RAW_STREAM = lambda: torch.cuda.current_stream().cuda_stream  # noqa: E731
+            except Exception:  # noqa: BLE001 - no torch: default stream
+                _RAW_STREAM = lambda: 0  # noqa: E731
+        return _RAW_STREAM()
     if isinstance(stream, int):
         return stream
     return stream.cuda_stream
@@ -73,6 +83,7 @@ class _Plan:
     def __init__(self):
         self._h = ctypes.c_void_p()
         self._dtype = None
+        self._tdtype = None
         self._device = _current_device()
 
     def __del__(self):
@@ -93,15 +104,22 @@ class _Plan:
             raise ValueError("plan has been destroyed")
         if isinstance(x, int):
             return x
-        addr = _ptr(x)
-        want = _DT_NAME[self._dtype]
-        if str(x.dtype) != f"torch.{want}":
-            raise ValueError(f"{name}: dtype {x.dtype} does not match the plan's {want}")
-        if self._device is not None and x.device.index is not None and x.device.index != self._device:
-            raise ValueError(f"{name}: tensor on cuda:{x.device.index}, plan on cuda:{self._device}")
+        tdt = self._tdtype
+        if tdt is None:
+            import torch
+
+            tdt = self._tdtype = torch.float32 if self._dtype == _lib.F32 else torch.float64
+        if not x.is_cuda:
+            raise ValueError(f"{name}: tensor must live in GPU memory")
+        if not x.is_contiguous():
+            raise ValueError(f"{name}: tensor must be contiguous")
+        if x.dtype is not tdt:
+            raise ValueError(f"{name}: dtype {x.dtype} does not match the plan's {_DT_NAME[self._dtype]}")
+        if self._device is not None and x.get_device() != self._device:
+            raise ValueError(f"{name}: tensor on cuda:{x.get_device()}, plan on cuda:{self._device}")
         if x.numel() < values:
             raise ValueError(f"{name}: {x.numel()} values, the plan needs {values}")
-        return addr
+        return x.data_ptr()
 
     @property
     def workspace_bytes(self) -> int:
