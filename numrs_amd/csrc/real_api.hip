@@ -213,7 +213,7 @@ static const RealRef& real_lookup(int dtype, int logl) {
 enum RealSpecial {
     RS_FUSED_REALFT = 1,  // args: nsig, L
     RS_FUSED_CONV = 2,    // args: nsig, L, m
-    RS_UNTANGLE = 3,      // args: nsig, L, dir
+    RS_UNTANGLE = 3,      // args: nsig, L, dir, element offset (complex values) into src and dst
     RS_PLACE = 4,         // args: nsig, n, m
     RS_SPEC = 5,          // args: nsig, L
     RS_N2 = 6,            // args: nsig, dir(0 = both directions by isign)
@@ -230,6 +230,36 @@ static Launch real_launch(int special, int src, int dst, std::initializer_list<i
     int i = 0;
     for (int64_t a : args) l.args[i++] = a;
     return l;
+}
+
+// Forward realft of N = 2^logn complex points per signal as the four-step's
+// two passes and the untangle chunk by chunk: each chunk is untangled right
+// after its second pass, which stores plainly, so the untangle re-reads the
+// chunk from the caches (MI355X, 64 x 2^21 FP32: 0.663 -> 0.621 ms; FP64
+// 32 x 2^21 0.614 -> 0.598 ms; with non-temporal second-pass stores the
+// chunked form took 0.675 ms).  NRFFT_REALFT_CHUNKED = 0: one untangle
+// launch after all chunks; 1: chunked with non-temporal stores.
+#ifndef NRFFT_REALFT_CHUNKED
+#define NRFFT_REALFT_CHUNKED 2
+#endif
+static int plan_realft_fwd_chunked(Plan& p, int logn, int64_t batch, size_t* tmp_elems) {
+    FourStep fs;
+    if (int rc = fourstep_setup(p, logn, &fs)) return rc;
+    const int64_t N = int64_t(1) << logn;
+    const int64_t chunk = (p.tune.chunk > 0 && p.tune.chunk < batch) ? p.tune.chunk : batch;
+    *tmp_elems = std::max<size_t>(*tmp_elems, size_t(N) * size_t(chunk));
+    for (int64_t c0 = 0; c0 < batch; c0 += chunk) {
+        const int64_t cb = std::min<int64_t>(chunk, batch - c0);
+        Launch la, lb;
+        if (int rc = fourstep_launches(p, fs, cb, B_IN, B_OUT, &la, &lb)) return rc;
+        la.off_src = c0 * N;
+        lb.off_dst = c0 * N;
+        if (NRFFT_REALFT_CHUNKED == 2) lb.desc.nt_out = 0;
+        p.launches.push_back(la);
+        p.launches.push_back(lb);
+        p.launches.push_back(real_launch(RS_UNTANGLE, B_OUT, B_OUT, {cb, N, 1, c0 * N}));
+    }
+    return NRFFT_OK;
 }
 
 int build_plan_real(Plan& p) {
@@ -256,9 +286,10 @@ int build_plan_real(Plan& p) {
             Plan fw = p, iv = p;
             fw.launches.clear();
             iv.launches.clear();
-            int rc = plan_c2c_1d(fw, logl, batch, B_IN, B_OUT, &tmp);
+            int rc = NRFFT_REALFT_CHUNKED ? plan_realft_fwd_chunked(fw, logl, batch, &tmp)
+                                          : plan_c2c_1d(fw, logl, batch, B_IN, B_OUT, &tmp);
             if (rc) return rc;
-            fw.launches.push_back(real_launch(RS_UNTANGLE, B_OUT, B_OUT, {batch, L, 1}));
+            if (!NRFFT_REALFT_CHUNKED) fw.launches.push_back(real_launch(RS_UNTANGLE, B_OUT, B_OUT, {batch, L, 1}));
             iv.launches.push_back(real_launch(RS_UNTANGLE, B_IN, B_OUT, {batch, L, -1}));
             rc = plan_c2c_1d(iv, logl, batch, B_OUT, B_OUT, &tmp);
             if (rc) return rc;
@@ -350,6 +381,8 @@ static int exec_real_t(const Plan& p, const Launch& l, void* const* bufs, int is
             return NRFFT_OK;
         }
         case RS_UNTANGLE: {
+            src = static_cast<cpx<T>*>(src) + a[3];
+            dst = static_cast<cpx<T>*>(dst) + a[3];
             if (src != dst) HIPCHK(hipMemcpyAsync(dst, src, size_t(a[0] * a[1]) * sizeof(cpx<T>),
                                                   hipMemcpyDeviceToDevice, stream));
             const int logn = ilog2((size_t)a[1]) + 1;
