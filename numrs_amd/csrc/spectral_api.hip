@@ -60,6 +60,42 @@ static void tag_launches(Plan& p, size_t from, int force, int only) {
     }
 }
 
+// The C2C of length 2^logn of a spectral plan, chunk by chunk: each chunk's
+// packing launch right before its first pass, its finishing launch right
+// after its second pass (which stores plainly), all through a chunk-sized
+// scratch TMP2 (complex values [0, chunk * 2^logn)), so the chunk's data
+// passes between the steps through the caches (MALL) rather than HBM.
+// pre(c0, cb) / post(c0, cb) return the launches for signals [c0, c0 + cb)
+// with their byte offsets set; the C2C launches run with isign `force` and
+// every launch only for calls of direction `only` (0 = both).
+template <class PRE, class POST>
+static int plan_spec_chunked(Plan& p, int logn, int64_t batch, int force, int only, PRE pre, POST post) {
+    FourStep fs;
+    if (int rc = fourstep_setup(p, logn, &fs)) return rc;
+    const int64_t N = int64_t(1) << logn;
+    const int64_t chunk = (p.tune.chunk > 0 && p.tune.chunk < batch) ? p.tune.chunk : batch;
+    for (int64_t c0 = 0; c0 < batch; c0 += chunk) {
+        const int64_t cb = std::min<int64_t>(chunk, batch - c0);
+        Launch a = pre(c0, cb);
+        a.args[7] = only;
+        p.launches.push_back(a);
+        Launch la, lb;
+        if (int rc = fourstep_launches(p, fs, cb, B_TMP2, B_TMP2, &la, &lb)) return rc;
+        lb.desc.nt_out = 0;
+        for (Launch* l : {&la, &lb}) {
+            l->args[6] = force;
+            l->args[7] = only;
+            p.launches.push_back(*l);
+        }
+        Launch b = post(c0, cb);
+        b.args[7] = only;
+        p.launches.push_back(b);
+    }
+    p.tmp2_offset = std::max<size_t>(p.tmp2_offset, size_t(N) * size_t(chunk));  // the C2C scratch (B_TMP) comes first
+    p.tmp_elems = std::max<size_t>(p.tmp_elems, p.tmp2_offset + size_t(N) * size_t(chunk));
+    return NRFFT_OK;
+}
+
 int build_plan_spectral(Plan& p) {
     const int64_t batch = (int64_t)p.batch;
     size_t tmp = 0;
@@ -98,6 +134,9 @@ int build_plan_spectral(Plan& p) {
                 p.launches.push_back(spec_launch(SP_CF1_FUSED, B_IN, B_OUT, {batch, n}));
                 return NRFFT_OK;
             }
+            // Whole batch per step: the chunked form (pack, passes and finish
+            // per chunk through a chunk-sized scratch) measured 0.833 ms
+            // against 0.818 at 64 x 2^20 FP32.
             p.launches.push_back(spec_launch(SP_CF1_PACK, B_IN, B_TMP2, {batch, n}));
             use_c2c_tune(p, logn);
             const size_t from = p.launches.size();
@@ -120,28 +159,36 @@ int build_plan_spectral(Plan& p) {
                 return NRFFT_OK;
             }
             use_c2c_tune(p, logl);
+            const int64_t sig = n * (int64_t(csize(p.dtype)) / 2);  // bytes of one signal
             // forward: permute-pack -> C2C(+1) -> untangle and rotate
-            Launch a = spec_launch(SP_CF2_FPACK, B_IN, B_TMP2, {batch, L});
-            a.args[7] = 1;
-            p.launches.push_back(a);
-            size_t from = p.launches.size();
-            if ((rc = plan_c2c_1d(p, logl, batch, B_TMP2, B_TMP2, &tmp))) return rc;
-            tag_launches(p, from, 1, 1);
-            Launch b = spec_launch(SP_CF2_FPOST, B_TMP2, B_OUT, {batch, L});
-            b.args[7] = 1;
-            p.launches.push_back(b);
-            // inverse: rotate and untangle -> C2C(-1) -> unpermute
+            rc = plan_spec_chunked(
+                p, logl, batch, 1, 1,
+                [&](int64_t c0, int64_t cb) {
+                    Launch l = spec_launch(SP_CF2_FPACK, B_IN, B_TMP2, {cb, L});
+                    l.off_src = c0 * sig;
+                    return l;
+                },
+                [&](int64_t c0, int64_t cb) {
+                    Launch l = spec_launch(SP_CF2_FPOST, B_TMP2, B_OUT, {cb, L});
+                    l.off_dst = c0 * sig;
+                    return l;
+                });
+            if (rc) return rc;
+            // inverse: rotate and untangle -> C2C(-1) -> unpermute, whole batch
+            // per step (chunked: 0.455 ms against 0.444 at 64 x 2^20 FP32;
+            // the forward direction gains from it, 0.440 -> 0.409 ms).
             Launch c = spec_launch(SP_CF2_IPRE, B_IN, B_TMP2, {batch, L});
             c.args[7] = -1;
             p.launches.push_back(c);
-            from = p.launches.size();
+            const size_t from = p.launches.size();
             if ((rc = plan_c2c_1d(p, logl, batch, B_TMP2, B_TMP2, &tmp))) return rc;
             tag_launches(p, from, -1, -1);
             Launch d = spec_launch(SP_CF2_IUNPACK, B_TMP2, B_OUT, {batch, L});
             d.args[7] = -1;
             p.launches.push_back(d);
-            p.tmp2_offset = tmp;
-            p.tmp_elems = tmp + size_t(batch) * size_t(L);
+            // one TMP2 for both directions: after the C2C scratch, the whole batch
+            p.tmp2_offset = std::max<size_t>(p.tmp2_offset, tmp);
+            p.tmp_elems = std::max<size_t>(p.tmp_elems, p.tmp2_offset + size_t(batch) * size_t(L));
             return NRFFT_OK;
         }
         case PK_RLFT3: {
@@ -193,8 +240,9 @@ static dim3 grid2(int64_t per, int64_t nsig, int64_t max_blocks = 65536) {
 template <typename T>
 static int exec_spec_t(const Plan& p, const Launch& l, void* const* bufs, int isign, hipStream_t s, DeviceCtx* ctx) {
     Twiddles<T> tw = ctx_twiddles<T>(ctx);
-    void* src = bufs[l.src];
-    void* dst = bufs[l.dst];
+    // Spectral launches carry byte offsets into their buffers (chunks).
+    void* src = static_cast<char*>(bufs[l.src]) + l.off_src;
+    void* dst = static_cast<char*>(bufs[l.dst]) + l.off_dst;
     const int64_t* a = l.args;
     const int64_t nsig = a[0];
     switch (l.special) {

@@ -119,6 +119,53 @@ def test_cosft2_round_trip(n):
     assert rel_l2(g, 0.5 * n * y) < 1e-12
 
 
+@pytest.mark.parametrize("dt,n,b", [("float32", 1 << 16, 150), ("float64", 1 << 16, 80), ("float32", 1 << 15, 300)])
+def test_cosft1_plan_chunked(dt, n, b):
+    # A batched device plan beyond one group runs its four-step C2C passes
+    # chunk by chunk; these batches span two chunks, the second partial.
+    # Signals of both chunks against the oracle, and the plan's result is its
+    # own inverse up to n/2.
+    import torch
+
+    npdt = np.float32 if dt == "float32" else np.float64
+    tdt = torch.float32 if dt == "float32" else torch.float64
+    x = _in(80, (n + 1) * b, npdt)
+    plan = numrs_amd.Cosft1Plan(dt, n, b)
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty_like(xt)
+    plan.execute(xt, yt)
+    y = yt.cpu().numpy().astype(np.float64)
+    for i in (0, b // 2, b - 1):
+        seg = slice(i * (n + 1), (i + 1) * (n + 1))
+        ref = oracle.cosft1(x[seg].astype(np.float64))
+        assert rel_l2(y[seg], ref) < 5 * TOL[npdt], i
+    zt = torch.empty_like(xt)
+    plan.execute(yt, zt)
+    assert rel_l2(zt.cpu().numpy().astype(np.float64), 0.5 * n * x.astype(np.float64)) < 20 * TOL[npdt]
+    assert zt.dtype == tdt
+
+
+@pytest.mark.parametrize("dt,n,b", [("float32", 1 << 16, 300), ("float64", 1 << 16, 150), ("float32", 1 << 17, 80)])
+@pytest.mark.parametrize("isign", [1, -1])
+def test_cosft2_plan_chunked(dt, n, b, isign):
+    # As above for cosft2 in both directions.  The forward direction also runs
+    # its permute-pack and its untangle-and-rotate step per chunk, through a
+    # chunk-sized scratch; the inverse runs those two over the whole batch.
+    import torch
+
+    npdt = np.float32 if dt == "float32" else np.float64
+    x = _in(81, n * b, npdt)
+    plan = numrs_amd.Cosft2Plan(dt, n, b)
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty_like(xt)
+    plan.execute(xt, yt, isign)
+    y = yt.cpu().numpy().astype(np.float64)
+    for i in (0, b // 2, b - 1):
+        seg = slice(i * n, (i + 1) * n)
+        ref = oracle.cosft2(x[seg].astype(np.float64), isign)
+        assert rel_l2(y[seg], ref) < 5 * TOL[npdt], (i, isign)
+
+
 def test_cosft2_invalid_isign():
     with pytest.raises(ValueError, match="Invalid isign value"):   # Cos_FT2.rs:265-271
         numrs_amd.cosft2(np.zeros(8), 8, 0)
