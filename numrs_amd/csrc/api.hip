@@ -291,8 +291,27 @@ int fourstep_setup(const Plan& p, int logn, FourStep* fs) {
     return NRFFT_OK;
 }
 
+// Swaps a load-mapped kernel of the same tile shape into a pass launch that
+// c2c_pass_launch sized for the plain kernel (the plan forces isign +1).
+static int use_map_kernel(const Plan& p, const KernelRef& km, const PassDesc& d, Launch* l) {
+    if (!km.fn || km.ti != l->ti || km.nt != l->nt) return fail(NRFFT_E_UNSUPPORTED, "no load-mapped first pass");
+    DeviceCtx* ctx = nullptr;
+    if (int rc = get_ctx(&ctx)) return rc;
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, km.fn, km.nt, 0));
+    if (per_cu < 1) per_cu = 1;
+    if (p.tune.wg_per_cu > 0 && p.tune.wg_per_cu < per_cu) per_cu = p.tune.wg_per_cu;
+    const int64_t ntiles = ((d.nb0 + km.ti - 1) / km.ti) * d.nb1 * d.nb2;
+    l->fn[0] = l->fn[1] = km.fn;
+    l->fn_dma[0] = l->fn_dma[1] = nullptr;
+    l->grid = (unsigned)std::min<int64_t>(ntiles, int64_t(per_cu) * ctx->ncu);
+    l->lds = km.lds_bytes;
+    return NRFFT_OK;
+}
+
 // Pass A and pass B of one chunk of cb transforms (offsets left at 0).
-int fourstep_launches(const Plan& p, const FourStep& fs, int64_t cb, int src, int dst, Launch* la, Launch* lb) {
+int fourstep_launches(const Plan& p, const FourStep& fs, int64_t cb, int src, int dst, Launch* la, Launch* lb,
+                      int lmap, int64_t sig_reals) {
     const int64_t N = int64_t(1) << fs.logn, N1 = int64_t(1) << fs.l1, N2 = int64_t(1) << fs.l2;
     // Pass A: FFT over n1 (stride N2), tile over n2, twiddle W_N^{k1 n2}; -> tmp[k1][n2].
     PassDesc a{};
@@ -307,7 +326,11 @@ int fourstep_launches(const Plan& p, const FourStep& fs, int64_t cb, int src, in
     a.nb2 = 1;
     a.tw_logn = fs.logn;
     a.tw_b0_shift = 0;
+    if (lmap != LMAP_PLAIN) a.si1 = sig_reals;
     if (int rc = c2c_pass_launch(p, MODE_TT, fs.l1, fs.ta, a, src, B_TMP, la)) return rc;
+    if (lmap != LMAP_PLAIN) {
+        if (int rc = use_map_kernel(p, map_lookup(p.dtype, lmap, fs.l1, fs.ta), a, la)) return rc;
+    }
     // Pass B: FFT over n2 (contiguous rows of tmp), rows k1 tiled; X[k2 N1 + k1].
     PassDesc b{};
     b.nb0 = N1;
@@ -477,7 +500,8 @@ int exec_launch(const Plan& p, const Launch& l, void* const* bufs, int isign, hi
     if (l.args[7] != 0 && ((l.args[7] > 0) != (isign > 0))) return NRFFT_OK;
     const int eff = l.args[6] != 0 ? (int)l.args[6] : isign;
     const int si = eff > 0 ? 0 : 1;
-    void* a = static_cast<char*>(bufs[l.src]) + l.off_src * (int64_t)csize(p.dtype);
+    void* a = static_cast<char*>(bufs[l.src]) + l.off_src * (int64_t)csize(p.dtype) +
+              l.off_src_real * (int64_t)(csize(p.dtype) / 2);
     void* b = static_cast<char*>(bufs[l.dst]) + l.off_dst * (int64_t)csize(p.dtype);
     if (l.kind == LK_COPY) {
         if (a != b) HIPCHK(hipMemcpyAsync(b, a, l.bytes, hipMemcpyDeviceToDevice, stream));

@@ -98,6 +98,7 @@ struct Launch {
     int nt_dma = 0;
     int64_t off_src = 0;  // element offsets (complex units) into src / dst
     int64_t off_dst = 0;
+    int64_t off_src_real = 0;  // further offset into src in reals (load-mapped passes)
     size_t bytes = 0;   // LK_COPY
     int special = 0;    // LK_REAL: which real-FFT kernel
     // args[0..5]: kernel arguments of LK_REAL launches.
@@ -151,7 +152,13 @@ struct FourStep {
     int ta = 0, tb = 0;            // log2 tile widths of pass A / pass B
 };
 int fourstep_setup(const Plan& p, int logn, FourStep* fs);
-int fourstep_launches(const Plan& p, const FourStep& fs, int64_t cb, int src, int dst, Launch* la, Launch* lb);
+// lmap != LMAP_PLAIN: pass A reads a real array of sig_reals reals per
+// signal through that load map (isign +1 only).
+int fourstep_launches(const Plan& p, const FourStep& fs, int64_t cb, int src, int dst, Launch* la, Launch* lb,
+                      int lmap = LMAP_PLAIN, int64_t sig_reals = 0);
+// The load-mapped first pass of (dtype, load map, log2 L, log2 tile width);
+// fn == nullptr if none.
+const KernelRef& map_lookup(int dtype, int lmap, int logl, int tilog);
 int exec_launch(const Plan& p, const Launch& l, void* const* bufs, int isign, hipStream_t stream, DeviceCtx* ctx);
 int col_tilog(int dtype);
 int exec_plan(const Plan& p, const void* in, void* out, void* tmp, const void* aux, int isign, hipStream_t stream,

@@ -98,9 +98,36 @@ __device__ __forceinline__ int fresh_tid(int tid) {
     return r;
 }
 
-template <typename T, int S, int L, int TI, int NT, int ML, int MS>
+// Load maps of a column pass (LMAP) over transforms of N = L * esi complex
+// points whose input is a real array of d.si1 reals per signal: complex
+// element m of a signal is the pair (x_{2m}, x_{2m+1}) of a sequence x read
+// straight from the reals y, so no packed copy of x is written first.
+//   LMAP_PLAIN   the complex input as it lies (every other pass);
+//   LMAP_EVENEXT x_d = y_{min(d, 2N - d)}: the 2N-point even extension of the
+//                N + 1 reals y_0 .. y_N (the DCT-I's sequence);
+//   LMAP_EVENODD x_i = y_{2i} (i < N), x_{2N-1-i} = y_{2i+1}: the even samples
+//                ascending, then the odd ones descending (the DCT-II's).
+// The first-pass tile holds every element i of its TI transforms, so with
+// si0 == 1 the element is m = b0 + t + i * esi.
+enum LoadMap { LMAP_PLAIN = 0, LMAP_EVENEXT = 1, LMAP_EVENODD = 2, LMAP_COUNT = 3 };
+
+// Positions in y of x_{2m} and x_{2m+1} under map MAP (N complex points).
+template <int MAP>
+__device__ __forceinline__ void map_pair(uint32_t m, uint32_t N, uint32_t& p0, uint32_t& p1) {
+    const bool lo = 2 * m < N;
+    if constexpr (MAP == LMAP_EVENEXT) {
+        p0 = lo ? 2 * m : 2 * N - 2 * m;
+        p1 = lo ? p0 + 1 : p0 - 1;
+    } else {
+        p0 = lo ? 4 * m : 4 * N - 4 * m - 1;
+        p1 = lo ? p0 + 2 : p0 - 2;
+    }
+}
+
+template <typename T, int S, int L, int TI, int NT, int ML, int MS, int LMAP = LMAP_PLAIN>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, NT), amdgpu_waves_per_eu(kc2c_waves<T, S, L, TI, NT, ML, MS>))) void
 k_c2c(const cpx<T>* in, cpx<T>* out, PassDesc d, Twiddles<T> tw) {
+    static_assert(LMAP == LMAP_PLAIN || ML == MAP_T, "load maps belong to column passes");
     using E = Engine<T, S, L, TI, NT, ML, MS>;
     using V2 = T __attribute__((ext_vector_type(2)));
     constexpr int EPT = E::EPT;
@@ -118,6 +145,7 @@ k_c2c(const cpx<T>* in, cpx<T>* out, PassDesc d, Twiddles<T> tw) {
 
     struct Ctx {
         const cpx<T>* ib;
+        const T* yb;  // LMAP: the signal's reals
         cpx<T>* ob;
         int64_t b0base;
         int nvalid;
@@ -131,12 +159,24 @@ k_c2c(const cpx<T>* in, cpx<T>* out, PassDesc d, Twiddles<T> tw) {
         const int64_t rem = d.nb0 - c.b0base;
         c.nvalid = rem < TI ? (int)rem : TI;
         c.ib = in + c.b0base * si0 + b1 * d.si1 + b2 * d.si2;
+        c.yb = reinterpret_cast<const T*>(in) + b1 * d.si1;  // LMAP: si1 counts reals, nb2 == 1
         c.ob = out + c.b0base * so0 + b1 * d.so1 + b2 * d.so2;
         return c;
     };
     auto load = [&](cpx<T>* v, const Ctx& c, int ltid) {
         E::load0(v, ltid, [&](int t, int i) -> cpx<T> {
-            if (TI == 1 || t < c.nvalid) return c.ib[t * si0 + i * esi];
+            if (TI == 1 || t < c.nvalid) {
+                if constexpr (LMAP == LMAP_PLAIN) {
+                    return c.ib[t * si0 + i * esi];
+                } else {
+                    // 32-bit indices: N <= 2^26 points, 4N reals at most.
+                    const uint32_t N = uint32_t(L) * uint32_t(esi);
+                    const uint32_t m = uint32_t(c.b0base) + uint32_t(t) + uint32_t(i) * uint32_t(esi);
+                    uint32_t p0, p1;
+                    map_pair<LMAP>(m, N, p0, p1);
+                    return cpx<T>{c.yb[p0], c.yb[p1]};
+                }
+            }
             return cpx<T>{(T)0, (T)0};
         });
     };

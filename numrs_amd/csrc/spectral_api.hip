@@ -96,6 +96,84 @@ static int plan_spec_chunked(Plan& p, int logn, int64_t batch, int force, int on
     return NRFFT_OK;
 }
 
+const KernelRef& map_lookup(int dtype, int lmap, int logl, int tilog) {
+    return dtype == NRFFT_F32 ? map_lookup_f32(lmap, logl, tilog) : map_lookup_f64(lmap, logl, tilog);
+}
+
+// NRFFT_SPEC_LOAD_MAP = 0: the DCTs beyond one group pack their sequence
+// into the workspace before the C2C instead of reading it in the first pass.
+#ifndef NRFFT_SPEC_LOAD_MAP
+#define NRFFT_SPEC_LOAD_MAP 1
+#endif
+
+// True when the C2C of length 2^logn can read its input through load map
+// lmap: a four-step length whose first pass has a load-mapped kernel of the
+// plan's tile width.
+static bool spec_map_fits(const Plan& p, int logn, int lmap) {
+    if (!NRFFT_SPEC_LOAD_MAP || logn <= C2C_MAX_LOGL_ || logn > 2 * C2C_MAX_LOGL_) return false;
+    FourStep fs;
+    if (fourstep_setup(p, logn, &fs)) return false;
+    return map_lookup(p.dtype, lmap, fs.l1, fs.ta).fn != nullptr;
+}
+
+// A DCT's C2C of length N = 2^logn whose first pass reads the caller's reals
+// (sig_reals per signal) through load map lmap, chunk by chunk, isign +1,
+// for calls of direction `only` (0 = both).  With per_chunk the finishing
+// launch post(c0, cb) follows each chunk's second pass, which then stores
+// plainly into a chunk-sized TMP2, so the finish reads the chunk from the
+// caches (MALL); otherwise TMP2 holds the whole batch and post(0, batch)
+// runs once after the last chunk.  Call only when spec_map_fits.
+#ifndef NRFFT_SPEC_MAP_CHUNK
+#define NRFFT_SPEC_MAP_CHUNK 0  // > 0: signals per chunk of the per-chunk form
+#endif
+template <class POST>
+static int plan_spec_mapped(Plan& p, int logn, int64_t batch, int lmap, int64_t sig_reals, int only, bool per_chunk,
+                            POST post) {
+    FourStep fs;
+    if (int rc = fourstep_setup(p, logn, &fs)) return rc;
+    const int64_t N = int64_t(1) << logn;
+    int64_t chunk = (p.tune.chunk > 0 && p.tune.chunk < batch) ? p.tune.chunk : batch;
+    if (per_chunk && NRFFT_SPEC_MAP_CHUNK > 0) chunk = std::min<int64_t>(batch, NRFFT_SPEC_MAP_CHUNK);
+    for (int64_t c0 = 0; c0 < batch; c0 += chunk) {
+        const int64_t cb = std::min<int64_t>(chunk, batch - c0);
+        Launch la, lb;
+        if (int rc = fourstep_launches(p, fs, cb, B_IN, B_TMP2, &la, &lb, lmap, sig_reals)) return rc;
+        la.off_src_real = c0 * sig_reals;
+        if (per_chunk) lb.desc.nt_out = 0;
+        else lb.off_dst = c0 * N;
+        for (Launch* l : {&la, &lb}) {
+            l->args[6] = 1;
+            l->args[7] = only;
+            p.launches.push_back(*l);
+        }
+        if (per_chunk) {
+            Launch b = post(c0, cb);
+            b.args[7] = only;
+            p.launches.push_back(b);
+        }
+    }
+    if (!per_chunk) {
+        Launch b = post(int64_t(0), batch);
+        b.args[7] = only;
+        p.launches.push_back(b);
+    }
+    p.tmp2_offset = std::max<size_t>(p.tmp2_offset, size_t(N) * size_t(chunk));  // the C2C scratch (B_TMP) comes first
+    p.tmp_elems = std::max<size_t>(p.tmp_elems, p.tmp2_offset + size_t(N) * size_t(per_chunk ? chunk : batch));
+    return NRFFT_OK;
+}
+
+// Where the finishing step of a load-mapped DCT runs (1 = after each chunk).
+// At 64 x 2^20 FP32 the whole-batch finish measured faster for both:
+// cosft1 0.652 ms against 0.783 (0.672 / 0.740 with chunks of 8 / 4
+// signals), cosft2 forward 0.379 against 0.451 (0.415 / 0.546); a chunk's
+// intermediate and its second pass's output together fill the MALL.
+#ifndef NRFFT_CF1_MAP_CHUNKED
+#define NRFFT_CF1_MAP_CHUNKED 0
+#endif
+#ifndef NRFFT_CF2_MAP_CHUNKED
+#define NRFFT_CF2_MAP_CHUNKED 0
+#endif
+
 int build_plan_spectral(Plan& p) {
     const int64_t batch = (int64_t)p.batch;
     size_t tmp = 0;
@@ -134,11 +212,20 @@ int build_plan_spectral(Plan& p) {
                 p.launches.push_back(spec_launch(SP_CF1_FUSED, B_IN, B_OUT, {batch, n}));
                 return NRFFT_OK;
             }
+            use_c2c_tune(p, logn);
+            if (spec_map_fits(p, logn, LMAP_EVENEXT)) {
+                const int64_t row = (n + 1) * (int64_t(csize(p.dtype)) / 2);  // bytes of one signal
+                return plan_spec_mapped(p, logn, batch, LMAP_EVENEXT, n + 1, 0, NRFFT_CF1_MAP_CHUNKED != 0,
+                                        [&](int64_t c0, int64_t cb) {
+                                            Launch l = spec_launch(SP_CF1_POST, B_TMP2, B_OUT, {cb, n});
+                                            l.off_dst = c0 * row;
+                                            return l;
+                                        });
+            }
             // Whole batch per step: the chunked form (pack, passes and finish
             // per chunk through a chunk-sized scratch) measured 0.833 ms
             // against 0.818 at 64 x 2^20 FP32.
             p.launches.push_back(spec_launch(SP_CF1_PACK, B_IN, B_TMP2, {batch, n}));
-            use_c2c_tune(p, logn);
             const size_t from = p.launches.size();
             if ((rc = plan_c2c_1d(p, logn, batch, B_TMP2, B_TMP2, &tmp))) return rc;
             tag_launches(p, from, 1, 0);
@@ -161,7 +248,14 @@ int build_plan_spectral(Plan& p) {
             use_c2c_tune(p, logl);
             const int64_t sig = n * (int64_t(csize(p.dtype)) / 2);  // bytes of one signal
             // forward: permute-pack -> C2C(+1) -> untangle and rotate
-            rc = plan_spec_chunked(
+            if (spec_map_fits(p, logl, LMAP_EVENODD)) {
+                rc = plan_spec_mapped(p, logl, batch, LMAP_EVENODD, n, 1, NRFFT_CF2_MAP_CHUNKED != 0,
+                                      [&](int64_t c0, int64_t cb) {
+                                          Launch l = spec_launch(SP_CF2_FPOST, B_TMP2, B_OUT, {cb, L});
+                                          l.off_dst = c0 * sig;
+                                          return l;
+                                      });
+            } else rc = plan_spec_chunked(
                 p, logl, batch, 1, 1,
                 [&](int64_t c0, int64_t cb) {
                     Launch l = spec_launch(SP_CF2_FPACK, B_IN, B_TMP2, {cb, L});
@@ -175,8 +269,10 @@ int build_plan_spectral(Plan& p) {
                 });
             if (rc) return rc;
             // inverse: rotate and untangle -> C2C(-1) -> unpermute, whole batch
-            // per step (chunked: 0.455 ms against 0.444 at 64 x 2^20 FP32;
-            // the forward direction gains from it, 0.440 -> 0.409 ms).
+            // per step (at 64 x 2^20 FP32, chunked: 0.455 ms against 0.444;
+            // the C2C's second pass storing through the permutation straight
+            // into y instead of the unpermute step: 0.448 against 0.447, its
+            // 4-byte stores filling half of each line, so it was not kept).
             Launch c = spec_launch(SP_CF2_IPRE, B_IN, B_TMP2, {batch, L});
             c.args[7] = -1;
             p.launches.push_back(c);

@@ -166,6 +166,55 @@ def test_cosft2_plan_chunked(dt, n, b, isign):
         assert rel_l2(y[seg], ref) < 5 * TOL[npdt], (i, isign)
 
 
+@pytest.mark.parametrize("dt,n,b", [("float64", 1 << 20, 3), ("float32", 1 << 20, 5), ("float32", 1 << 22, 2)])
+def test_cosft1_plan_long(dt, n, b):
+    # Long DCT-Is whose first C2C pass reads the even extension of each
+    # signal straight from its n + 1 reals (signals at odd real offsets):
+    # 512- and 2048-point first passes, 8-wide tiles.
+    import scipy.fft as sfft
+    import torch
+
+    npdt = np.float32 if dt == "float32" else np.float64
+    x = _in(82, (n + 1) * b, npdt)
+    plan = numrs_amd.Cosft1Plan(dt, n, b)
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty_like(xt)
+    plan.execute(xt, yt)
+    y = yt.cpu().numpy().astype(np.float64)
+    for i in range(b):
+        seg = slice(i * (n + 1), (i + 1) * (n + 1))
+        ref = oracle.cosft1(x[seg].astype(np.float64))
+        exact = 0.5 * sfft.dct(x[seg].astype(np.float64), type=1)
+        assert rel_l2(y[seg], ref) < _oracle_bound(npdt, ref, exact), i
+        if npdt == np.float64:
+            assert rel_l2(y[seg], exact) < 2e-13, i
+
+
+@pytest.mark.parametrize("dt,n,b", [("float64", 1 << 21, 3), ("float64", 1 << 19, 3), ("float32", 1 << 21, 5)])
+@pytest.mark.parametrize("isign", [1, -1])
+def test_cosft2_plan_long(dt, n, b, isign):
+    # As above for cosft2: the forward first pass reads the even samples
+    # ascending and the odd ones descending straight from the signal (FP64
+    # 512-point first passes 8 wide at 2^21, 256-point 16 wide at 2^19).
+    import scipy.fft as sfft
+    import torch
+
+    npdt = np.float32 if dt == "float32" else np.float64
+    x = _in(83, n * b, npdt)
+    plan = numrs_amd.Cosft2Plan(dt, n, b)
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty_like(xt)
+    plan.execute(xt, yt, isign)
+    y = yt.cpu().numpy().astype(np.float64)
+    for i in range(b):
+        seg = slice(i * n, (i + 1) * n)
+        ref = oracle.cosft2(x[seg].astype(np.float64), isign)
+        exact = 0.5 * sfft.dct(x[seg].astype(np.float64), type=2 if isign == 1 else 3)
+        assert rel_l2(y[seg], ref) < _oracle_bound(npdt, ref, exact), (i, isign)
+        if npdt == np.float64:
+            assert rel_l2(y[seg], exact) < 2e-13, (i, isign)
+
+
 def test_cosft2_invalid_isign():
     with pytest.raises(ValueError, match="Invalid isign value"):   # Cos_FT2.rs:265-271
         numrs_amd.cosft2(np.zeros(8), 8, 0)
