@@ -219,6 +219,18 @@ size_t csize(int dtype) { return dtype == NRFFT_F32 ? 8 : 16; }
 // Preferred tile width (log2) for column-mapped passes.
 int col_tilog(int dtype) { return dtype == NRFFT_F32 ? 4 : 3; }
 
+// NRFFT_C2C_OVERLAP = 1: batched 1-D C2C plans of several chunks run each
+// chunk's second pass on a second stream, beside the next chunk's first
+// (exec_overlap).  Results are bit-identical, but on MI355X it is slower:
+// FP32 64 x 2^20 0.569 ms against 0.441, FP64 32 x 2^20 0.532 against
+// 0.409, FP32 256 x 2^18 0.525 against 0.434, FP32 32 x 2^22 1.570 against
+// 1.208.  Both passes are bound by the same HBM stream, so running them
+// side by side adds no bandwidth, and the two scratch halves push the
+// intermediate out of the MALL.  Off.
+#ifndef NRFFT_C2C_OVERLAP
+#define NRFFT_C2C_OVERLAP 0
+#endif
+
 // 1-D batched C2C of length N = 2^p over `batch` contiguous transforms, from
 // buffer src to buffer dst (B_TMP used as the intermediate of two-pass sizes).
 // Output element k of transform b lands at dst[b*N*ostride_b... ] -- contiguous.
@@ -252,16 +264,21 @@ int plan_c2c_1d(Plan& p, int logn, int64_t batch, int src, int dst, size_t* tmp_
     // Batch chunking: pass A and pass B of one chunk run back to back so the
     // chunk's intermediate can stay in the last-level cache (256 MB MALL).
     const int64_t chunk = (p.tune.chunk > 0 && p.tune.chunk < batch) ? p.tune.chunk : batch;
-    *tmp_elems = std::max<size_t>(*tmp_elems, size_t(N) * size_t(chunk));
-    for (int64_t c0 = 0; c0 < batch; c0 += chunk) {
+    // A C2C plan of several chunks alone in its plan may overlap chunk k's
+    // pass B with chunk k+1's pass A (two scratch halves).
+    const bool overlap = NRFFT_C2C_OVERLAP && p.kind == PK_C2C && p.launches.empty() && chunk < batch;
+    *tmp_elems = std::max<size_t>(*tmp_elems, size_t(N) * size_t(chunk) * (overlap ? 2 : 1));
+    for (int64_t c0 = 0, k = 0; c0 < batch; c0 += chunk, ++k) {
         const int64_t cb = std::min<int64_t>(chunk, batch - c0);
         Launch la, lb;
         if (int rc = fourstep_launches(p, fs, cb, src, dst, &la, &lb)) return rc;
         la.off_src = c0 * N;
         lb.off_dst = c0 * N;
+        if (overlap) la.off_dst = lb.off_src = (k & 1) * N * chunk;
         p.launches.push_back(la);
         p.launches.push_back(lb);
     }
+    p.overlap = overlap;
     return NRFFT_OK;
 }
 
@@ -479,6 +496,42 @@ int plan_c2c_nd(Plan& p, const std::vector<size_t>& nn, int64_t batch) {
 // ---------------------------------------------------------------------------
 // Execution.
 // ---------------------------------------------------------------------------
+// Overlapped execution of a plan with p.overlap: launches (A_k, B_k) per
+// chunk k, A_k writing scratch half k mod 2 and B_k reading it.  A_k runs on
+// the caller's stream s0, B_k on the device's auxiliary stream s1 (default
+// priority, non-blocking), ordered by events: B_k after A_k, A_k after
+// B_{k-2} (the scratch half it overwrites), and s0 after the last B, so
+// whatever the caller enqueues next on s0 sees the whole result.  Events
+// and the stream are per device, reused under one lock while enqueuing.
+struct OverlapCtx {
+    hipStream_t s1 = nullptr;
+    std::vector<hipEvent_t> ev;  // [2k] after A_k, [2k+1] after B_k
+};
+static std::mutex g_ov_mu;
+static std::map<int, OverlapCtx> g_ov;
+
+static int exec_overlap(const Plan& p, void* const* bufs, int isign, hipStream_t s0, DeviceCtx* ctx) {
+    std::lock_guard<std::mutex> lk(g_ov_mu);
+    OverlapCtx& o = g_ov[ctx->device];
+    if (!o.s1) HIPCHK(hipStreamCreateWithFlags(&o.s1, hipStreamNonBlocking));
+    const size_t nl = p.launches.size();
+    while (o.ev.size() < nl) {
+        hipEvent_t e = nullptr;
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        o.ev.push_back(e);
+    }
+    for (size_t k = 0; 2 * k + 1 < nl; ++k) {
+        if (k >= 2) HIPCHK(hipStreamWaitEvent(s0, o.ev[2 * (k - 2) + 1], 0));
+        if (int rc = exec_launch(p, p.launches[2 * k], bufs, isign, s0, ctx)) return rc;
+        HIPCHK(hipEventRecord(o.ev[2 * k], s0));
+        HIPCHK(hipStreamWaitEvent(o.s1, o.ev[2 * k], 0));
+        if (int rc = exec_launch(p, p.launches[2 * k + 1], bufs, isign, o.s1, ctx)) return rc;
+        HIPCHK(hipEventRecord(o.ev[2 * k + 1], o.s1));
+    }
+    HIPCHK(hipStreamWaitEvent(s0, o.ev[nl - 1], 0));
+    return NRFFT_OK;
+}
+
 int exec_plan(const Plan& p, const void* in, void* out, void* tmp, const void* aux, int isign, hipStream_t stream,
               int first, int count, void* aux2) {
     DeviceCtx* ctx = nullptr;
@@ -487,6 +540,7 @@ int exec_plan(const Plan& p, const void* in, void* out, void* tmp, const void* a
     int idx = -1;
     void* tmp2 = tmp ? static_cast<char*>(tmp) + p.tmp2_offset * csize(p.dtype) : nullptr;
     void* bufs[B_COUNT] = {const_cast<void*>(in), out, tmp, const_cast<void*>(aux), tmp2, aux2};
+    if (p.overlap && first == 0 && count < 0) return exec_overlap(p, bufs, isign, stream, ctx);
     for (const Launch& l : p.launches) {
         ++idx;
         if (idx < first || (count >= 0 && idx >= first + count)) continue;

@@ -122,6 +122,10 @@ struct Plan {
     // workspace) instead of the two-pass four-step along the axis: set when
     // the four-step's workspace could not be allocated.
     bool strided_one_pass = false;
+    // 1-D C2C four-step over several chunks whose launches alternate pass A,
+    // pass B per chunk with the scratch double-buffered: executed with each
+    // chunk's pass B on a second stream (exec_plan).
+    bool overlap = false;
 };
 
 extern thread_local std::string g_err;
