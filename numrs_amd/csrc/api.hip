@@ -309,7 +309,8 @@ int fourstep_setup(const Plan& p, int logn, FourStep* fs) {
 }
 
 // Swaps a load-mapped kernel of the same tile shape into a pass launch that
-// c2c_pass_launch sized for the plain kernel (the plan forces isign +1).
+// c2c_pass_launch sized for the plain kernel (the plan forces the map's
+// isign).
 static int use_map_kernel(const Plan& p, const KernelRef& km, const PassDesc& d, Launch* l) {
     if (!km.fn || km.ti != l->ti || km.nt != l->nt) return fail(NRFFT_E_UNSUPPORTED, "no load-mapped first pass");
     DeviceCtx* ctx = nullptr;

@@ -157,7 +157,7 @@ struct FourStep {
 };
 int fourstep_setup(const Plan& p, int logn, FourStep* fs);
 // lmap != LMAP_PLAIN: pass A reads a real array of sig_reals reals per
-// signal through that load map (isign +1 only).
+// signal through that load map (isign +1, or -1 for LMAP_CF2INV).
 int fourstep_launches(const Plan& p, const FourStep& fs, int64_t cb, int src, int dst, Launch* la, Launch* lb,
                       int lmap = LMAP_PLAIN, int64_t sig_reals = 0);
 // The load-mapped first pass of (dtype, load map, log2 L, log2 tile width);

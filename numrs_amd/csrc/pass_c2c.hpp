@@ -11,6 +11,7 @@
 #include <type_traits>
 
 #include "device_fft.hpp"
+#include "real_fft.hpp"
 
 namespace nrfft {
 
@@ -107,9 +108,11 @@ __device__ __forceinline__ int fresh_tid(int tid) {
 //                N + 1 reals y_0 .. y_N (the DCT-I's sequence);
 //   LMAP_EVENODD x_i = y_{2i} (i < N), x_{2N-1-i} = y_{2i+1}: the even samples
 //                ascending, then the odd ones descending (the DCT-II's).
+//   LMAP_CF2INV  z_m of the cosft2 inverse (map_cf2inv): the rotation and
+//                realft-inverse untangle of the 2N reals Y, isign -1.
 // The first-pass tile holds every element i of its TI transforms, so with
 // si0 == 1 the element is m = b0 + t + i * esi.
-enum LoadMap { LMAP_PLAIN = 0, LMAP_EVENEXT = 1, LMAP_EVENODD = 2, LMAP_COUNT = 3 };
+enum LoadMap { LMAP_PLAIN = 0, LMAP_EVENEXT = 1, LMAP_EVENODD = 2, LMAP_CF2INV = 3, LMAP_COUNT = 4 };
 // NRFFT_LMAP_PAIR = 1: load-mapped passes run each tile beside its mirror
 // on one XCD (ctx_of in k_c2c).  64 x 2^20 FP32 on MI355X: cosft1 0.657 ->
 // 0.620 ms (read requests past L2 3.6x -> 1.22x the bytes of y), cosft2
@@ -129,6 +132,37 @@ __device__ __forceinline__ void map_pair(uint32_t m, uint32_t N, uint32_t& p0, u
         p0 = lo ? 4 * m : 4 * N - 4 * m - 1;
         p1 = lo ? p0 + 2 : p0 - 2;
     }
+}
+
+// z_m, 0 <= m < N, of the cosft2 inverse (Cos_FT2.rs:116-173 inverse branch
+// read with NR's indices; k_cosft2_ipre computes the same per bin pair):
+// with n = 2N (logn = log2 n), the pair p = min(m, N - m) is
+//   V_k = exp(-i pi k / 2n) (Y_k + i Y_{n-k}),  k = p and N - p,
+// untangled with conj W_n^p (isign -1) into z_p and z_{N-p}; z_0 =
+// dc_inverse(Y_0, sqrt2 Y_N) and z_{N/2} pairs with itself.
+template <typename T>
+__device__ __forceinline__ cpx<T> map_cf2inv(const T* ys, uint32_t m, uint32_t N, int logn, const Twiddles<T>& tw) {
+    const uint32_t n = 2 * N, nq = N / 2;
+    if (m == 0) return dc_inverse(cpx<T>{ys[0], (T)1.41421356237309504880 * ys[N]});
+    cpx<T> za, zb;
+    if (m == nq) {
+        const cpx<T> vm = cmul(conj(twiddle_big<1>(tw, nq, logn + 2)), cpx<T>{ys[nq], ys[n - nq]});
+        untangle_pair(vm, vm, conj(twiddle_big<1>(tw, nq, logn)), (T)0.5, za, zb);
+        return zb;
+    }
+    const bool lo = m < nq;
+    const uint32_t p = lo ? m : N - m;
+    // One table twiddle per element: W_{4n}^p; W_{4n}^N = exp(i pi / 4) and
+    // W_n^p = (W_{4n}^p)^4 follow without loads.
+    const cpx<T> r4 = twiddle_big<1>(tw, p, logn + 2);
+    const T h = (T)0.70710678118654752440;
+    const cpx<T> w8{h, h};
+    const cpx<T> r2 = cmul(r4, r4);
+    const cpx<T> wn = cmul(r2, r2);
+    const cpx<T> va = cmul(conj(r4), cpx<T>{ys[p], ys[n - p]});
+    const cpx<T> vb = cmul(conj(cmul(w8, conj(r4))), cpx<T>{ys[N - p], ys[N + p]});
+    untangle_pair(va, vb, conj(wn), (T)0.5, za, zb);
+    return cpx<T>{lo ? za.x : zb.x, lo ? za.y : zb.y};  // per component: a struct select went through scratch
 }
 
 template <typename T, int S, int L, int TI, int NT, int ML, int MS, int LMAP = LMAP_PLAIN>
@@ -193,9 +227,13 @@ k_c2c(const cpx<T>* in, cpx<T>* out, PassDesc d, Twiddles<T> tw) {
                     // 32-bit indices: N <= 2^26 points, 4N reals at most.
                     const uint32_t N = uint32_t(L) * uint32_t(esi);
                     const uint32_t m = uint32_t(c.b0base) + uint32_t(t) + uint32_t(i) * uint32_t(esi);
-                    uint32_t p0, p1;
-                    map_pair<LMAP>(m, N, p0, p1);
-                    return cpx<T>{c.yb[p0], c.yb[p1]};
+                    if constexpr (LMAP == LMAP_CF2INV) {
+                        return map_cf2inv(c.yb, m, N, __builtin_ctz(N) + 1, tw);
+                    } else {
+                        uint32_t p0, p1;
+                        map_pair<LMAP>(m, N, p0, p1);
+                        return cpx<T>{c.yb[p0], c.yb[p1]};
+                    }
                 }
             }
             return cpx<T>{(T)0, (T)0};

@@ -117,8 +117,8 @@ static bool spec_map_fits(const Plan& p, int logn, int lmap) {
 }
 
 // A DCT's C2C of length N = 2^logn whose first pass reads the caller's reals
-// (sig_reals per signal) through load map lmap, chunk by chunk, isign +1,
-// for calls of direction `only` (0 = both).  With per_chunk the finishing
+// (sig_reals per signal) through load map lmap, chunk by chunk, with isign
+// `force`, for calls of direction `only` (0 = both).  With per_chunk the finishing
 // launch post(c0, cb) follows each chunk's second pass, which then stores
 // plainly into a chunk-sized TMP2, so the finish reads the chunk from the
 // caches (MALL); otherwise TMP2 holds the whole batch and post(0, batch)
@@ -127,8 +127,8 @@ static bool spec_map_fits(const Plan& p, int logn, int lmap) {
 #define NRFFT_SPEC_MAP_CHUNK 0  // > 0: signals per chunk of the per-chunk form
 #endif
 template <class POST>
-static int plan_spec_mapped(Plan& p, int logn, int64_t batch, int lmap, int64_t sig_reals, int only, bool per_chunk,
-                            POST post) {
+static int plan_spec_mapped(Plan& p, int logn, int64_t batch, int lmap, int64_t sig_reals, int force, int only,
+                            bool per_chunk, POST post) {
     FourStep fs;
     if (int rc = fourstep_setup(p, logn, &fs)) return rc;
     const int64_t N = int64_t(1) << logn;
@@ -142,7 +142,7 @@ static int plan_spec_mapped(Plan& p, int logn, int64_t batch, int lmap, int64_t 
         if (per_chunk) lb.desc.nt_out = 0;
         else lb.off_dst = c0 * N;
         for (Launch* l : {&la, &lb}) {
-            l->args[6] = 1;
+            l->args[6] = force;
             l->args[7] = only;
             p.launches.push_back(*l);
         }
@@ -215,7 +215,7 @@ int build_plan_spectral(Plan& p) {
             use_c2c_tune(p, logn);
             if (spec_map_fits(p, logn, LMAP_EVENEXT)) {
                 const int64_t row = (n + 1) * (int64_t(csize(p.dtype)) / 2);  // bytes of one signal
-                return plan_spec_mapped(p, logn, batch, LMAP_EVENEXT, n + 1, 0, NRFFT_CF1_MAP_CHUNKED != 0,
+                return plan_spec_mapped(p, logn, batch, LMAP_EVENEXT, n + 1, 1, 0, NRFFT_CF1_MAP_CHUNKED != 0,
                                         [&](int64_t c0, int64_t cb) {
                                             Launch l = spec_launch(SP_CF1_POST, B_TMP2, B_OUT, {cb, n});
                                             l.off_dst = c0 * row;
@@ -249,7 +249,7 @@ int build_plan_spectral(Plan& p) {
             const int64_t sig = n * (int64_t(csize(p.dtype)) / 2);  // bytes of one signal
             // forward: permute-pack -> C2C(+1) -> untangle and rotate
             if (spec_map_fits(p, logl, LMAP_EVENODD)) {
-                rc = plan_spec_mapped(p, logl, batch, LMAP_EVENODD, n, 1, NRFFT_CF2_MAP_CHUNKED != 0,
+                rc = plan_spec_mapped(p, logl, batch, LMAP_EVENODD, n, 1, 1, NRFFT_CF2_MAP_CHUNKED != 0,
                                       [&](int64_t c0, int64_t cb) {
                                           Launch l = spec_launch(SP_CF2_FPOST, B_TMP2, B_OUT, {cb, L});
                                           l.off_dst = c0 * sig;
@@ -268,11 +268,21 @@ int build_plan_spectral(Plan& p) {
                     return l;
                 });
             if (rc) return rc;
-            // inverse: rotate and untangle -> C2C(-1) -> unpermute, whole batch
-            // per step (at 64 x 2^20 FP32, chunked: 0.455 ms against 0.444;
-            // the C2C's second pass storing through the permutation straight
-            // into y instead of the unpermute step: 0.448 against 0.447, its
-            // 4-byte stores filling half of each line, so it was not kept).
+            // inverse: rotate and untangle -> C2C(-1) -> unpermute.  With a
+            // load-mapped first pass the rotation and untangle happen in its
+            // loads (64 x 2^20 FP32: 0.390 ms against 0.444 for the separate
+            // pre-step).  Otherwise whole batch per step (chunked: 0.455 ms
+            // against 0.444; the C2C's second pass storing through the
+            // permutation straight into y instead of the unpermute step:
+            // 0.448 against 0.447, its 4-byte stores filling half of each
+            // line, so it was not kept).
+            if (spec_map_fits(p, logl, LMAP_CF2INV)) {
+                // The rotation and untangle as the C2C's first-pass load map.
+                return plan_spec_mapped(p, logl, batch, LMAP_CF2INV, n, -1, -1, false, [&](int64_t, int64_t cb) {
+                    Launch d = spec_launch(SP_CF2_IUNPACK, B_TMP2, B_OUT, {cb, L});
+                    return d;
+                });
+            }
             Launch c = spec_launch(SP_CF2_IPRE, B_IN, B_TMP2, {batch, L});
             c.args[7] = -1;
             p.launches.push_back(c);
