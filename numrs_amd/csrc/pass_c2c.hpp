@@ -140,29 +140,51 @@ __device__ __forceinline__ void map_pair(uint32_t m, uint32_t N, uint32_t& p0, u
 //   V_k = exp(-i pi k / 2n) (Y_k + i Y_{n-k}),  k = p and N - p,
 // untangled with conj W_n^p (isign -1) into z_p and z_{N-p}; z_0 =
 // dc_inverse(Y_0, sqrt2 Y_N) and z_{N/2} pairs with itself.
+// cf2inv_core: the general element (0 < p < N/2) given r4 = W_{4n}^p;
+// W_{4n}^N = exp(i pi / 4) and W_n^p = (W_{4n}^p)^4 need no table.
 template <typename T>
-__device__ __forceinline__ cpx<T> map_cf2inv(const T* ys, uint32_t m, uint32_t N, int logn, const Twiddles<T>& tw) {
-    const uint32_t n = 2 * N, nq = N / 2;
-    if (m == 0) return dc_inverse(cpx<T>{ys[0], (T)1.41421356237309504880 * ys[N]});
-    cpx<T> za, zb;
-    if (m == nq) {
-        const cpx<T> vm = cmul(conj(twiddle_big<1>(tw, nq, logn + 2)), cpx<T>{ys[nq], ys[n - nq]});
-        untangle_pair(vm, vm, conj(twiddle_big<1>(tw, nq, logn)), (T)0.5, za, zb);
-        return zb;
-    }
-    const bool lo = m < nq;
-    const uint32_t p = lo ? m : N - m;
-    // One table twiddle per element: W_{4n}^p; W_{4n}^N = exp(i pi / 4) and
-    // W_n^p = (W_{4n}^p)^4 follow without loads.
-    const cpx<T> r4 = twiddle_big<1>(tw, p, logn + 2);
+__device__ __forceinline__ cpx<T> cf2inv_core(const T* ys, uint32_t p, uint32_t N, cpx<T> r4, bool lo) {
+    const uint32_t n = 2 * N;
     const T h = (T)0.70710678118654752440;
     const cpx<T> w8{h, h};
     const cpx<T> r2 = cmul(r4, r4);
     const cpx<T> wn = cmul(r2, r2);
     const cpx<T> va = cmul(conj(r4), cpx<T>{ys[p], ys[n - p]});
     const cpx<T> vb = cmul(conj(cmul(w8, conj(r4))), cpx<T>{ys[N - p], ys[N + p]});
+    cpx<T> za, zb;
     untangle_pair(va, vb, conj(wn), (T)0.5, za, zb);
     return cpx<T>{lo ? za.x : zb.x, lo ? za.y : zb.y};  // per component: a struct select went through scratch
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> map_cf2inv(const T* ys, uint32_t m, uint32_t N, int logn, const Twiddles<T>& tw) {
+    const uint32_t n = 2 * N, nq = N / 2;
+    if (m == 0) return dc_inverse(cpx<T>{ys[0], (T)1.41421356237309504880 * ys[N]});
+    if (m == nq) {
+        cpx<T> za, zb;
+        const cpx<T> vm = cmul(conj(twiddle_big<1>(tw, nq, logn + 2)), cpx<T>{ys[nq], ys[n - nq]});
+        untangle_pair(vm, vm, conj(twiddle_big<1>(tw, nq, logn)), (T)0.5, za, zb);
+        return zb;
+    }
+    const bool lo = m < nq;
+    const uint32_t p = lo ? m : N - m;
+    return cf2inv_core(ys, p, N, twiddle_big<1>(tw, p, logn + 2), lo);
+}
+
+// exp(2 pi i r / 128), r < 16: the step between the rows a first-pass
+// butterfly loads, in units of the cosft2 inverse's rotation W_{4n} (rows
+// N/16 apart; index r 16/R0 for a stage-0 radix R0, rows N/R0 apart).
+template <typename T>
+__device__ __forceinline__ cpx<T> w128(int r) {
+    constexpr double c[16][2] = {
+        {1.00000000000000000000, 0.00000000000000000000}, {0.99879545620517240501, 0.04906767432741801493},
+        {0.99518472667219692873, 0.09801714032956060363}, {0.98917650996478101444, 0.14673047445536174793},
+        {0.98078528040323043058, 0.19509032201612824808}, {0.97003125319454397424, 0.24298017990326387094},
+        {0.95694033573220882438, 0.29028467725446233105}, {0.94154406518302080631, 0.33688985339222005111},
+        {0.92387953251128673848, 0.38268343236508978178}, {0.90398929312344333820, 0.42755509343028208491},
+        {0.88192126434835504956, 0.47139673682599764204}, {0.85772861000027211809, 0.51410274419322166128},
+        {0.83146961230254523567, 0.55557023301960217765}, {0.80320753148064494287, 0.59569930449243335691},
+        {0.77301045336273699338, 0.63439328416364548779}, {0.74095112535495910588, 0.67155895484701833009}};
+    return cpx<T>{(T)c[r][0], (T)c[r][1]};
 }
 
 template <typename T, int S, int L, int TI, int NT, int ML, int MS, int LMAP = LMAP_PLAIN>
@@ -219,6 +241,44 @@ k_c2c(const cpx<T>* in, cpx<T>* out, PassDesc d, Twiddles<T> tw) {
         return c;
     };
     auto load = [&](cpx<T>* v, const Ctx& c, int ltid) {
+        if constexpr (LMAP == LMAP_CF2INV) {
+            // Butterfly (t, j) of stage 0 loads rows i = j + r L/R0, r < R0:
+            // elements m = m0 + r N/R0 with m0 < N/R0, so r < R0/2 is the
+            // lower half (p = m) and W_{4n}^m = W_{4n}^{m0} exp(2 pi i r /
+            // (8 R0)): one table twiddle per butterfly.  The upper half uses
+            // W_{4n}^{N-m} = exp(i pi / 4) conj(W_{4n}^m); m = 0 and m = N/2
+            // (m0 = 0) take map_cf2inv.
+            constexpr int R0 = E::P::radix(0), NB0 = L / R0, BPT0 = EPT / R0;
+            static_assert(16 % R0 == 0, "stage-0 radix divides 16");
+            const uint32_t N = uint32_t(L) * uint32_t(esi);
+            const int logn = __builtin_ctz(N) + 1;
+            const uint32_t step = uint32_t(NB0) * uint32_t(esi);
+            const T h = (T)0.70710678118654752440;
+#pragma unroll
+            for (int b = 0; b < BPT0; ++b) {
+                int t, j;
+                E::template bmap<MAP_T, NB0>(ltid + b * NT, t, j);
+                if (TI == 1 || t < c.nvalid) {
+                    const uint32_t m0 = uint32_t(c.b0base) + uint32_t(t) + uint32_t(j) * uint32_t(esi);
+                    const cpx<T> w0 = twiddle_big<1>(tw, m0, logn + 2);
+#pragma unroll
+                    for (int r = 0; r < R0; ++r) {
+                        const uint32_t m = m0 + uint32_t(r) * step;
+                        if ((r == 0 || r == R0 / 2) && m0 == 0) {
+                            v[b * R0 + r] = map_cf2inv(c.yb, m, N, logn, tw);
+                        } else {
+                            const cpx<T> wm = cmul(w0, w128<T>(r * (16 / R0)));
+                            const cpx<T> r4 = r < R0 / 2 ? wm : cmul(cpx<T>{h, h}, conj(wm));
+                            v[b * R0 + r] = cf2inv_core(c.yb, r < R0 / 2 ? m : N - m, N, r4, r < R0 / 2);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < R0; ++r) v[b * R0 + r] = cpx<T>{(T)0, (T)0};
+                }
+            }
+            return;
+        }
         E::load0(v, ltid, [&](int t, int i) -> cpx<T> {
             if (TI == 1 || t < c.nvalid) {
                 if constexpr (LMAP == LMAP_PLAIN) {
@@ -227,13 +287,9 @@ k_c2c(const cpx<T>* in, cpx<T>* out, PassDesc d, Twiddles<T> tw) {
                     // 32-bit indices: N <= 2^26 points, 4N reals at most.
                     const uint32_t N = uint32_t(L) * uint32_t(esi);
                     const uint32_t m = uint32_t(c.b0base) + uint32_t(t) + uint32_t(i) * uint32_t(esi);
-                    if constexpr (LMAP == LMAP_CF2INV) {
-                        return map_cf2inv(c.yb, m, N, __builtin_ctz(N) + 1, tw);
-                    } else {
-                        uint32_t p0, p1;
-                        map_pair<LMAP>(m, N, p0, p1);
-                        return cpx<T>{c.yb[p0], c.yb[p1]};
-                    }
+                    uint32_t p0, p1;
+                    map_pair<LMAP>(m, N, p0, p1);
+                    return cpx<T>{c.yb[p0], c.yb[p1]};
                 }
             }
             return cpx<T>{(T)0, (T)0};

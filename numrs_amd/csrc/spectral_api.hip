@@ -270,7 +270,7 @@ int build_plan_spectral(Plan& p) {
             if (rc) return rc;
             // inverse: rotate and untangle -> C2C(-1) -> unpermute.  With a
             // load-mapped first pass the rotation and untangle happen in its
-            // loads (64 x 2^20 FP32: 0.390 ms against 0.444 for the separate
+            // loads (64 x 2^20 FP32: 0.362 ms against 0.444 for the separate
             // pre-step).  Otherwise whole batch per step (chunked: 0.455 ms
             // against 0.444; the C2C's second pass storing through the
             // permutation straight into y instead of the unpermute step:

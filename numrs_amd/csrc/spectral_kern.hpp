@@ -58,7 +58,7 @@ const SpecRef& spec_lookup_f64(int logl);
 // (k_c2c with a load map; isign +1, or -1 for LMAP_CF2INV; column mapping
 // on both sides), for pass lengths 2^MAP_MIN_LOGL .. 2^MAP_MAX_LOGL and tile
 // widths 8 and 16 (the planner's defaults for both precisions).
-constexpr int MAP_MIN_LOGL = 6, MAP_MAX_LOGL = 11, MAP_MIN_TILOG = 3, MAP_MAX_TILOG = 4;
+constexpr int MAP_MIN_LOGL = 7, MAP_MAX_LOGL = 11, MAP_MIN_TILOG = 3, MAP_MAX_TILOG = 4;
 
 template <typename T, int LMAP, int LOGL, int TILOG>
 KernelRef map_ref_one() {
