@@ -163,6 +163,10 @@ int fourstep_launches(const Plan& p, const FourStep& fs, int64_t cb, int src, in
 // The load-mapped first pass of (dtype, load map, log2 L, log2 tile width);
 // fn == nullptr if none.
 const KernelRef& map_lookup(int dtype, int lmap, int logl, int tilog);
+// True when the C2C of length 2^logn can read its input through load map
+// lmap: a four-step length whose first pass has a load-mapped kernel of the
+// plan's tile width.
+bool map_fits(const Plan& p, int logn, int lmap);
 int exec_launch(const Plan& p, const Launch& l, void* const* bufs, int isign, hipStream_t stream, DeviceCtx* ctx);
 int col_tilog(int dtype);
 int exec_plan(const Plan& p, const void* in, void* out, void* tmp, const void* aux, int isign, hipStream_t stream,

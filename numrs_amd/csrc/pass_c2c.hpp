@@ -110,9 +110,11 @@ __device__ __forceinline__ int fresh_tid(int tid) {
 //                ascending, then the odd ones descending (the DCT-II's).
 //   LMAP_CF2INV  z_m of the cosft2 inverse (map_cf2inv): the rotation and
 //                realft-inverse untangle of the 2N reals Y, isign -1.
+//   LMAP_RFINV   z_m of the realft inverse (rfinv_core): the inverse
+//                untangle of the packed spectrum d (N complex), isign -1.
 // The first-pass tile holds every element i of its TI transforms, so with
 // si0 == 1 the element is m = b0 + t + i * esi.
-enum LoadMap { LMAP_PLAIN = 0, LMAP_EVENEXT = 1, LMAP_EVENODD = 2, LMAP_CF2INV = 3, LMAP_COUNT = 4 };
+enum LoadMap { LMAP_PLAIN = 0, LMAP_EVENEXT = 1, LMAP_EVENODD = 2, LMAP_CF2INV = 3, LMAP_RFINV = 4, LMAP_COUNT = 5 };
 // NRFFT_LMAP_PAIR = 1: load-mapped passes run each tile beside its mirror
 // on one XCD (ctx_of in k_c2c).  64 x 2^20 FP32 on MI355X: cosft1 0.657 ->
 // 0.620 ms (read requests past L2 3.6x -> 1.22x the bytes of y), cosft2
@@ -168,6 +170,42 @@ __device__ __forceinline__ cpx<T> map_cf2inv(const T* ys, uint32_t m, uint32_t N
     const bool lo = m < nq;
     const uint32_t p = lo ? m : N - m;
     return cf2inv_core(ys, p, N, twiddle_big<1>(tw, p, logn + 2), lo);
+}
+
+// z_m of the realft inverse (Real_FT.rs:119-213 read with NR's indices;
+// k_untangle<T, -1> computes the same per bin pair) for 0 < m < N, m != N/2:
+// the pair p = min(m, N - m) of the packed spectrum d untangled with
+// w = W_n^{-p} (n = 2N) and c2 = 1/2, z_p and z_{N-p}; z_0 = dc_inverse(d_0)
+// and z_{N/2} = d_{N/2}.
+template <typename T>
+__device__ __forceinline__ cpx<T> rfinv_core(const cpx<T>* d, uint32_t p, uint32_t N, cpx<T> w, bool lo) {
+    cpx<T> x, y;
+    untangle_pair(d[p], d[N - p], w, (T)0.5, x, y);
+    return cpx<T>{lo ? x.x : y.x, lo ? x.y : y.y};
+}
+
+// exp(pi i r / 16), r < 16: the row step of the realft inverse's W_n (rows
+// N/16 apart; index r 16/R0 for a stage-0 radix R0).
+template <typename T>
+__device__ __forceinline__ cpx<T> w32(int r) {
+    constexpr double c[16][2] = {
+        {1.0, 0.0},
+        {0.980785280403230449126182, 0.195090322016128267848285},
+        {0.923879532511286756128183, 0.382683432365089771728460},
+        {0.831469612302545237078788, 0.555570233019602224742831},
+        {0.707106781186547524400844, 0.707106781186547524400844},
+        {0.555570233019602224742831, 0.831469612302545237078788},
+        {0.382683432365089771728460, 0.923879532511286756128183},
+        {0.195090322016128267848285, 0.980785280403230449126182},
+        {0.0, 1.0},
+        {-0.195090322016128267848285, 0.980785280403230449126182},
+        {-0.382683432365089771728460, 0.923879532511286756128183},
+        {-0.555570233019602224742831, 0.831469612302545237078788},
+        {-0.707106781186547524400844, 0.707106781186547524400844},
+        {-0.831469612302545237078788, 0.555570233019602224742831},
+        {-0.923879532511286756128183, 0.382683432365089771728460},
+        {-0.980785280403230449126182, 0.195090322016128267848285}};
+    return cpx<T>{(T)c[r][0], (T)c[r][1]};
 }
 
 // exp(2 pi i r / 128), r < 16: the step between the rows a first-pass
@@ -241,13 +279,15 @@ k_c2c(const cpx<T>* in, cpx<T>* out, PassDesc d, Twiddles<T> tw) {
         return c;
     };
     auto load = [&](cpx<T>* v, const Ctx& c, int ltid) {
-        if constexpr (LMAP == LMAP_CF2INV) {
+        if constexpr (LMAP == LMAP_CF2INV || LMAP == LMAP_RFINV) {
             // Butterfly (t, j) of stage 0 loads rows i = j + r L/R0, r < R0:
             // elements m = m0 + r N/R0 with m0 < N/R0, so r < R0/2 is the
-            // lower half (p = m) and W_{4n}^m = W_{4n}^{m0} exp(2 pi i r /
-            // (8 R0)): one table twiddle per butterfly.  The upper half uses
-            // W_{4n}^{N-m} = exp(i pi / 4) conj(W_{4n}^m); m = 0 and m = N/2
-            // (m0 = 0) take map_cf2inv.
+            // lower half (p = m), and one table twiddle per butterfly gives
+            // every element's: CF2INV W_{4n}^m = W_{4n}^{m0} exp(2 pi i r /
+            // (8 R0)), the upper half W_{4n}^{N-m} = exp(i pi / 4)
+            // conj(W_{4n}^m); RFINV W_n^m = W_n^{m0} exp(pi i r / R0), the
+            // lower half w = conj(W_n^m), the upper W_n^{-(N-m)} = -W_n^m.
+            // m = 0 and m = N/2 (m0 = 0) are special.
             constexpr int R0 = E::P::radix(0), NB0 = L / R0, BPT0 = EPT / R0;
             static_assert(16 % R0 == 0, "stage-0 radix divides 16");
             const uint32_t N = uint32_t(L) * uint32_t(esi);
@@ -260,16 +300,30 @@ k_c2c(const cpx<T>* in, cpx<T>* out, PassDesc d, Twiddles<T> tw) {
                 E::template bmap<MAP_T, NB0>(ltid + b * NT, t, j);
                 if (TI == 1 || t < c.nvalid) {
                     const uint32_t m0 = uint32_t(c.b0base) + uint32_t(t) + uint32_t(j) * uint32_t(esi);
-                    const cpx<T> w0 = twiddle_big<1>(tw, m0, logn + 2);
+                    const cpx<T> w0 = twiddle_big<1>(tw, m0, LMAP == LMAP_CF2INV ? logn + 2 : logn);
+                    const cpx<T>* d = reinterpret_cast<const cpx<T>*>(c.yb);  // RFINV: the packed spectrum
 #pragma unroll
                     for (int r = 0; r < R0; ++r) {
                         const uint32_t m = m0 + uint32_t(r) * step;
-                        if ((r == 0 || r == R0 / 2) && m0 == 0) {
-                            v[b * R0 + r] = map_cf2inv(c.yb, m, N, logn, tw);
+                        const bool lo = r < R0 / 2;
+                        if constexpr (LMAP == LMAP_CF2INV) {
+                            if ((r == 0 || r == R0 / 2) && m0 == 0) {
+                                v[b * R0 + r] = map_cf2inv(c.yb, m, N, logn, tw);
+                            } else {
+                                const cpx<T> wm = cmul(w0, w128<T>(r * (16 / R0)));
+                                const cpx<T> r4 = lo ? wm : cmul(cpx<T>{h, h}, conj(wm));
+                                v[b * R0 + r] = cf2inv_core(c.yb, lo ? m : N - m, N, r4, lo);
+                            }
                         } else {
-                            const cpx<T> wm = cmul(w0, w128<T>(r * (16 / R0)));
-                            const cpx<T> r4 = r < R0 / 2 ? wm : cmul(cpx<T>{h, h}, conj(wm));
-                            v[b * R0 + r] = cf2inv_core(c.yb, r < R0 / 2 ? m : N - m, N, r4, r < R0 / 2);
+                            if (r == 0 && m0 == 0) {
+                                v[b * R0 + r] = dc_inverse(d[0]);
+                            } else if (r == R0 / 2 && m0 == 0) {
+                                v[b * R0 + r] = d[N / 2];
+                            } else {
+                                const cpx<T> wm = cmul(w0, w32<T>(r * (16 / R0)));
+                                const cpx<T> w = lo ? conj(wm) : cpx<T>{-wm.x, -wm.y};
+                                v[b * R0 + r] = rfinv_core(d, lo ? m : N - m, N, w, lo);
+                            }
                         }
                     }
                 } else {

@@ -290,9 +290,29 @@ int build_plan_real(Plan& p) {
                                           : plan_c2c_1d(fw, logl, batch, B_IN, B_OUT, &tmp);
             if (rc) return rc;
             if (!NRFFT_REALFT_CHUNKED) fw.launches.push_back(real_launch(RS_UNTANGLE, B_OUT, B_OUT, {batch, L, 1}));
-            iv.launches.push_back(real_launch(RS_UNTANGLE, B_IN, B_OUT, {batch, L, -1}));
-            rc = plan_c2c_1d(iv, logl, batch, B_OUT, B_OUT, &tmp);
-            if (rc) return rc;
+            if (map_fits(iv, logl, LMAP_RFINV)) {
+                // The inverse untangle as the C2C's first-pass load map, chunk
+                // by chunk (signals in place stay safe: a chunk's first pass
+                // has read all of its signals before its second pass writes).
+                FourStep fs;
+                if ((rc = fourstep_setup(iv, logl, &fs))) return rc;
+                const int64_t chunk = (iv.tune.chunk > 0 && iv.tune.chunk < batch) ? iv.tune.chunk : batch;
+                for (int64_t c0 = 0; c0 < batch; c0 += chunk) {
+                    const int64_t cb = std::min<int64_t>(chunk, batch - c0);
+                    Launch la, lb;
+                    if ((rc = fourstep_launches(iv, fs, cb, B_IN, B_OUT, &la, &lb, LMAP_RFINV, 2 * L))) return rc;
+                    la.off_src_real = c0 * 2 * L;
+                    lb.off_dst = c0 * L;
+                    la.args[6] = lb.args[6] = -1;
+                    iv.launches.push_back(la);
+                    iv.launches.push_back(lb);
+                }
+                tmp = std::max<size_t>(tmp, size_t(L) * size_t(chunk));
+            } else {
+                iv.launches.push_back(real_launch(RS_UNTANGLE, B_IN, B_OUT, {batch, L, -1}));
+                rc = plan_c2c_1d(iv, logl, batch, B_OUT, B_OUT, &tmp);
+                if (rc) return rc;
+            }
             for (auto& l : fw.launches) {
                 l.args[7] = 1;
                 p.launches.push_back(l);

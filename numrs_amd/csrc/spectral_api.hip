@@ -109,7 +109,7 @@ const KernelRef& map_lookup(int dtype, int lmap, int logl, int tilog) {
 // True when the C2C of length 2^logn can read its input through load map
 // lmap: a four-step length whose first pass has a load-mapped kernel of the
 // plan's tile width.
-static bool spec_map_fits(const Plan& p, int logn, int lmap) {
+bool map_fits(const Plan& p, int logn, int lmap) {
     if (!NRFFT_SPEC_LOAD_MAP || logn <= C2C_MAX_LOGL_ || logn > 2 * C2C_MAX_LOGL_) return false;
     FourStep fs;
     if (fourstep_setup(p, logn, &fs)) return false;
@@ -122,7 +122,7 @@ static bool spec_map_fits(const Plan& p, int logn, int lmap) {
 // launch post(c0, cb) follows each chunk's second pass, which then stores
 // plainly into a chunk-sized TMP2, so the finish reads the chunk from the
 // caches (MALL); otherwise TMP2 holds the whole batch and post(0, batch)
-// runs once after the last chunk.  Call only when spec_map_fits.
+// runs once after the last chunk.  Call only when map_fits.
 #ifndef NRFFT_SPEC_MAP_CHUNK
 #define NRFFT_SPEC_MAP_CHUNK 0  // > 0: signals per chunk of the per-chunk form
 #endif
@@ -213,7 +213,7 @@ int build_plan_spectral(Plan& p) {
                 return NRFFT_OK;
             }
             use_c2c_tune(p, logn);
-            if (spec_map_fits(p, logn, LMAP_EVENEXT)) {
+            if (map_fits(p, logn, LMAP_EVENEXT)) {
                 const int64_t row = (n + 1) * (int64_t(csize(p.dtype)) / 2);  // bytes of one signal
                 return plan_spec_mapped(p, logn, batch, LMAP_EVENEXT, n + 1, 1, 0, NRFFT_CF1_MAP_CHUNKED != 0,
                                         [&](int64_t c0, int64_t cb) {
@@ -248,7 +248,7 @@ int build_plan_spectral(Plan& p) {
             use_c2c_tune(p, logl);
             const int64_t sig = n * (int64_t(csize(p.dtype)) / 2);  // bytes of one signal
             // forward: permute-pack -> C2C(+1) -> untangle and rotate
-            if (spec_map_fits(p, logl, LMAP_EVENODD)) {
+            if (map_fits(p, logl, LMAP_EVENODD)) {
                 rc = plan_spec_mapped(p, logl, batch, LMAP_EVENODD, n, 1, 1, NRFFT_CF2_MAP_CHUNKED != 0,
                                       [&](int64_t c0, int64_t cb) {
                                           Launch l = spec_launch(SP_CF2_FPOST, B_TMP2, B_OUT, {cb, L});
@@ -276,7 +276,7 @@ int build_plan_spectral(Plan& p) {
             // permutation straight into y instead of the unpermute step:
             // 0.448 against 0.447, its 4-byte stores filling half of each
             // line, so it was not kept).
-            if (spec_map_fits(p, logl, LMAP_CF2INV)) {
+            if (map_fits(p, logl, LMAP_CF2INV)) {
                 // The rotation and untangle as the C2C's first-pass load map.
                 return plan_spec_mapped(p, logl, batch, LMAP_CF2INV, n, -1, -1, false, [&](int64_t, int64_t cb) {
                     Launch d = spec_launch(SP_CF2_IUNPACK, B_TMP2, B_OUT, {cb, L});

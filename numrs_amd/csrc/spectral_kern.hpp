@@ -55,7 +55,7 @@ const SpecRef& spec_lookup_f32(int logl);
 const SpecRef& spec_lookup_f64(int logl);
 
 // First four-step passes that read a DCT's sequence straight from its reals
-// (k_c2c with a load map; isign +1, or -1 for LMAP_CF2INV; column mapping
+// (k_c2c with a load map; isign +1, or -1 for the inverse maps; column mapping
 // on both sides), for pass lengths 2^MAP_MIN_LOGL .. 2^MAP_MAX_LOGL and tile
 // widths 8 and 16 (the planner's defaults for both precisions).
 constexpr int MAP_MIN_LOGL = 7, MAP_MAX_LOGL = 11, MAP_MIN_TILOG = 3, MAP_MAX_TILOG = 4;
@@ -65,7 +65,7 @@ KernelRef map_ref_one() {
     constexpr int L = 1 << LOGL;
     constexpr int TI = 1 << TILOG;
     constexpr int NT = TI * L / 16;
-    constexpr int S = LMAP == LMAP_CF2INV ? -1 : 1;
+    constexpr int S = (LMAP == LMAP_CF2INV || LMAP == LMAP_RFINV) ? -1 : 1;
     constexpr int LDS = kc2c_lds_bytes<T, S, L, TI, NT, MAP_T, MAP_T>();
     if constexpr (NT <= 1024 && LDS <= lds_limit_bytes()) {
         return KernelRef{reinterpret_cast<const void*>(&k_c2c<T, S, L, TI, NT, MAP_T, MAP_T, LMAP>), NT, TI, LDS};
@@ -91,6 +91,7 @@ struct MapTable {
         map_fill<T, LMAP_EVENEXT>(t[LMAP_EVENEXT], std::make_integer_sequence<int, NL>{});
         map_fill<T, LMAP_EVENODD>(t[LMAP_EVENODD], std::make_integer_sequence<int, NL>{});
         map_fill<T, LMAP_CF2INV>(t[LMAP_CF2INV], std::make_integer_sequence<int, NL>{});
+        map_fill<T, LMAP_RFINV>(t[LMAP_RFINV], std::make_integer_sequence<int, NL>{});
     }
 };
 

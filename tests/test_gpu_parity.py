@@ -271,10 +271,10 @@ def test_c2c_max_length(dtype):
 
 def test_realft_plan_multipass_chunked_f32():
     # 36 FP32 signals of 2^21 reals: the 2^20-point C2C runs in chunks of 16
-    # transforms (16, 16, 4), then the untangle pass.  Signals of the first,
-    # a middle and the tail chunk against numpy's FP64 rfft (NR's sign:
-    # F = conj(rfft), packed [F_0, F_{n/2}, Re F_1, Im F_1, ...]); then the
-    # inverse returns (n/2) x.
+    # transforms (16, 16, 4), each followed by its untangle.  Signals of the
+    # first, a middle and the tail chunk against numpy's FP64 rfft (NR's
+    # sign: F = conj(rfft), packed [F_0, F_{n/2}, Re F_1, Im F_1, ...]);
+    # then the inverse (untangle in its first pass) returns (n/2) x.
     import torch
 
     n, b = 1 << 21, 36
@@ -296,9 +296,10 @@ def test_realft_plan_multipass_chunked_f32():
 
 
 def test_realft_plan_multipass_chunked_f64():
-    # 40 FP64 signals of 2^18 reals: the 2^17-point transform's second pass
-    # with the untangle fused in, over several chunks and a partial one;
-    # every signal against the oracle, then the inverse returns (n/2) x.
+    # 40 FP64 signals of 2^18 reals: the 2^17-point four-step over several
+    # chunks and a partial one, each chunk's untangle after its second pass;
+    # every signal against the oracle, then the inverse (its untangle read
+    # by the first pass of each chunk, in place) returns (n/2) x.
     import torch
 
     n, b = 1 << 18, 40
