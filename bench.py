@@ -561,8 +561,13 @@ def _extra_convlv() -> dict:
     del plan, d, r, a
     torch.cuda.empty_cache()
 
-    # batched realft beyond one group: 64 x 2^21 FP32 reals, forward, in place
-    # (C2C of 2^20 points in 16-transform chunks + the untangle pass).
+    # batched realft beyond one group: 64 x 2^21 FP32 reals in place, forward
+    # (C2C of 2^20 points in 16-transform chunks, each chunk's untangle after
+    # its second pass) and inverse (the untangle in each chunk's first pass).
+    # The inverse scales by n/2, so repeating it on one buffer would overflow:
+    # it is timed over 20 copies of a forward spectrum, one per call, after
+    # a settle phase of inverses on a scratch copy (the first launch of a
+    # plan's inverse kernels costs ~9 ms once).
     b, n = 64, 1 << 21
     x = torch.empty(b * n, dtype=torch.float32, device="cuda")
     fill_normal(x, 2, b * n)
@@ -574,7 +579,17 @@ def _extra_convlv() -> dict:
         "ms": round(ms, 4), "GBps": round(byt / (ms * 1e-3) / 1e9, 1),
         "frac_hbm": round(byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "GFLOPs": round(2.5 * n * 21 * b / (ms * 1e-3) / 1e9, 1), "launches": plan.num_launches}
-    del plan, x
+    fill_normal(x, 2, b * n)
+    plan.execute(x, 1)
+    copies = [x.clone() for _ in range(20)]
+    it = iter(copies)
+    _settle(lambda: plan.execute(x, -1), 0.3)
+    ms = time_events(lambda: plan.execute(next(it), -1), 20)
+    res["realft_f32_64x2p21_inv"] = {
+        "ms": round(ms, 4), "GBps": round(byt / (ms * 1e-3) / 1e9, 1),
+        "frac_hbm": round(byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "GFLOPs": round(2.5 * n * 21 * b / (ms * 1e-3) / 1e9, 1)}
+    del plan, x, copies
     torch.cuda.empty_cache()
     return res
 
