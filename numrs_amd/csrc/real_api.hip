@@ -20,6 +20,11 @@ namespace nrfft {
     } while (0)
 
 constexpr int REAL_MAX_LOGL = 13;  // fused single-group path: n/2 <= 8192
+// NRFFT_SPEC_RAW = 0: convlv / correl beyond one group untangle both forward
+// spectra in passes of their own before the spectrum operation.
+#ifndef NRFFT_SPEC_RAW
+#define NRFFT_SPEC_RAW 1
+#endif
 
 // ---------------------------------------------------------------------------
 // Simple kernels of the multi-pass path (n/2 > 8192) and of small sizes.
@@ -78,8 +83,11 @@ __global__ void k_place(const T* r, T* out, int64_t n, int64_t m, int64_t nsig) 
 // a <- inverse-untangle(op(a, b) / no2) on forward-untangled packed spectra:
 // bin pairs (p, L - p), p < L/2, PAIR_U per thread with every load first;
 // the thread of p = 0 also takes the packed DC / Nyquist slot and the
-// self-paired bin L/2.
-template <typename T>
+// self-paired bin L/2.  RAW: a and b are the L-point C2C outputs before the
+// forward untangle, which each pair applies first (k_untangle<T, 1>'s
+// arithmetic: w = W_n^p, c2 = -1/2; the packed slot dc_forward; bin L/2 is
+// its own untangle), so the two forward untangle passes are not run.
+template <typename T, bool RAW>
 __global__ __launch_bounds__(PAIR_NT) void k_spec(cpx<T>* a, const cpx<T>* b, int64_t L64, int64_t nsig, int op,
                                                   int logn, Twiddles<T> tw) {
     const int L = int(L64), np = L / 2;
@@ -105,6 +113,20 @@ __global__ __launch_bounds__(PAIR_NT) void k_spec(cpx<T>* a, const cpx<T>* b, in
             for (int u = 0; u < PAIR_U; ++u) {
                 const int p = pair_at(p0, u);
                 if (p >= np) continue;
+                if constexpr (RAW) {
+                    if (p == 0) {
+                        xa[u] = dc_forward(xa[u]);
+                        ya[u] = dc_forward(ya[u]);
+                    } else {
+                        cpx<T> f1, f2;
+                        untangle_pair(xa[u], xb[u], w[u], (T)-0.5, f1, f2);
+                        xa[u] = f1;
+                        xb[u] = f2;
+                        untangle_pair(ya[u], yb[u], w[u], (T)-0.5, f1, f2);
+                        ya[u] = f1;
+                        yb[u] = f2;
+                    }
+                }
                 if (p == 0) {
                     const cpx<T> y0{spec_apply_real(op, xa[u].x, ya[u].x, inv_no2),
                                     spec_apply_real(op, xa[u].y, ya[u].y, inv_no2)};
@@ -215,7 +237,7 @@ enum RealSpecial {
     RS_FUSED_CONV = 2,    // args: nsig, L, m
     RS_UNTANGLE = 3,      // args: nsig, L, dir, element offset (complex values) into src and dst
     RS_PLACE = 4,         // args: nsig, n, m
-    RS_SPEC = 5,          // args: nsig, L
+    RS_SPEC = 5,          // args: nsig, L, raw (1: both spectra before their forward untangle)
     RS_N2 = 6,            // args: nsig, dir(0 = both directions by isign)
     RS_CORREL_DIRECT = 7, // args: npairs, n
     RS_CONV_N2 = 8,       // args: nsig, m
@@ -347,14 +369,16 @@ int build_plan_real(Plan& p) {
         rc = plan_c2c_1d(p, logl, batch, B_AUX, B_TMP2, &tmp);
     }
     if (rc) return rc;
-    p.launches.push_back(real_launch(RS_UNTANGLE, B_TMP2, B_TMP2, {batch, L, 1}));
+    // The forward untangles of both spectra happen inside the spectrum
+    // operation's pairs (RS_SPEC with args[2] = 1), not as passes of their own.
+    if (!NRFFT_SPEC_RAW) p.launches.push_back(real_launch(RS_UNTANGLE, B_TMP2, B_TMP2, {batch, L, 1}));
     if ((rc = plan_c2c_1d(p, logl, batch, B_IN, B_OUT, &tmp))) return rc;
-    p.launches.push_back(real_launch(RS_UNTANGLE, B_OUT, B_OUT, {batch, L, 1}));
+    if (!NRFFT_SPEC_RAW) p.launches.push_back(real_launch(RS_UNTANGLE, B_OUT, B_OUT, {batch, L, 1}));
     // Both forward spectra use isign = +1 whatever isign the call carries
     // (for convlv isign selects convolution or deconvolution).
     for (auto& l : p.launches)
         if (l.kind == LK_C2C) l.args[6] = 1;
-    p.launches.push_back(real_launch(RS_SPEC, B_OUT, B_OUT, {batch, L}));
+    p.launches.push_back(real_launch(RS_SPEC, B_OUT, B_OUT, {batch, L, NRFFT_SPEC_RAW ? 1 : 0}));
     // The inverse C2C of the last step runs with isign = -1 whatever isign
     // the call carries (isign selects conv/deconv); args[6] marks it.
     const size_t before = p.launches.size();
@@ -423,8 +447,12 @@ static int exec_real_t(const Plan& p, const Launch& l, void* const* bufs, int is
         case RS_SPEC: {
             int op = p.kind == PK_CORREL ? OP_CORREL : (isign > 0 ? OP_CONV : OP_DECONV);
             const int logn = ilog2((size_t)a[1]) + 1;
-            hipLaunchKernelGGL((k_spec<T>), grid_pairs(a[1] / 2, a[0]), dim3(PAIR_NT), 0, stream, (cpx<T>*)dst,
-                               (const cpx<T>*)bufs[B_TMP2], a[1], a[0], op, logn, tw);
+            if (a[2])
+                hipLaunchKernelGGL((k_spec<T, true>), grid_pairs(a[1] / 2, a[0]), dim3(PAIR_NT), 0, stream,
+                                   (cpx<T>*)dst, (const cpx<T>*)bufs[B_TMP2], a[1], a[0], op, logn, tw);
+            else
+                hipLaunchKernelGGL((k_spec<T, false>), grid_pairs(a[1] / 2, a[0]), dim3(PAIR_NT), 0, stream,
+                                   (cpx<T>*)dst, (const cpx<T>*)bufs[B_TMP2], a[1], a[0], op, logn, tw);
             HIPCHK(hipGetLastError());
             return NRFFT_OK;
         }
