@@ -25,6 +25,7 @@ struct PassDesc {
     int32_t tw_b0_shift;
     int32_t xcd_remap;      // 1: give each XCD a contiguous run of tiles
     int32_t nt_out;         // 1: non-temporal output stores (the plan does not re-read them)
+    int64_t map_arg;        // load map argument (LMAP_PLACE: the response length m)
 };
 
 // Persistent pass kernel: group g walks tiles g, g + G, g + 2G, ... (G =
@@ -112,9 +113,19 @@ __device__ __forceinline__ int fresh_tid(int tid) {
 //                realft-inverse untangle of the 2N reals Y, isign -1.
 //   LMAP_RFINV   z_m of the realft inverse (rfinv_core): the inverse
 //                untangle of the packed spectrum d (N complex), isign -1.
+//   LMAP_PLACE   x_e = placed_response(r, e, 2N, m): convlv's response of m
+//                = d.map_arg reals padded to 2N (Convolve.rs:41-63).
 // The first-pass tile holds every element i of its TI transforms, so with
 // si0 == 1 the element is m = b0 + t + i * esi.
-enum LoadMap { LMAP_PLAIN = 0, LMAP_EVENEXT = 1, LMAP_EVENODD = 2, LMAP_CF2INV = 3, LMAP_RFINV = 4, LMAP_COUNT = 5 };
+enum LoadMap {
+    LMAP_PLAIN = 0,
+    LMAP_EVENEXT = 1,
+    LMAP_EVENODD = 2,
+    LMAP_CF2INV = 3,
+    LMAP_RFINV = 4,
+    LMAP_PLACE = 5,
+    LMAP_COUNT = 6
+};
 // NRFFT_LMAP_PAIR = 1: load-mapped passes run each tile beside its mirror
 // on one XCD (ctx_of in k_c2c).  64 x 2^20 FP32 on MI355X: cosft1 0.657 ->
 // 0.620 ms (read requests past L2 3.6x -> 1.22x the bytes of y), cosft2
@@ -341,9 +352,14 @@ k_c2c(const cpx<T>* in, cpx<T>* out, PassDesc d, Twiddles<T> tw) {
                     // 32-bit indices: N <= 2^26 points, 4N reals at most.
                     const uint32_t N = uint32_t(L) * uint32_t(esi);
                     const uint32_t m = uint32_t(c.b0base) + uint32_t(t) + uint32_t(i) * uint32_t(esi);
-                    uint32_t p0, p1;
-                    map_pair<LMAP>(m, N, p0, p1);
-                    return cpx<T>{c.yb[p0], c.yb[p1]};
+                    if constexpr (LMAP == LMAP_PLACE) {
+                        const int64_t e = 2 * int64_t(m), n2 = 2 * int64_t(N);
+                        return cpx<T>{placed_response(c.yb, e, n2, d.map_arg), placed_response(c.yb, e + 1, n2, d.map_arg)};
+                    } else {
+                        uint32_t p0, p1;
+                        map_pair<LMAP>(m, N, p0, p1);
+                        return cpx<T>{c.yb[p0], c.yb[p1]};
+                    }
                 }
             }
             return cpx<T>{(T)0, (T)0};

@@ -362,7 +362,24 @@ int build_plan_real(Plan& p) {
     size_t tmp = 0;
     use_c2c_tune(p, logl);
     int rc;
-    if (p.kind == PK_CONVLV) {
+    if (p.kind == PK_CONVLV && map_fits(p, logl, LMAP_PLACE)) {
+        // The response's C2C reads the m taps through the placement map:
+        // no padded copy of the response is written.
+        FourStep fs;
+        if ((rc = fourstep_setup(p, logl, &fs))) return rc;
+        const int64_t chunk = (p.tune.chunk > 0 && p.tune.chunk < batch) ? p.tune.chunk : batch;
+        for (int64_t c0 = 0; c0 < batch; c0 += chunk) {
+            const int64_t cb = std::min<int64_t>(chunk, batch - c0);
+            Launch la, lb;
+            if ((rc = fourstep_launches(p, fs, cb, B_AUX, B_TMP2, &la, &lb, LMAP_PLACE, m))) return rc;
+            la.desc.map_arg = m;
+            la.off_src_real = c0 * m;
+            lb.off_dst = c0 * L;
+            p.launches.push_back(la);
+            p.launches.push_back(lb);
+        }
+        tmp = std::max<size_t>(tmp, size_t(L) * size_t(chunk));
+    } else if (p.kind == PK_CONVLV) {
         p.launches.push_back(real_launch(RS_PLACE, B_AUX, B_TMP2, {batch, (int64_t)n, m}));
         rc = plan_c2c_1d(p, logl, batch, B_TMP2, B_TMP2, &tmp);
     } else {

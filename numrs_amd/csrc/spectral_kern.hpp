@@ -54,7 +54,8 @@ struct SpecTable {
 const SpecRef& spec_lookup_f32(int logl);
 const SpecRef& spec_lookup_f64(int logl);
 
-// First four-step passes that read a DCT's sequence straight from its reals
+// First four-step passes that read a DCT's or real transform's input
+// straight from the caller's reals
 // (k_c2c with a load map; isign +1, or -1 for the inverse maps; column mapping
 // on both sides), for pass lengths 2^MAP_MIN_LOGL .. 2^MAP_MAX_LOGL and tile
 // widths 8 and 16 (the planner's defaults for both precisions).
@@ -92,6 +93,7 @@ struct MapTable {
         map_fill<T, LMAP_EVENODD>(t[LMAP_EVENODD], std::make_integer_sequence<int, NL>{});
         map_fill<T, LMAP_CF2INV>(t[LMAP_CF2INV], std::make_integer_sequence<int, NL>{});
         map_fill<T, LMAP_RFINV>(t[LMAP_RFINV], std::make_integer_sequence<int, NL>{});
+        map_fill<T, LMAP_PLACE>(t[LMAP_PLACE], std::make_integer_sequence<int, NL>{});
     }
 };
 
