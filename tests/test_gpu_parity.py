@@ -247,6 +247,43 @@ def test_convlv_batch_f32_multipass(n, b, isign):
         assert rel_l2(y[i], ref) < 1e-5, (n, isign, i)
 
 
+@pytest.mark.parametrize("isign", [1, -1])
+def test_convlv_batch_f64_multipass(isign):
+    # FP64 convlv beyond one group with 8 signals of 2^16: the response's
+    # C2C reads its taps through the placement in its first pass (16-wide
+    # tiles), both forward untangles inside the spectrum step; every signal
+    # against the oracle (deconvolution to 1e-9, as for single signals).
+    n, b, m = 1 << 16, 8, 301
+    d = synth.normal(n + 7, n * b).reshape(b, n)
+    if isign == 1:
+        r = np.stack([synth.normal(70 + i, m) for i in range(b)])
+    else:
+        r = np.stack([0.3 / m * synth.normal(70 + i, m) for i in range(b)])
+        r[:, (m - 1) // 2] = 1.0
+    y = numrs_amd.convlv_batch_responses(d, r, isign)
+    for i in range(b):
+        ref = oracle.convlv(d[i], r[i], isign)
+        assert rel_l2(y[i], ref) < (TOL64 if isign == 1 else 1e-9), (isign, i)
+
+
+def test_correl_plan_f64_multipass():
+    # FP64 correl device plan beyond one group, 8 pairs of 2^16: every pair
+    # against the oracle.
+    import torch
+
+    n, b = 1 << 16, 8
+    d1 = synth.normal(90, n * b)
+    d2 = synth.normal(91, n * b)
+    plan = numrs_amd.CorrelPlan("float64", n, b)
+    t1, t2 = torch.from_numpy(d1).cuda(), torch.from_numpy(d2).cuda()
+    out = torch.empty_like(t1)
+    plan.execute(t1, t2, out)
+    got = out.cpu().numpy()
+    for i in range(b):
+        s = slice(i * n, (i + 1) * n)
+        assert rel_l2(got[s], oracle.correl(d1[s], d2[s])) < TOL64, i
+
+
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
 def test_c2c_max_length(dtype):
     # The longest 1-D transform the planner takes, N = 2^26 (two passes of
