@@ -369,6 +369,36 @@ int fourstep_launches(const Plan& p, const FourStep& fs, int64_t cb, int src, in
     return c2c_pass_launch(p, MODE_JT, fs.l2, fs.tb, b, B_TMP, dst, lb);
 }
 
+// The C2C of length N = 2^logn over `batch` signals whose first pass reads
+// src through load map lmap (sig_reals reals per signal, map argument
+// map_arg), chunk by chunk, into dst (N complex values per signal); the
+// launches run with isign `force` and only for calls of direction `only`
+// (0 = both).  In place (src == dst) is safe: a chunk's first pass has read
+// all of its signals before its second pass writes them.  The caller checks
+// map_fits first.
+int plan_c2c_mapped(Plan& p, int logn, int64_t batch, int src, int dst, int lmap, int64_t sig_reals, int64_t map_arg,
+                    int force, int only, size_t* tmp_elems) {
+    FourStep fs;
+    if (int rc = fourstep_setup(p, logn, &fs)) return rc;
+    const int64_t N = int64_t(1) << logn;
+    const int64_t chunk = (p.tune.chunk > 0 && p.tune.chunk < batch) ? p.tune.chunk : batch;
+    for (int64_t c0 = 0; c0 < batch; c0 += chunk) {
+        const int64_t cb = std::min<int64_t>(chunk, batch - c0);
+        Launch la, lb;
+        if (int rc = fourstep_launches(p, fs, cb, src, dst, &la, &lb, lmap, sig_reals)) return rc;
+        la.desc.map_arg = map_arg;
+        la.off_src_real = c0 * sig_reals;
+        lb.off_dst = c0 * N;
+        for (Launch* l : {&la, &lb}) {
+            l->args[6] = force;
+            l->args[7] = only;
+            p.launches.push_back(*l);
+        }
+    }
+    *tmp_elems = std::max<size_t>(*tmp_elems, size_t(N) * size_t(chunk));
+    return NRFFT_OK;
+}
+
 // Strided axes of >= 4096 points take the two-pass four-step along the axis
 // when at least this many columns sit side by side (inner), so pass B gets
 // 16-wide tiles; narrower axes run as one column pass.

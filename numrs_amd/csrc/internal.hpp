@@ -163,6 +163,9 @@ int fourstep_launches(const Plan& p, const FourStep& fs, int64_t cb, int src, in
 // The load-mapped first pass of (dtype, load map, log2 L, log2 tile width);
 // fn == nullptr if none.
 const KernelRef& map_lookup(int dtype, int lmap, int logl, int tilog);
+// The chunked C2C whose first pass reads src through load map lmap (api.hip).
+int plan_c2c_mapped(Plan& p, int logn, int64_t batch, int src, int dst, int lmap, int64_t sig_reals, int64_t map_arg,
+                    int force, int only, size_t* tmp_elems);
 // True when the C2C of length 2^logn can read its input through load map
 // lmap: a four-step length whose first pass has a load-mapped kernel of the
 // plan's tile width.

@@ -314,22 +314,8 @@ int build_plan_real(Plan& p) {
             if (!NRFFT_REALFT_CHUNKED) fw.launches.push_back(real_launch(RS_UNTANGLE, B_OUT, B_OUT, {batch, L, 1}));
             if (map_fits(iv, logl, LMAP_RFINV)) {
                 // The inverse untangle as the C2C's first-pass load map, chunk
-                // by chunk (signals in place stay safe: a chunk's first pass
-                // has read all of its signals before its second pass writes).
-                FourStep fs;
-                if ((rc = fourstep_setup(iv, logl, &fs))) return rc;
-                const int64_t chunk = (iv.tune.chunk > 0 && iv.tune.chunk < batch) ? iv.tune.chunk : batch;
-                for (int64_t c0 = 0; c0 < batch; c0 += chunk) {
-                    const int64_t cb = std::min<int64_t>(chunk, batch - c0);
-                    Launch la, lb;
-                    if ((rc = fourstep_launches(iv, fs, cb, B_IN, B_OUT, &la, &lb, LMAP_RFINV, 2 * L))) return rc;
-                    la.off_src_real = c0 * 2 * L;
-                    lb.off_dst = c0 * L;
-                    la.args[6] = lb.args[6] = -1;
-                    iv.launches.push_back(la);
-                    iv.launches.push_back(lb);
-                }
-                tmp = std::max<size_t>(tmp, size_t(L) * size_t(chunk));
+                // by chunk, in place.
+                if ((rc = plan_c2c_mapped(iv, logl, batch, B_IN, B_OUT, LMAP_RFINV, 2 * L, 0, -1, 0, &tmp))) return rc;
             } else {
                 iv.launches.push_back(real_launch(RS_UNTANGLE, B_IN, B_OUT, {batch, L, -1}));
                 rc = plan_c2c_1d(iv, logl, batch, B_OUT, B_OUT, &tmp);
@@ -365,20 +351,7 @@ int build_plan_real(Plan& p) {
     if (p.kind == PK_CONVLV && map_fits(p, logl, LMAP_PLACE)) {
         // The response's C2C reads the m taps through the placement map:
         // no padded copy of the response is written.
-        FourStep fs;
-        if ((rc = fourstep_setup(p, logl, &fs))) return rc;
-        const int64_t chunk = (p.tune.chunk > 0 && p.tune.chunk < batch) ? p.tune.chunk : batch;
-        for (int64_t c0 = 0; c0 < batch; c0 += chunk) {
-            const int64_t cb = std::min<int64_t>(chunk, batch - c0);
-            Launch la, lb;
-            if ((rc = fourstep_launches(p, fs, cb, B_AUX, B_TMP2, &la, &lb, LMAP_PLACE, m))) return rc;
-            la.desc.map_arg = m;
-            la.off_src_real = c0 * m;
-            lb.off_dst = c0 * L;
-            p.launches.push_back(la);
-            p.launches.push_back(lb);
-        }
-        tmp = std::max<size_t>(tmp, size_t(L) * size_t(chunk));
+        rc = plan_c2c_mapped(p, logl, batch, B_AUX, B_TMP2, LMAP_PLACE, m, m, 1, 0, &tmp);
     } else if (p.kind == PK_CONVLV) {
         p.launches.push_back(real_launch(RS_PLACE, B_AUX, B_TMP2, {batch, (int64_t)n, m}));
         rc = plan_c2c_1d(p, logl, batch, B_TMP2, B_TMP2, &tmp);

@@ -118,61 +118,24 @@ bool map_fits(const Plan& p, int logn, int lmap) {
 
 // A DCT's C2C of length N = 2^logn whose first pass reads the caller's reals
 // (sig_reals per signal) through load map lmap, chunk by chunk, with isign
-// `force`, for calls of direction `only` (0 = both).  With per_chunk the finishing
-// launch post(c0, cb) follows each chunk's second pass, which then stores
-// plainly into a chunk-sized TMP2, so the finish reads the chunk from the
-// caches (MALL); otherwise TMP2 holds the whole batch and post(0, batch)
-// runs once after the last chunk.  Call only when map_fits.
-#ifndef NRFFT_SPEC_MAP_CHUNK
-#define NRFFT_SPEC_MAP_CHUNK 0  // > 0: signals per chunk of the per-chunk form
-#endif
+// `force`, for calls of direction `only` (0 = both), into a whole-batch TMP2;
+// then the finishing launch post(0, batch) once after the last chunk.
+// (Finishing each chunk right after its second pass, through a chunk-sized
+// TMP2, measured slower at 64 x 2^20 FP32: cosft1 0.783 ms against 0.652,
+// cosft2 forward 0.451 against 0.379; a chunk's intermediate and its second
+// pass's output together fill the MALL.)  Call only when map_fits.
 template <class POST>
 static int plan_spec_mapped(Plan& p, int logn, int64_t batch, int lmap, int64_t sig_reals, int force, int only,
-                            bool per_chunk, POST post) {
-    FourStep fs;
-    if (int rc = fourstep_setup(p, logn, &fs)) return rc;
-    const int64_t N = int64_t(1) << logn;
-    int64_t chunk = (p.tune.chunk > 0 && p.tune.chunk < batch) ? p.tune.chunk : batch;
-    if (per_chunk && NRFFT_SPEC_MAP_CHUNK > 0) chunk = std::min<int64_t>(batch, NRFFT_SPEC_MAP_CHUNK);
-    for (int64_t c0 = 0; c0 < batch; c0 += chunk) {
-        const int64_t cb = std::min<int64_t>(chunk, batch - c0);
-        Launch la, lb;
-        if (int rc = fourstep_launches(p, fs, cb, B_IN, B_TMP2, &la, &lb, lmap, sig_reals)) return rc;
-        la.off_src_real = c0 * sig_reals;
-        if (per_chunk) lb.desc.nt_out = 0;
-        else lb.off_dst = c0 * N;
-        for (Launch* l : {&la, &lb}) {
-            l->args[6] = force;
-            l->args[7] = only;
-            p.launches.push_back(*l);
-        }
-        if (per_chunk) {
-            Launch b = post(c0, cb);
-            b.args[7] = only;
-            p.launches.push_back(b);
-        }
-    }
-    if (!per_chunk) {
-        Launch b = post(int64_t(0), batch);
-        b.args[7] = only;
-        p.launches.push_back(b);
-    }
-    p.tmp2_offset = std::max<size_t>(p.tmp2_offset, size_t(N) * size_t(chunk));  // the C2C scratch (B_TMP) comes first
-    p.tmp_elems = std::max<size_t>(p.tmp_elems, p.tmp2_offset + size_t(N) * size_t(per_chunk ? chunk : batch));
+                            POST post) {
+    size_t tmp = 0;
+    if (int rc = plan_c2c_mapped(p, logn, batch, B_IN, B_TMP2, lmap, sig_reals, 0, force, only, &tmp)) return rc;
+    Launch b = post(int64_t(0), batch);
+    b.args[7] = only;
+    p.launches.push_back(b);
+    p.tmp2_offset = std::max<size_t>(p.tmp2_offset, tmp);  // the C2C scratch (B_TMP) comes first
+    p.tmp_elems = std::max<size_t>(p.tmp_elems, p.tmp2_offset + (size_t(batch) << logn));
     return NRFFT_OK;
 }
-
-// Where the finishing step of a load-mapped DCT runs (1 = after each chunk).
-// At 64 x 2^20 FP32 the whole-batch finish measured faster for both:
-// cosft1 0.652 ms against 0.783 (0.672 / 0.740 with chunks of 8 / 4
-// signals), cosft2 forward 0.379 against 0.451 (0.415 / 0.546); a chunk's
-// intermediate and its second pass's output together fill the MALL.
-#ifndef NRFFT_CF1_MAP_CHUNKED
-#define NRFFT_CF1_MAP_CHUNKED 0
-#endif
-#ifndef NRFFT_CF2_MAP_CHUNKED
-#define NRFFT_CF2_MAP_CHUNKED 0
-#endif
 
 int build_plan_spectral(Plan& p) {
     const int64_t batch = (int64_t)p.batch;
@@ -215,12 +178,11 @@ int build_plan_spectral(Plan& p) {
             use_c2c_tune(p, logn);
             if (map_fits(p, logn, LMAP_EVENEXT)) {
                 const int64_t row = (n + 1) * (int64_t(csize(p.dtype)) / 2);  // bytes of one signal
-                return plan_spec_mapped(p, logn, batch, LMAP_EVENEXT, n + 1, 1, 0, NRFFT_CF1_MAP_CHUNKED != 0,
-                                        [&](int64_t c0, int64_t cb) {
-                                            Launch l = spec_launch(SP_CF1_POST, B_TMP2, B_OUT, {cb, n});
-                                            l.off_dst = c0 * row;
-                                            return l;
-                                        });
+                return plan_spec_mapped(p, logn, batch, LMAP_EVENEXT, n + 1, 1, 0, [&](int64_t c0, int64_t cb) {
+                    Launch l = spec_launch(SP_CF1_POST, B_TMP2, B_OUT, {cb, n});
+                    l.off_dst = c0 * row;
+                    return l;
+                });
             }
             // Whole batch per step: the chunked form (pack, passes and finish
             // per chunk through a chunk-sized scratch) measured 0.833 ms
@@ -249,8 +211,7 @@ int build_plan_spectral(Plan& p) {
             const int64_t sig = n * (int64_t(csize(p.dtype)) / 2);  // bytes of one signal
             // forward: permute-pack -> C2C(+1) -> untangle and rotate
             if (map_fits(p, logl, LMAP_EVENODD)) {
-                rc = plan_spec_mapped(p, logl, batch, LMAP_EVENODD, n, 1, 1, NRFFT_CF2_MAP_CHUNKED != 0,
-                                      [&](int64_t c0, int64_t cb) {
+                rc = plan_spec_mapped(p, logl, batch, LMAP_EVENODD, n, 1, 1, [&](int64_t c0, int64_t cb) {
                                           Launch l = spec_launch(SP_CF2_FPOST, B_TMP2, B_OUT, {cb, L});
                                           l.off_dst = c0 * sig;
                                           return l;
@@ -278,7 +239,7 @@ int build_plan_spectral(Plan& p) {
             // line, so it was not kept).
             if (map_fits(p, logl, LMAP_CF2INV)) {
                 // The rotation and untangle as the C2C's first-pass load map.
-                return plan_spec_mapped(p, logl, batch, LMAP_CF2INV, n, -1, -1, false, [&](int64_t, int64_t cb) {
+                return plan_spec_mapped(p, logl, batch, LMAP_CF2INV, n, -1, -1, [&](int64_t, int64_t cb) {
                     Launch d = spec_launch(SP_CF2_IUNPACK, B_TMP2, B_OUT, {cb, L});
                     return d;
                 });
