@@ -231,6 +231,96 @@ int col_tilog(int dtype) { return dtype == NRFFT_F32 ? 4 : 3; }
 #define NRFFT_C2C_OVERLAP 0
 #endif
 
+static int use_map_kernel(const Plan& p, const KernelRef& km, const PassDesc& d, Launch* l);
+
+// Three-pass C2C of N = N1 * Na * Mb (M = Na * Mb), chunk by chunk:
+//   pass 1: N1-point FFTs down the columns m of x[m + M n1] (stride M), times
+//           W_N^{k1 m} -> T[k1][m];
+//   pass 2: in place, the four-step's first pass of each row k1 as an
+//           M-point transform: Na-point FFTs down its columns mb (stride Mb),
+//           times W_M^{ka mb} -> T[k1][ka][mb];
+//   pass 3: Mb-point FFTs along the rows (k1, ka), tiled over k1, stored to
+//           X[k1 + N1 (ka + Na kb)].
+// X[k1 + N1 kM] is the M-point transform of row k1 (the outer four-step) and
+// kM = ka + Na kb is the inner four-step's output order.  Every pass is a
+// column pass or a row pass with 16-wide (FP32) / 8-wide (FP64) tiles.
+// lmap != LMAP_PLAIN: pass 1 reads src through that load map (sig_reals
+// reals per signal, argument map_arg); the launches run with isign `force`
+// and only for calls of direction `only` when those are non-zero.
+int plan_c2c_3pass(Plan& p, int logn, int64_t batch, int src, int dst, size_t* tmp_elems, int lmap, int64_t sig_reals,
+                   int64_t map_arg, int force, int only) {
+    int l1, la, lb;
+    three_pass_split(logn, &l1, &la, &lb);
+    const int64_t N = int64_t(1) << logn, N1 = int64_t(1) << l1, Na = int64_t(1) << la, Mb = int64_t(1) << lb;
+    const int64_t M = Na * Mb;
+    const int want = col_tilog(p.dtype);
+    const int t1 = pick_tilog(p.dtype, MODE_TT, l1, M, want), t2 = pick_tilog(p.dtype, MODE_TT, la, Mb, want);
+    const int t3 = pick_tilog(p.dtype, MODE_JT, lb, N1, want);
+    if (t1 < 0 || t2 < 0 || t3 < 0) return fail(NRFFT_E_UNSUPPORTED, "no three-pass kernels");
+    const int64_t chunk = (p.tune.chunk > 0 && p.tune.chunk < batch) ? p.tune.chunk : batch;
+    for (int64_t c0 = 0; c0 < batch; c0 += chunk) {
+        const int64_t cb = std::min<int64_t>(chunk, batch - c0);
+        PassDesc a{};
+        a.nb0 = M;
+        a.si0 = 1;
+        a.so0 = 1;
+        a.esi = (int32_t)M;
+        a.eso = (int32_t)M;
+        a.nb1 = cb;
+        a.si1 = lmap != LMAP_PLAIN ? sig_reals : N;
+        a.so1 = N;
+        a.nb2 = 1;
+        a.tw_logn = logn;
+        Launch l1a;
+        if (int rc = c2c_pass_launch(p, MODE_TT, l1, t1, a, src, B_TMP, &l1a)) return rc;
+        if (lmap != LMAP_PLAIN) {
+            if (int rc = use_map_kernel(p, map_lookup(p.dtype, lmap, l1, t1), a, &l1a)) return rc;
+            l1a.desc.map_arg = map_arg;
+            l1a.off_src_real = c0 * sig_reals;
+        } else {
+            l1a.off_src = c0 * N;
+        }
+        PassDesc b{};
+        b.nb0 = Mb;
+        b.si0 = 1;
+        b.so0 = 1;
+        b.esi = (int32_t)Mb;
+        b.eso = (int32_t)Mb;
+        b.nb1 = N1;
+        b.si1 = M;
+        b.so1 = M;
+        b.nb2 = cb;
+        b.si2 = N;
+        b.so2 = N;
+        b.tw_logn = la + lb;
+        Launch l2;
+        if (int rc = c2c_pass_launch(p, MODE_TT, la, t2, b, B_TMP, B_TMP, &l2)) return rc;
+        PassDesc c{};
+        c.nb0 = N1;
+        c.si0 = (int32_t)M;
+        c.esi = 1;
+        c.so0 = 1;
+        c.eso = (int32_t)(N1 * Na);
+        c.nb1 = Na;
+        c.si1 = Mb;
+        c.so1 = N1;
+        c.nb2 = cb;
+        c.si2 = N;
+        c.so2 = N;
+        c.nt_out = 1;  // the plan's result, not re-read
+        Launch l3;
+        if (int rc = c2c_pass_launch(p, MODE_JT, lb, t3, c, B_TMP, dst, &l3)) return rc;
+        l3.off_dst = c0 * N;
+        for (Launch* l : {&l1a, &l2, &l3}) {
+            if (force) l->args[6] = force;
+            if (only) l->args[7] = only;
+            p.launches.push_back(*l);
+        }
+    }
+    *tmp_elems = std::max<size_t>(*tmp_elems, size_t(N) * size_t(chunk));
+    return NRFFT_OK;
+}
+
 // 1-D batched C2C of length N = 2^p over `batch` contiguous transforms, from
 // buffer src to buffer dst (B_TMP used as the intermediate of two-pass sizes).
 // Output element k of transform b lands at dst[b*N*ostride_b... ] -- contiguous.
@@ -259,6 +349,9 @@ int plan_c2c_1d(Plan& p, int logn, int64_t batch, int src, int dst, size_t* tmp_
         return plan_add_c2c_pass(p, MODE_JJ, logn, 0, d, src, dst);
     }
     if (logn > 2 * C2C_MAX_LOGL_) return fail(NRFFT_E_UNSUPPORTED, "transform longer than 2^26");
+    // (Split-precision plans keep the two-pass form and its MFMA pass lengths.)
+    if (logn >= C2C_3PASS_MIN_LOGN && !(p.kind == PK_C2C && p.extra != 0))
+        return plan_c2c_3pass(p, logn, batch, src, dst, tmp_elems);
     FourStep fs;
     if (int rc = fourstep_setup(p, logn, &fs)) return rc;
     // Batch chunking: pass A and pass B of one chunk run back to back so the
@@ -378,6 +471,8 @@ int fourstep_launches(const Plan& p, const FourStep& fs, int64_t cb, int src, in
 // map_fits first.
 int plan_c2c_mapped(Plan& p, int logn, int64_t batch, int src, int dst, int lmap, int64_t sig_reals, int64_t map_arg,
                     int force, int only, size_t* tmp_elems) {
+    if (logn >= FUSED_3PASS_MIN_LOGN)
+        return plan_c2c_3pass(p, logn, batch, src, dst, tmp_elems, lmap, sig_reals, map_arg, force, only);
     FourStep fs;
     if (int rc = fourstep_setup(p, logn, &fs)) return rc;
     const int64_t N = int64_t(1) << logn;

@@ -163,6 +163,25 @@ int fourstep_launches(const Plan& p, const FourStep& fs, int64_t cb, int src, in
 // The load-mapped first pass of (dtype, load map, log2 L, log2 tile width);
 // fn == nullptr if none.
 const KernelRef& map_lookup(int dtype, int lmap, int logl, int tilog);
+// Long 1-D C2C: from 2^C2C_3PASS_MIN_LOGN points plain C2C plans run three
+// passes of N1 x Na x Mb points instead of the two-pass four-step (whose
+// passes beyond 2^22 points get tiles of 8 or fewer transforms; 16 x 2^22
+// FP32: 0.577 ms against 0.595, 8 x 2^23: 0.580 against 1.376).  Paths that
+// fold other steps into the four-step's passes (load maps, realft's
+// per-chunk untangle) keep the two-pass form below 2^FUSED_3PASS_MIN_LOGN,
+// where it is within a few percent and their fused steps gain more.
+#ifndef NRFFT_C2C_3PASS_MIN_LOGN
+#define NRFFT_C2C_3PASS_MIN_LOGN 22
+#endif
+constexpr int C2C_3PASS_MIN_LOGN = NRFFT_C2C_3PASS_MIN_LOGN;
+constexpr int FUSED_3PASS_MIN_LOGN = C2C_3PASS_MIN_LOGN > 23 ? C2C_3PASS_MIN_LOGN : 23;
+inline void three_pass_split(int logn, int* l1, int* la, int* lb) {
+    *l1 = (logn + 2) / 3;
+    *la = (logn - *l1) / 2;
+    *lb = logn - *l1 - *la;
+}
+int plan_c2c_3pass(Plan& p, int logn, int64_t batch, int src, int dst, size_t* tmp_elems, int lmap = LMAP_PLAIN,
+                   int64_t sig_reals = 0, int64_t map_arg = 0, int force = 0, int only = 0);
 // The chunked C2C whose first pass reads src through load map lmap (api.hip).
 int plan_c2c_mapped(Plan& p, int logn, int64_t batch, int src, int dst, int lmap, int64_t sig_reals, int64_t map_arg,
                     int force, int only, size_t* tmp_elems);

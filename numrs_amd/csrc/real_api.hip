@@ -308,10 +308,13 @@ int build_plan_real(Plan& p) {
             Plan fw = p, iv = p;
             fw.launches.clear();
             iv.launches.clear();
-            int rc = NRFFT_REALFT_CHUNKED ? plan_realft_fwd_chunked(fw, logl, batch, &tmp)
-                                          : plan_c2c_1d(fw, logl, batch, B_IN, B_OUT, &tmp);
+            // From 2^FUSED_3PASS_MIN_LOGN the C2C is the three-pass plan and
+            // the untangle one whole-batch pass after it.
+            const bool chunked = NRFFT_REALFT_CHUNKED && logl < FUSED_3PASS_MIN_LOGN;
+            int rc = chunked ? plan_realft_fwd_chunked(fw, logl, batch, &tmp)
+                             : plan_c2c_1d(fw, logl, batch, B_IN, B_OUT, &tmp);
             if (rc) return rc;
-            if (!NRFFT_REALFT_CHUNKED) fw.launches.push_back(real_launch(RS_UNTANGLE, B_OUT, B_OUT, {batch, L, 1}));
+            if (!chunked) fw.launches.push_back(real_launch(RS_UNTANGLE, B_OUT, B_OUT, {batch, L, 1}));
             if (map_fits(iv, logl, LMAP_RFINV)) {
                 // The inverse untangle as the C2C's first-pass load map, chunk
                 // by chunk, in place.

@@ -111,6 +111,12 @@ const KernelRef& map_lookup(int dtype, int lmap, int logl, int tilog) {
 // plan's tile width.
 bool map_fits(const Plan& p, int logn, int lmap) {
     if (!NRFFT_SPEC_LOAD_MAP || logn <= C2C_MAX_LOGL_ || logn > 2 * C2C_MAX_LOGL_) return false;
+    if (logn >= FUSED_3PASS_MIN_LOGN) {  // the three-pass plan's first pass
+        int l1, la, lb;
+        three_pass_split(logn, &l1, &la, &lb);
+        const int t1 = pick_tilog(p.dtype, MODE_TT, l1, int64_t(1) << (la + lb), col_tilog(p.dtype));
+        return t1 >= 0 && map_lookup(p.dtype, lmap, l1, t1).fn != nullptr;
+    }
     FourStep fs;
     if (fourstep_setup(p, logn, &fs)) return false;
     return map_lookup(p.dtype, lmap, fs.l1, fs.ta).fn != nullptr;

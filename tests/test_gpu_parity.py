@@ -226,7 +226,7 @@ def test_c2c_f64_default_tuning(logn, batch):
         assert rel_l2(got[2 * i * n:2 * (i + 1) * n], z.view(np.float64)) < TOL64, (logn, i)
 
 
-@pytest.mark.parametrize("n,b,isign", [(1 << 16, 6, 1), (1 << 16, 6, -1), (1 << 21, 20, 1)])
+@pytest.mark.parametrize("n,b,isign", [(1 << 16, 6, 1), (1 << 16, 6, -1), (1 << 21, 20, 1), (1 << 24, 2, 1)])
 def test_convlv_batch_f32_multipass(n, b, isign):
     # FP32 convlv beyond one group: response placement, forward C2Cs (in
     # 16-transform chunks at n = 2^21) + untangles, the spectrum operation
@@ -284,6 +284,30 @@ def test_correl_plan_f64_multipass():
         assert rel_l2(got[s], oracle.correl(d1[s], d2[s])) < TOL64, i
 
 
+@pytest.mark.parametrize("dtype,logn,batch", [("float32", 23, 5), ("float64", 23, 3), ("float32", 24, 3)])
+def test_c2c_three_pass(dtype, logn, batch):
+    # Transforms of 2^23 points and more run as three passes (N1 x Na x Mb,
+    # every pass <= 2^9 points here) in chunks (2^23 FP32: chunks of 2, 2, 1):
+    # the first and last transform against numpy's FP64 FFT, both signs.
+    import torch
+
+    n = 1 << logn
+    x = synth.uniform(logn + 40, 2 * n * batch)
+    if dtype == "float32":
+        x = f32round(x)
+    t = torch.from_numpy(x.astype(np.float32 if dtype == "float32" else np.float64)).cuda()
+    o = torch.empty_like(t)
+    plan = numrs_amd.C2CPlan(dtype, [n], batch)
+    tol = TOL32 if dtype == "float32" else TOL64
+    for isign in (1, -1):
+        plan.execute(t, o, isign)
+        got = o.cpu().numpy().astype(np.float64)
+        for i in (0, batch - 1):
+            z = x[2 * i * n:2 * (i + 1) * n].view(np.complex128)
+            ref = n * np.fft.ifft(z) if isign == 1 else np.fft.fft(z)
+            assert rel_l2(got[2 * i * n:2 * (i + 1) * n], ref.view(np.float64)) < tol, (isign, i)
+
+
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
 def test_c2c_max_length(dtype):
     # The longest 1-D transform the planner takes, N = 2^26 (two passes of
@@ -329,6 +353,31 @@ def test_realft_plan_multipass_chunked_f32():
     plan.execute(t, -1)
     back = t.cpu().numpy()
     for i in (0, 17, 35):
+        assert rel_l2(back[i * n:(i + 1) * n], (n / 2) * x[i * n:(i + 1) * n].astype(np.float64)) < TOL32, i
+
+
+def test_realft_plan_three_pass_f32():
+    # 3 FP32 signals of 2^24 reals: the 2^23-point C2C runs as three passes
+    # (chunks of 2 and 1) with one untangle pass after it; the inverse reads
+    # the untangle through its first pass, in place.  First and last signal
+    # against numpy's FP64 rfft, then the inverse returns (n/2) x.
+    import torch
+
+    n, b = 1 << 24, 3
+    x = synth.normal(32, b * n).astype(np.float32)
+    t = torch.from_numpy(x).cuda()
+    plan = numrs_amd.RealFTPlan("float32", n, b)
+    plan.execute(t, 1)
+    got = t.cpu().numpy()
+    for i in (0, b - 1):
+        f = np.conj(np.fft.rfft(x[i * n:(i + 1) * n].astype(np.float64)))
+        ref = np.empty(n)
+        ref[0], ref[1] = f[0].real, f[n // 2].real
+        ref[2::2], ref[3::2] = f[1:n // 2].real, f[1:n // 2].imag
+        assert rel_l2(got[i * n:(i + 1) * n], ref) < TOL32, i
+    plan.execute(t, -1)
+    back = t.cpu().numpy()
+    for i in (0, b - 1):
         assert rel_l2(back[i * n:(i + 1) * n], (n / 2) * x[i * n:(i + 1) * n].astype(np.float64)) < TOL32, i
 
 

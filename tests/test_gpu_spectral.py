@@ -166,11 +166,13 @@ def test_cosft2_plan_chunked(dt, n, b, isign):
         assert rel_l2(y[seg], ref) < 5 * TOL[npdt], (i, isign)
 
 
-@pytest.mark.parametrize("dt,n,b", [("float64", 1 << 20, 3), ("float32", 1 << 20, 5), ("float32", 1 << 22, 2)])
+@pytest.mark.parametrize("dt,n,b", [("float64", 1 << 20, 3), ("float32", 1 << 20, 5), ("float32", 1 << 22, 2),
+                                    ("float32", 1 << 23, 2)])
 def test_cosft1_plan_long(dt, n, b):
     # Long DCT-Is whose first C2C pass reads the even extension of each
     # signal straight from its n + 1 reals (signals at odd real offsets):
-    # 512- and 2048-point first passes, 8-wide tiles.
+    # 512- and 2048-point first passes, 8-wide tiles; at 2^23 the first of
+    # three passes.
     import scipy.fft as sfft
     import torch
 
@@ -190,7 +192,8 @@ def test_cosft1_plan_long(dt, n, b):
             assert rel_l2(y[seg], exact) < 2e-13, i
 
 
-@pytest.mark.parametrize("dt,n,b", [("float64", 1 << 21, 3), ("float64", 1 << 19, 3), ("float32", 1 << 21, 5)])
+@pytest.mark.parametrize("dt,n,b", [("float64", 1 << 21, 3), ("float64", 1 << 19, 3), ("float32", 1 << 21, 5),
+                                    ("float32", 1 << 24, 2)])
 @pytest.mark.parametrize("isign", [1, -1])
 def test_cosft2_plan_long(dt, n, b, isign):
     # As above for cosft2: the forward first pass reads the even samples
