@@ -159,7 +159,8 @@ int build_plan_spectral(Plan& p) {
             // into the workspace (Z1 then Z2), then the realft untangle of
             // both into the full n-bin spectra -- no packing pass.  (Writing
             // Z1 / Z2 into the outputs with stride n and untangling in place
-            // measured 5% slower.)
+            // measured 5% slower; so did untangling chunk by chunk right after
+            // both arrays' passes, 64 x 2^20 FP32 0.81-0.86 ms against 0.78.)
             const int64_t N = n / 2;
             use_c2c_tune(p, logn - 1);
             const size_t from = p.launches.size();
@@ -331,7 +332,8 @@ static int exec_spec_t(const Plan& p, const Launch& l, void* const* bufs, int is
         case SP_TW_REAL: {
             const cpx<T>* z1 = (const cpx<T>*)src;
             hipLaunchKernelGGL((k_twofft_real<T>), grid_pairs(a[1] / 4, 2 * nsig), dim3(PAIR_NT), 0, s, z1,
-                               z1 + nsig * (a[1] / 2), (cpx<T>*)dst, (cpx<T>*)bufs[B_AUX2], a[1], nsig,
+                               z1 + nsig * (a[1] / 2), (cpx<T>*)dst,
+                               (cpx<T>*)(static_cast<char*>(bufs[B_AUX2]) + l.off_dst), a[1], nsig,
                                ilog2((size_t)a[1]), tw);
             break;
         }

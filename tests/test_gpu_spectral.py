@@ -359,3 +359,21 @@ def test_device_plans_multipass_chunked_f32():
             seg = slice(i * n, (i + 1) * n)
             exact = 0.5 * sfft.dct(y[seg].astype(np.float64), type=2 if isign == 1 else 3)
             assert rel_l2(got[seg], exact) < tol, ("cosft2", isign, i)
+
+
+def test_twofft_chunked_f64():
+    # FP64 n = 2^16, 130 signals: each array's 2^15-point C2C runs in chunks
+    # of 128 transforms (128, 2) before the whole-batch untangle.  Signals of
+    # both chunks against the oracle.
+    import torch
+
+    n, b = 1 << 16, 130
+    d1, d2 = _in(84, b * n, np.float64), _in(85, b * n, np.float64)
+    f1 = torch.empty(2 * b * n, dtype=torch.float64, device="cuda")
+    f2 = torch.empty_like(f1)
+    numrs_amd.TwofftPlan("float64", n, b).execute(torch.from_numpy(d1).cuda(), torch.from_numpy(d2).cuda(), f1, f2)
+    g1, g2 = f1.cpu().numpy(), f2.cpu().numpy()
+    for i in (0, 63, 64, 100, 128, 129):
+        r1, r2 = oracle.twofft(d1[i * n:(i + 1) * n], d2[i * n:(i + 1) * n])
+        seg = slice(2 * i * n, 2 * (i + 1) * n)
+        assert rel_l2(g1[seg], r1) < 1e-12 and rel_l2(g2[seg], r2) < 1e-12, i
