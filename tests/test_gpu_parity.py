@@ -284,11 +284,14 @@ def test_correl_plan_f64_multipass():
         assert rel_l2(got[s], oracle.correl(d1[s], d2[s])) < TOL64, i
 
 
-@pytest.mark.parametrize("dtype,logn,batch", [("float32", 23, 5), ("float64", 23, 3), ("float32", 24, 3)])
+@pytest.mark.parametrize("dtype,logn,batch", [("float32", 22, 6), ("float64", 22, 4), ("float32", 23, 5),
+                                              ("float64", 23, 3), ("float32", 24, 3), ("float64", 24, 2),
+                                              ("float32", 25, 2)])
 def test_c2c_three_pass(dtype, logn, batch):
-    # Transforms of 2^23 points and more run as three passes (N1 x Na x Mb,
-    # every pass <= 2^9 points here) in chunks (2^23 FP32: chunks of 2, 2, 1):
-    # the first and last transform against numpy's FP64 FFT, both signs.
+    # Transforms of 2^22 points and more run as three passes N1 x Na x Mb
+    # (2^8 x 2^7 x 2^7 at 2^22 ... 2^9 x 2^8 x 2^8 at 2^25) in chunks (2^23
+    # FP32: chunks of 2, 2, 1): the first and last transform against numpy's
+    # FP64 FFT, both signs.
     import torch
 
     n = 1 << logn

@@ -90,6 +90,29 @@ def test_cosft1_vs_oracle(dt, n):
     assert rel_l2(g, 0.5 * n * y.astype(np.float64)) < 10 * TOL[dt], n
 
 
+@pytest.mark.parametrize("dt,n,b", [(np.float32, 16384, 5), (np.float32, 4096, 3), (np.float64, 8192, 4)])
+@pytest.mark.parametrize("inplace", [True, False])
+def test_cosft1_fused_batched(dt, n, b, inplace):
+    # Batches of the one-group cosft1 whose signals of n + 1 values start at
+    # any offset: every signal against the oracle and scipy's DCT-I, in place
+    # and out of place.
+    import scipy.fft as sfft
+    import torch
+
+    y = _in(86, b * (n + 1), dt)
+    t = torch.from_numpy(y).cuda()
+    o = t if inplace else torch.empty_like(t)
+    numrs_amd.Cosft1Plan(np.dtype(dt).name, n, b).execute(t, o)
+    got = o.cpu().numpy()
+    for i in range(b):
+        seg = slice(i * (n + 1), (i + 1) * (n + 1))
+        y64 = y[seg].astype(np.float64)
+        ref, exact = oracle.cosft1(y64), 0.5 * sfft.dct(y64, type=1)
+        assert rel_l2(got[seg], ref) < _oracle_bound(dt, ref, exact), i
+        if dt == np.float64:
+            assert rel_l2(got[seg], exact) < 2e-13, i
+
+
 # ---- cosft2 -------------------------------------------------------------------
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 @pytest.mark.parametrize("n", [2, 4, 8, 64, 16384, 32768, 1 << 17])
