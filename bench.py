@@ -439,8 +439,37 @@ def extra_configs(args, only: str = "all") -> dict:
         res.update(_extra_uneven())
     if only in ("all", "spectral"):
         res.update(_extra_spectral())
+    if only in ("all", "long"):
+        res.update(_extra_long())
     if only in ("all", "fourfs"):
         res.update(_extra_fourfs())
+    return res
+
+
+def _extra_long() -> dict:
+    """Long 1-D transforms (2^23, 2^26 points: three passes of at most 2^9
+    points), 512 MiB batches, device resident, forward.  Bytes are
+    compulsory: the batch read once and written once."""
+    import torch
+
+    import numrs_amd
+
+    res = {}
+    for dt, logn, b in (("float32", 23, 8), ("float32", 26, 1), ("float64", 23, 4)):
+        n = 1 << logn
+        tdt = getattr(torch, dt)
+        x = torch.empty(2 * n * b, dtype=tdt, device="cuda")
+        numrs_amd.gen_uniform(x, 50 + logn)
+        y = torch.empty_like(x)
+        plan = numrs_amd.C2CPlan(dt, [n], b)
+        _settle(lambda: plan.execute(x, y, 1), 0.3)
+        ms = time_events(lambda: plan.execute(x, y, 1), 20)
+        byt = 2.0 * x.numel() * x.element_size()
+        res[f"c2c_{'f32' if dt == 'float32' else 'f64'}_{b}x2p{logn}"] = {
+            "ms": round(ms, 4), "GBps": round(byt / (ms * 1e-3) / 1e9, 1),
+            "frac_hbm": round(byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "GFLOPs": round(5.0 * n * logn * b / (ms * 1e-3) / 1e9, 1), "launches": plan.num_launches}
+        del plan, x, y
     return res
 
 
@@ -772,7 +801,8 @@ def main() -> int:
     ap.add_argument("--cpu-uneven-items", type=int, default=400, help="uneven items in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--only", choices=["headline", "four1", "fourn", "convlv", "uneven", "spectral", "fourfs"], default=None,
+    ap.add_argument("--only", choices=["headline", "four1", "fourn", "convlv", "uneven", "spectral", "long", "fourfs"],
+                    default=None,
                     help="run one workload only (profiling runs); the JSON line then holds just that workload")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 512 MiB copy kernel (counter calibration for the traffic figure)")
