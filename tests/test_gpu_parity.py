@@ -190,10 +190,12 @@ def test_realft_f32(logn):
     assert rel_l2(y, (n / 2) * x) < 1e-5
 
 
-@pytest.mark.parametrize("logn,batch", [(16, 300), (17, 150), (18, 70), (19, 40), (21, 10), (22, 6)])
+@pytest.mark.parametrize("logn,batch", [(14, 300), (15, 300), (16, 300), (17, 150), (18, 70), (19, 40), (21, 10),
+                                        (22, 6)])
 def test_c2c_f32_default_tuning(logn, batch):
-    # The FP32 four-step defaults: 512-point pass-B rows below 2^19, 2048-point
-    # rows above; 64 / 128 MiB chunks, each batch here ending in a partial
+    # The FP32 defaults: the per-length split table (2^14 .. 2^18), 2048-point
+    # pass-B rows from 2^19, three passes from 2^22; 32 / 64 / 128 MiB chunks
+    # (256 transforms at 2^14 and 2^15), each batch here ending in a partial
     # chunk.  First and last transform against numpy (isign = +1 is N ifft).
     import torch
 
@@ -209,10 +211,11 @@ def test_c2c_f32_default_tuning(logn, batch):
         assert rel_l2(got[2 * i * n:2 * (i + 1) * n], z.view(np.float64)) < TOL32, (logn, i)
 
 
-@pytest.mark.parametrize("logn,batch", [(17, 40), (19, 20), (21, 6)])
+@pytest.mark.parametrize("logn,batch", [(14, 600), (16, 150), (17, 40), (18, 40), (19, 20), (21, 6)])
 def test_c2c_f64_default_tuning(logn, batch):
-    # FP64 four-step defaults with a partial last chunk; first and last
-    # transform against numpy.
+    # FP64 four-step defaults (8-wide pass-A tiles from 512-point columns;
+    # 32 MiB chunks at 2^14, 64 MiB to 2^17, 128 MiB above) with a partial
+    # last chunk; first and last transform against numpy.
     import torch
 
     n = 1 << logn
