@@ -275,11 +275,11 @@ def cpu_baseline(args) -> dict:
       headline (64 x 2^20): the literal four1 of FFT_1.rs:5-107 over
         transforms on all usable cores (OpenMP, as FFTProcessor::fft_batch
         spreads them with rayon, FFT_1.rs:185-189), and on one core;
-      config 1: one four1 of 2^16 (serial, as the reference runs one call);
-      config 2: NR fourn 256^3 (serial: the reference has no parallel fourn);
-      config 3: convlv over signals (Convolve.rs:246-249) on all cores and
-        one core;
-      config 5: fft_batch over the uneven items (1-D literal four1, 2-D
+      four1 2^16: one transform (serial, as the reference runs one call);
+      fourn 256^3: NR fourn (serial: the reference has no parallel fourn);
+      convlv 4096 x 2^14, m = 257: over signals (Convolve.rs:246-249) on all
+        cores and one core;
+      the uneven batch: fft_batch over its items (1-D literal four1, 2-D
         fourn) on all cores and one core.
     The oracle is built with -march=native on this host when gcc is here
     (oracle/liborc_native.so), else the portable build is used."""
@@ -318,7 +318,7 @@ def cpu_baseline(args) -> dict:
                            "sample": f"{one} transforms of N=2^20, median of 3 runs: {med1:.2f} s"},
            "host": dict(host, build=build)}
 
-    # config 1: four1 2^16 FP64, uniform seed 0, one transform
+    # four1 2^16 FP64, uniform seed 0, one transform
     n1 = 1 << 16
     x1 = synth.uniform(0, 2 * n1)
 
@@ -328,10 +328,10 @@ def cpu_baseline(args) -> dict:
         return d
 
     t1, _ = _median_time(c1, 21)
-    res["config1_four1_f64_2p16"] = {"us": round(t1 * 1e6, 1), "GFLOPs": round(5.0 * n1 * 16 / t1 / 1e9, 3),
+    res["four1_f64_2p16"] = {"us": round(t1 * 1e6, 1), "GFLOPs": round(5.0 * n1 * 16 / t1 / 1e9, 3),
                                      "cores": 1, "sample": "one transform, median of 21 runs"}
 
-    # config 3: convlv 4096 x 2^14, m = 257 (FP64 on the FP32-rounded inputs)
+    # convlv 4096 x 2^14, m = 257 (FP64 on the FP32-rounded inputs)
     b3, n3, m3 = 4096, 1 << 14, 257
     d3 = synth.normal(2, b3 * n3).astype(np.float32).astype(np.float64)
     r3 = synth.gaussian_windowed_responses(2, b3, m3, 40.0, start=b3 * n3).astype(np.float32).astype(np.float64)
@@ -340,7 +340,7 @@ def cpu_baseline(args) -> dict:
     one3 = 256
     t31, _ = _median_time(lambda: oracle.convlv_batch(d3, n3, r3.reshape(-1), m3, one3, 1, a3, 1, L), 3)
     fl3 = 3 * 2.5 * n3 * 14
-    res["config3_convlv_4096x2p14_m257"] = {
+    res["convlv_4096x2p14_m257"] = {
         "ms": round(t3 * 1e3, 2), "signals_per_s": round(b3 / t3, 1), "GFLOPs": round(fl3 * b3 / t3 / 1e9, 3),
         "GBps": round((2.0 * n3 + m3) * 8 * b3 / t3 / 1e9, 2), "cores": used3,
         "sample": f"all 4096 signals, median of 3 runs on {used3} threads",
@@ -348,7 +348,7 @@ def cpu_baseline(args) -> dict:
                         "sample": f"{one3} signals, median of 3 runs"}}
     del d3, r3, a3
 
-    # config 5: the uneven batch's first items (the list is in random order)
+    # the uneven batch's first items (the list is in random order)
     items = synth.uneven_shapes(4, 10000)
     offs, acc = [], 0
     for it in items:
@@ -369,22 +369,22 @@ def cpu_baseline(args) -> dict:
     uneven(k5, threads)  # warm-up
     r5 = sorted(uneven(k5, threads) for _ in range(3))[1]
     r51 = sorted(uneven(k51, 1) for _ in range(3))[1]
-    res["config5_uneven_first_items"] = {
+    res["uneven_first_items"] = {
         "GFLOPs": round(uneven_flops(k5) / r5[0] / 1e9, 3), "items_per_s": round(k5 / r5[0], 1),
         "cores": r5[1], "sample": f"items 0..{k5 - 1} of the 10,000 (shapes drawn in random order), "
                                   f"median of 3 runs: {r5[0]:.2f} s",
         "single_core": {"GFLOPs": round(uneven_flops(k51) / r51[0] / 1e9, 3),
                         "sample": f"items 0..{k51 - 1}, median of 3 runs: {r51[0]:.2f} s"}}
 
-    # config 2: NR fourn 256^3 FP64 (serial), one run after none: ~10 s
+    # NR fourn 256^3 FP64 (serial), one run after none: ~10 s
     x2 = synth.normal(1, 2 * 256 ** 3).astype(np.float32).astype(np.float64)
     t0 = time.perf_counter()
     oracle.fourn_with(L, x2, [256, 256, 256], 1)
     t2 = time.perf_counter() - t0
-    res["config2_fourn_256cubed_f64"] = {"s": round(t2, 3), "GFLOPs": round(5.0 * 256 ** 3 * 24 / t2 / 1e9, 3),
-                                         "cores": 1, "sample": "one forward transform (FP32-rounded N(0,1) "
-                                                               "seed 1, the GPU bench's input)"}
-    out["configs"] = res
+    res["fourn_256cubed_f64"] = {"s": round(t2, 3), "GFLOPs": round(5.0 * 256 ** 3 * 24 / t2 / 1e9, 3),
+                                 "cores": 1, "sample": "one forward transform (FP32-rounded N(0,1) "
+                                                       "seed 1, the GPU bench's input)"}
+    out["workloads"] = res
     out["_fourn_oracle_output"] = x2
     return out
 
@@ -479,7 +479,7 @@ def _extra_four1() -> dict:
     import numrs_amd
 
     res = {}
-    # four1 N=2^16 FP64, single transform, device resident (config 1 shape).
+    # four1 N=2^16 FP64, single transform, device resident.
     n = 1 << 16
     x = torch.empty(2 * n, dtype=torch.float64, device="cuda")
     fill_uniform(x, 0, 0, 2 * n)
@@ -525,7 +525,7 @@ def _extra_fourn() -> dict:
     import numrs_amd
 
     res = {}
-    # fourn 256^3 FP32 and FP64 (config 2 shape): total, and each axis pass
+    # fourn 256^3 FP32 and FP64: total, and each axis pass
     # timed between HIP events inside the plan's own sequence (a pass timed
     # repeated on its own finds its 128-256 MiB array warm in the MALL).
     # The FP32 result's relative L2 is against the FP64 path on the same
@@ -572,7 +572,7 @@ def _extra_convlv() -> dict:
     from numrs_amd import synth
 
     res = {}
-    # batched convlv 4096 x 2^14 FP32, m = 257 per-signal responses (config 3 shape).
+    # batched convlv 4096 x 2^14 FP32, m = 257 per-signal responses.
 
     b, n, m = 4096, 1 << 14, 257
     d = torch.empty(b * n, dtype=torch.float32, device="cuda")
@@ -838,7 +838,7 @@ def main() -> int:
         log("[bench] cpu baseline")
         cb = cpu_baseline(args)
         ref = cb.pop("_fourn_oracle_output")
-        cb["configs"]["config2_fourn_256cubed_f64"]["gpu_fp64_rel_l2_vs_oracle"] = _fourn_gpu_vs_oracle(ref)
+        cb["workloads"]["fourn_256cubed_f64"]["gpu_fp64_rel_l2_vs_oracle"] = _fourn_gpu_vs_oracle(ref)
         del ref
         out["cpu_baseline"] = cb
     else:

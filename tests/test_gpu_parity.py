@@ -35,7 +35,7 @@ def test_four1_f64_all_lengths(logn):
         assert rel_l2(y, oracle.four1_literal(x, isign)) < TOL64, (nn, isign)
 
 
-def test_four1_config1_roundtrip():
+def test_four1_2p16_roundtrip():
     # N = 2^16 FP64, U[-1,1) seed 0: forward (+1) then inverse (-1) * 1/N.
     nn = 1 << 16
     x = synth.uniform(0, 2 * nn)
@@ -597,7 +597,7 @@ def test_autocorrel():
     assert rel_l2(numrs_amd.autocorrel_fast(x), oracle.correl(x, x)) < TOL64
 
 
-# ---- errors on the GPU box equal the CPU-side ones -----------------------------
+# ---- errors of GPU calls equal the CPU-side ones -------------------------------
 def test_errors_on_gpu():
     with pytest.raises(numrs_amd.ConvlvError):
         numrs_amd.convlv([1.0, 2.0], [1.0, 2.0, 3.0], 1)
