@@ -639,4 +639,90 @@ __global__ void k_rlft3_untangle(cpx<T>* data, cpx<T>* speq, int64_t nn1, int64_
     }
 }
 
+// rlft3 forward, last axis fused with the untangle.  The rows (i1, i2) of
+// each volume pair with their mirrors m = (-i1, -i2); the pairs are
+// enumerated per volume as i1 = 0 with i2 <= nn2/2, 0 < i1 < nn1/2 with
+// every i2, i1 = nn1/2 with i2 <= nn2/2 (nn1 nn2 / 2 + 2 pairs; four rows
+// are their own mirror).  A tile holds the H = nn3/2-point rows of TI/2
+// pairs, slot t the row and slot t + TI/2 its mirror, read from z (the
+// other axes already transformed, out of place); after the FFTs each row r
+// of the tile, with m its partner, writes
+//   k = 0        data[r][0] = untangle(Z_r[0], Z_m[0], 1).a,
+//                speq[r]    = untangle(Z_m[0], Z_r[0], 1).b,
+//   0 < k < H/2  data[r][k] = untangle(Z_r[k], Z_m[H-k], W^k).a,
+//   k = H/2      data[r][k] = Z_r[k],
+//   H/2 < k      data[r][k] = untangle(Z_m[H-k], Z_r[k], W^(H-k)).b,
+// W = W_nn3 (isign +1): the pairs k_rlft3_untangle forms.  A row that is
+// its own mirror is written by its first slot only.
+template <typename T, int H, int TI, int NT>
+__global__ __launch_bounds__(NT) void k_rlft3_rows(const cpx<T>* z, cpx<T>* data, cpx<T>* speq, int64_t npairs,
+                                                   int64_t nn1, int64_t nn2, Twiddles<T> tw) {
+    using E = Engine<T, 1, H, TI, NT, MAP_J, MAP_J>;
+    constexpr int HALF = TI / 2;
+    static_assert(TI >= 2 && TI % 2 == 0 && H >= 16 && NT >= TI, "rlft3 row tile");
+    constexpr int LTM = engine_ltm<E>(E::LDS_ELEMS * E::CSIZE);
+    constexpr int LOGNN3 = ilog2c(H) + 1;
+    __shared__ cpx<T> lds[E::LDS_ELEMS];
+    __shared__ __align__(16) cpx<T> ltw[LTM > 0 ? E::tw_elems(LTM) : 1];
+    __shared__ int64_t rows[TI];  // global row of each slot, -1 = none
+    const int tid = threadIdx.x;
+    if (tid < TI) {
+        const int64_t pi = int64_t(blockIdx.x) * HALF + (tid % HALF);
+        int64_t row = -1;
+        if (pi < npairs) {
+            const int64_t plane = nn1 * nn2, h2 = nn2 / 2 + 1, per = plane / 2 + 2;
+            const int64_t s = pi / per;
+            int64_t q = pi - s * per, i1, i2;
+            if (q < h2) {
+                i1 = 0;
+                i2 = q;
+            } else if ((q -= h2) < (nn1 / 2 - 1) * nn2) {
+                i1 = 1 + q / nn2;
+                i2 = q % nn2;
+            } else {
+                i1 = nn1 / 2;
+                i2 = q - (nn1 / 2 - 1) * nn2;
+            }
+            if (tid >= HALF) {
+                i1 = (nn1 - i1) & (nn1 - 1);
+                i2 = (nn2 - i2) & (nn2 - 1);
+            }
+            row = s * plane + i1 * nn2 + i2;
+        }
+        rows[tid] = row;
+    }
+    if constexpr (LTM > 0) E::template stage_tw_to_lds<LTM>(ltw, tw.stage, tid);  // read after the first exchange
+    lds_barrier();  // rows published
+    E::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int i) -> cpx<T> {
+        const int64_t row = rows[t];
+        return row >= 0 ? z[row * H + i] : cpx<T>{(T)0, (T)0};
+    }, false, ltw);
+    const T c2 = (T)-0.5;
+    for (int w = tid; w < TI * H; w += NT) {
+        const int t = w / H, k = w % H;
+        const int tm = t < HALF ? t + HALF : t - HALF;  // the partner's slot
+        const int64_t r = rows[t];
+        if (r < 0 || (t >= HALF && r == rows[tm])) continue;
+        const cpx<T> a = lds[E::lidx(t, k)];
+        cpx<T> o, oa, ob;
+        if (k == 0) {
+            const cpx<T> b = lds[E::lidx(tm, 0)];
+            const cpx<T> one{(T)1, (T)0};
+            untangle_pair(a, b, one, c2, oa, ob);
+            o = oa;
+            untangle_pair(b, a, one, c2, oa, ob);
+            speq[r] = ob;
+        } else if (k == H / 2) {
+            o = a;
+        } else if (k < H / 2) {
+            untangle_pair(a, lds[E::lidx(tm, H - k)], twiddle_big<1>(tw, uint32_t(k), LOGNN3), c2, oa, ob);
+            o = oa;
+        } else {
+            untangle_pair(lds[E::lidx(tm, H - k)], a, twiddle_big<1>(tw, uint32_t(H - k), LOGNN3), c2, oa, ob);
+            o = ob;
+        }
+        data[r * H + k] = o;
+    }
+}
+
 }  // namespace nrfft

@@ -32,7 +32,15 @@ enum SpecSpecial {
     SP_CF2_IUNPACK,   // args: nsig, L
     SP_R3_COPY,       // args: nsig, plane, h
     SP_R3_UNTANGLE,   // args: nsig, nn1, nn2, h, log2 nn3
+    SP_R3_ROWS,       // args: row pairs (batch (nn1 nn2 / 2 + 2)), nn1, nn2, h
 };
+
+// NRFFT_R3_FUSED = 0: the rlft3 forward always runs fourn, the speq copy and
+// the untangle as separate passes; 1: where the shape allows, the last axis
+// (rows of h = nn3/2 points) runs last and carries the untangle and speq.
+#ifndef NRFFT_R3_FUSED
+#define NRFFT_R3_FUSED 1
+#endif
 
 static Launch spec_launch(int special, int src, int dst, std::initializer_list<int64_t> args) {
     Launch l;
@@ -271,16 +279,56 @@ int build_plan_spectral(Plan& p) {
             std::vector<size_t> dims;
             for (int64_t v : {nn1, nn2, h})
                 if (v > 1) dims.push_back((size_t)v);
-            // forward: fourn(+1) on the data (in place) -> speq <- k3 = 0 column -> untangle
+            const int64_t vol = nn1 * nn2 * h;
+            const bool fused = NRFFT_R3_FUSED && nn1 >= 2 && nn2 >= 2 && nn1 < 4096 && nn2 < 4096 && h >= 16 &&
+                               h <= 2048 && vol <= 0x7fffffffLL && spec_lookup(p.dtype, ilog2((size_t)h)).r3rows;
             size_t from = p.launches.size();
-            if (!dims.empty() && (rc = plan_c2c_nd(p, dims, batch))) return rc;
-            tag_launches(p, from, 1, 1);
-            Launch a = spec_launch(SP_R3_COPY, B_OUT, B_AUX2, {batch, nn1 * nn2, h});
-            a.args[7] = 1;
-            p.launches.push_back(a);
-            Launch b = spec_launch(SP_R3_UNTANGLE, B_OUT, B_OUT, {batch, nn1, nn2, h, ilog2((size_t)nn3)});
-            b.args[7] = 1;
-            p.launches.push_back(b);
+            if (fused) {
+                // forward: axis 1 (in -> data, in place for a device plan),
+                // axis 0 (data -> TMP2), then the rows of h points with the
+                // untangle and speq (TMP2 -> data).
+                const int want = col_tilog(p.dtype);
+                PassDesc d1{};
+                d1.nb0 = h;
+                d1.si0 = 1;
+                d1.so0 = 1;
+                d1.esi = (int32_t)h;
+                d1.eso = (int32_t)h;
+                d1.nb1 = nn1 * batch;
+                d1.si1 = nn2 * h;
+                d1.so1 = nn2 * h;
+                d1.nb2 = 1;
+                const int t1 = pick_tilog(p.dtype, MODE_TT, ilog2((size_t)nn2), h, want);
+                if (t1 < 0) return fail(NRFFT_E_UNSUPPORTED, "no column kernel");
+                if ((rc = plan_add_c2c_pass(p, MODE_TT, ilog2((size_t)nn2), t1, d1, B_IN, B_OUT))) return rc;
+                PassDesc d0{};
+                d0.nb0 = nn2 * h;
+                d0.si0 = 1;
+                d0.so0 = 1;
+                d0.esi = (int32_t)(nn2 * h);
+                d0.eso = (int32_t)(nn2 * h);
+                d0.nb1 = batch;
+                d0.si1 = vol;
+                d0.so1 = vol;
+                d0.nb2 = 1;
+                const int t0 = pick_tilog(p.dtype, MODE_TT, ilog2((size_t)nn1), nn2 * h, want);
+                if (t0 < 0) return fail(NRFFT_E_UNSUPPORTED, "no column kernel");
+                if ((rc = plan_add_c2c_pass(p, MODE_TT, ilog2((size_t)nn1), t0, d0, B_OUT, B_TMP2))) return rc;
+                tag_launches(p, from, 1, 1);
+                Launch f = spec_launch(SP_R3_ROWS, B_TMP2, B_OUT, {batch * (nn1 * nn2 / 2 + 2), nn1, nn2, h});
+                f.args[7] = 1;
+                p.launches.push_back(f);
+            } else {
+                // forward: fourn(+1) on the data (in place) -> speq <- k3 = 0 column -> untangle
+                if (!dims.empty() && (rc = plan_c2c_nd(p, dims, batch))) return rc;
+                tag_launches(p, from, 1, 1);
+                Launch a = spec_launch(SP_R3_COPY, B_OUT, B_AUX2, {batch, nn1 * nn2, h});
+                a.args[7] = 1;
+                p.launches.push_back(a);
+                Launch b = spec_launch(SP_R3_UNTANGLE, B_OUT, B_OUT, {batch, nn1, nn2, h, ilog2((size_t)nn3)});
+                b.args[7] = 1;
+                p.launches.push_back(b);
+            }
             // inverse: untangle -> fourn(-1)
             Launch c = spec_launch(SP_R3_UNTANGLE, B_OUT, B_OUT, {batch, nn1, nn2, h, ilog2((size_t)nn3)});
             c.args[7] = -1;
@@ -288,6 +336,10 @@ int build_plan_spectral(Plan& p) {
             from = p.launches.size();
             if (!dims.empty() && (rc = plan_c2c_nd(p, dims, batch))) return rc;
             tag_launches(p, from, -1, -1);
+            if (fused) {  // TMP2 (the forward's axis-0 output) after the inverse plan's own scratch
+                p.tmp2_offset = std::max<size_t>(p.tmp2_offset, p.tmp_elems);
+                p.tmp_elems = p.tmp2_offset + size_t(batch) * size_t(vol);
+            }
             return NRFFT_OK;
         }
         default:
@@ -388,6 +440,17 @@ static int exec_spec_t(const Plan& p, const Launch& l, void* const* bufs, int is
             hipLaunchKernelGGL((k_rlft3_copycol<T>), dim3(grid_of(nsig * a[1], 256)), dim3(256), 0, s,
                                (const cpx<T>*)src, (cpx<T>*)dst, a[1], a[2], nsig);
             break;
+        case SP_R3_ROWS: {
+            const SpecRef& r = spec_lookup(p.dtype, ilog2((size_t)a[3]));
+            const cpx<T>* zp = (const cpx<T>*)src;
+            cpx<T>* dp = (cpx<T>*)dst;
+            cpx<T>* sp = (cpx<T>*)bufs[B_AUX2];
+            int64_t npairs = a[0], n1v = a[1], n2v = a[2];
+            void* args[] = {&zp, &dp, &sp, &npairs, &n1v, &n2v, &tw};
+            const int64_t half = r.r3_ti / 2;
+            HIPCHK(hipLaunchKernel(r.r3rows, dim3((unsigned)((npairs + half - 1) / half)), dim3(r.r3_nt), args, 0, s));
+            return NRFFT_OK;
+        }
         case SP_R3_UNTANGLE: {
             const int64_t nk = a[3] > 1 ? a[3] / 2 : 1;
             const unsigned g = grid_of(nsig * a[1] * a[2] * nk, 256);

@@ -21,7 +21,22 @@ struct SpecRef {
     const void* cosft1;
     const void* cosft2[2];  // [0] forward, [1] inverse
     int nt, ti;
+    const void* r3rows;  // rlft3's fused last axis for rows of L points (16 <= L <= 2048), r3_ti rows a tile
+    int r3_nt, r3_ti;
 };
+
+// The fused rlft3 last-axis kernel for rows of L points: TI = 4096 / L FP32
+// (2048 / L FP64) rows a tile, at least 2.
+template <typename T, int L>
+constexpr int r3_ti() {
+    constexpr int q = (sizeof(T) == 4 ? 4096 : 2048) / L;
+    return q < 2 ? 2 : q;
+}
+template <typename T, int L>
+const void* r3rows_ref() {
+    if constexpr (L >= 16 && L <= 2048) return reinterpret_cast<const void*>(&k_rlft3_rows<T, L, r3_ti<T, L>(), r3_ti<T, L>() * L / 16>);
+    else return nullptr;
+}
 
 template <typename T, int LOGL>
 SpecRef spec_ref_one() {
@@ -34,7 +49,10 @@ SpecRef spec_ref_one() {
                    {reinterpret_cast<const void*>(&k_cosft2<T, L, TI, NT, 1>),
                     reinterpret_cast<const void*>(&k_cosft2<T, L, TI, NT, -1>)},
                    NT,
-                   TI};
+                   TI,
+                   r3rows_ref<T, L>(),
+                   r3_ti<T, L>() * L / 16,
+                   r3_ti<T, L>()};
 }
 
 template <typename T, int... LS>
@@ -46,7 +64,7 @@ template <typename T>
 struct SpecTable {
     SpecRef t[SPEC_MAX_LOGL + 1];
     SpecTable() {
-        for (auto& e : t) e = SpecRef{nullptr, nullptr, {nullptr, nullptr}, 0, 0};
+        for (auto& e : t) e = SpecRef{nullptr, nullptr, {nullptr, nullptr}, 0, 0, nullptr, 0, 0};
         spec_fill<T>(t, std::make_integer_sequence<int, spec_max_logl<T>()>{});
     }
 };

@@ -247,8 +247,10 @@ def test_cosft2_invalid_isign():
 
 
 # ---- rlft3 --------------------------------------------------------------------
+# From nn1, nn2 >= 2 and 32 <= nn3 <= 4096 the forward fuses the untangle
+# into the last axis (rows of nn3/2 points with their mirror rows).
 SHAPES = [(4, 8, 16), (1, 8, 16), (8, 8, 8), (2, 2, 4), (1, 1, 8), (4, 4, 2), (16, 32, 64), (8, 64, 256),
-          (1, 1, 1 << 15)]
+          (1, 1, 1 << 15), (2, 2, 32), (32, 8, 128), (4, 16, 4096), (2, 256, 64)]
 
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
