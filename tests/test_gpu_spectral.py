@@ -402,3 +402,27 @@ def test_twofft_chunked_f64():
         r1, r2 = oracle.twofft(d1[i * n:(i + 1) * n], d2[i * n:(i + 1) * n])
         seg = slice(2 * i * n, 2 * (i + 1) * n)
         assert rel_l2(g1[seg], r1) < 1e-12 and rel_l2(g2[seg], r2) < 1e-12, i
+
+
+@pytest.mark.parametrize("dt,shape,b", [(np.float32, (8, 4, 64), 3), (np.float64, (2, 16, 256), 2),
+                                        (np.float32, (16, 2, 4096), 2)])
+def test_rlft3_plan_fused_batched(dt, shape, b):
+    # Batched device plans on the fused forward (row pairs enumerated per
+    # volume, nn1 > nn2 and nn1 < nn2, the longest fused rows): every
+    # volume's data and speq against the oracle, then the inverse round trip.
+    import torch
+
+    nn1, nn2, nn3 = shape
+    vol, ns = nn1 * nn2 * nn3, 2 * nn1 * nn2
+    x = _in(87, b * vol, dt)
+    t = torch.from_numpy(x).cuda()
+    sp = torch.zeros(b * ns, dtype=t.dtype, device="cuda")
+    plan = numrs_amd.Rlft3Plan(np.dtype(dt).name, nn1, nn2, nn3, b)
+    plan.execute(t, sp, 1)
+    gd, gs = t.cpu().numpy(), sp.cpu().numpy()
+    for i in range(b):
+        rd, rs = oracle.rlft3(x[i * vol:(i + 1) * vol].astype(np.float64), np.zeros(ns), nn1, nn2, nn3, 1)
+        assert rel_l2(gd[i * vol:(i + 1) * vol], rd) < TOL[dt], ("data", i)
+        assert rel_l2(gs[i * ns:(i + 1) * ns], rs) < TOL[dt], ("speq", i)
+    plan.execute(t, sp, -1)
+    assert rel_l2(t.cpu().numpy(), 0.5 * vol * x.astype(np.float64)) < 10 * TOL[dt]
