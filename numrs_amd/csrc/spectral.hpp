@@ -642,8 +642,9 @@ __global__ void k_rlft3_untangle(cpx<T>* data, cpx<T>* speq, int64_t nn1, int64_
 // rlft3 forward, last axis fused with the untangle.  The rows (i1, i2) of
 // each volume pair with their mirrors m = (-i1, -i2); the pairs are
 // enumerated per volume as i1 = 0 with i2 <= nn2/2, 0 < i1 < nn1/2 with
-// every i2, i1 = nn1/2 with i2 <= nn2/2 (nn1 nn2 / 2 + 2 pairs; four rows
-// are their own mirror).  A tile holds the H = nn3/2-point rows of TI/2
+// every i2, i1 = nn1/2 with i2 <= nn2/2 (nn1 nn2 / 2 + 2 pairs, four rows
+// their own mirror; nn1 = 1, the 2-D case: i1 = 0 only, nn2 / 2 + 1 pairs,
+// two rows their own mirror).  A tile holds the H = nn3/2-point rows of TI/2
 // pairs, slot t the row and slot t + TI/2 its mirror, read from z (the
 // other axes already transformed, out of place); after the FFTs each row r
 // of the tile, with m its partner, writes
@@ -670,7 +671,7 @@ __global__ __launch_bounds__(NT) void k_rlft3_rows(const cpx<T>* z, cpx<T>* data
         const int64_t pi = int64_t(blockIdx.x) * HALF + (tid % HALF);
         int64_t row = -1;
         if (pi < npairs) {
-            const int64_t plane = nn1 * nn2, h2 = nn2 / 2 + 1, per = plane / 2 + 2;
+            const int64_t plane = nn1 * nn2, h2 = nn2 / 2 + 1, per = nn1 >= 2 ? plane / 2 + 2 : h2;
             const int64_t s = pi / per;
             int64_t q = pi - s * per, i1, i2;
             if (q < h2) {

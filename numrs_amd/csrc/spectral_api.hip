@@ -32,7 +32,7 @@ enum SpecSpecial {
     SP_CF2_IUNPACK,   // args: nsig, L
     SP_R3_COPY,       // args: nsig, plane, h
     SP_R3_UNTANGLE,   // args: nsig, nn1, nn2, h, log2 nn3
-    SP_R3_ROWS,       // args: row pairs (batch (nn1 nn2 / 2 + 2)), nn1, nn2, h
+    SP_R3_ROWS,       // args: row pairs of the batch, nn1, nn2, h
 };
 
 // NRFFT_R3_FUSED = 0: the rlft3 forward always runs fourn, the speq copy and
@@ -280,13 +280,13 @@ int build_plan_spectral(Plan& p) {
             for (int64_t v : {nn1, nn2, h})
                 if (v > 1) dims.push_back((size_t)v);
             const int64_t vol = nn1 * nn2 * h;
-            const bool fused = NRFFT_R3_FUSED && nn1 >= 2 && nn2 >= 2 && nn1 < 4096 && nn2 < 4096 && h >= 16 &&
-                               h <= 2048 && vol <= 0x7fffffffLL && spec_lookup(p.dtype, ilog2((size_t)h)).r3rows;
+            const bool fused = NRFFT_R3_FUSED && nn2 >= 2 && nn1 < 4096 && nn2 < 4096 && h >= 16 && h <= 2048 &&
+                               vol <= 0x7fffffffLL && spec_lookup(p.dtype, ilog2((size_t)h)).r3rows;
             size_t from = p.launches.size();
             if (fused) {
-                // forward: axis 1 (in -> data, in place for a device plan),
-                // axis 0 (data -> TMP2), then the rows of h points with the
-                // untangle and speq (TMP2 -> data).
+                // forward: axis 1 (in -> data, in place for a device plan;
+                // in -> TMP2 when nn1 = 1), axis 0 (data -> TMP2), then the
+                // rows of h points with the untangle and speq (TMP2 -> data).
                 const int want = col_tilog(p.dtype);
                 PassDesc d1{};
                 d1.nb0 = h;
@@ -300,7 +300,11 @@ int build_plan_spectral(Plan& p) {
                 d1.nb2 = 1;
                 const int t1 = pick_tilog(p.dtype, MODE_TT, ilog2((size_t)nn2), h, want);
                 if (t1 < 0) return fail(NRFFT_E_UNSUPPORTED, "no column kernel");
-                if ((rc = plan_add_c2c_pass(p, MODE_TT, ilog2((size_t)nn2), t1, d1, B_IN, B_OUT))) return rc;
+                if ((rc = plan_add_c2c_pass(p, MODE_TT, ilog2((size_t)nn2), t1, d1, B_IN, nn1 >= 2 ? B_OUT : B_TMP2)))
+                    return rc;
+            }
+            if (fused && nn1 >= 2) {
+                const int want = col_tilog(p.dtype);
                 PassDesc d0{};
                 d0.nb0 = nn2 * h;
                 d0.si0 = 1;
@@ -314,8 +318,11 @@ int build_plan_spectral(Plan& p) {
                 const int t0 = pick_tilog(p.dtype, MODE_TT, ilog2((size_t)nn1), nn2 * h, want);
                 if (t0 < 0) return fail(NRFFT_E_UNSUPPORTED, "no column kernel");
                 if ((rc = plan_add_c2c_pass(p, MODE_TT, ilog2((size_t)nn1), t0, d0, B_OUT, B_TMP2))) return rc;
+            }
+            if (fused) {
                 tag_launches(p, from, 1, 1);
-                Launch f = spec_launch(SP_R3_ROWS, B_TMP2, B_OUT, {batch * (nn1 * nn2 / 2 + 2), nn1, nn2, h});
+                const int64_t per = nn1 >= 2 ? nn1 * nn2 / 2 + 2 : nn2 / 2 + 1;  // row pairs of a volume
+                Launch f = spec_launch(SP_R3_ROWS, B_TMP2, B_OUT, {batch * per, nn1, nn2, h});
                 f.args[7] = 1;
                 p.launches.push_back(f);
             } else {

@@ -247,10 +247,11 @@ def test_cosft2_invalid_isign():
 
 
 # ---- rlft3 --------------------------------------------------------------------
-# From nn1, nn2 >= 2 and 32 <= nn3 <= 4096 the forward fuses the untangle
-# into the last axis (rows of nn3/2 points with their mirror rows).
+# From nn2 >= 2 and 32 <= nn3 <= 4096 the forward fuses the untangle into
+# the last axis (rows of nn3/2 points with their mirror rows; nn1 = 1 is the
+# 2-D case).
 SHAPES = [(4, 8, 16), (1, 8, 16), (8, 8, 8), (2, 2, 4), (1, 1, 8), (4, 4, 2), (16, 32, 64), (8, 64, 256),
-          (1, 1, 1 << 15), (2, 2, 32), (32, 8, 128), (4, 16, 4096), (2, 256, 64)]
+          (1, 1, 1 << 15), (2, 2, 32), (32, 8, 128), (4, 16, 4096), (2, 256, 64), (1, 16, 64), (1, 256, 512)]
 
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
@@ -405,10 +406,10 @@ def test_twofft_chunked_f64():
 
 
 @pytest.mark.parametrize("dt,shape,b", [(np.float32, (8, 4, 64), 3), (np.float64, (2, 16, 256), 2),
-                                        (np.float32, (16, 2, 4096), 2)])
+                                        (np.float32, (16, 2, 4096), 2), (np.float64, (1, 8, 128), 3)])
 def test_rlft3_plan_fused_batched(dt, shape, b):
     # Batched device plans on the fused forward (row pairs enumerated per
-    # volume, nn1 > nn2 and nn1 < nn2, the longest fused rows): every
+    # volume, nn1 > nn2 and nn1 < nn2, the longest fused rows, 2-D): every
     # volume's data and speq against the oracle, then the inverse round trip.
     import torch
 
