@@ -38,6 +38,23 @@ __device__ __forceinline__ T cos_quarter_pi() {
 #endif
 constexpr bool CF2_LDS = NRFFT_CF2_LDS != 0;
 
+// NRFFT_CF1_PERSIST = 1: the one-group cosft1 whose tile takes more than
+// half the LDS (one group per CU: FP32 n = 16384, FP64 n = 8192) runs
+// persistent over signals with the next signal's loads in flight during the
+// finishing (2: during the FFT as well, a second register set).  MI355X,
+// 4096 x 2^14 FP32 0.270 -> 0.252 ms, FP64 4096 x 2^13 0.244 -> 0.228; with
+// two or more groups per CU it measured 7-10% slower, and the second
+// register set spilled (0.378 ms).
+#ifndef NRFFT_CF1_PERSIST
+#define NRFFT_CF1_PERSIST 1
+#endif
+constexpr bool CF1_PERSIST = NRFFT_CF1_PERSIST != 0;
+template <typename T, int L, int TI, int NT>
+constexpr bool cf1_persistent() {
+    using E = Engine<T, 1, L, TI, NT, MAP_J, MAP_J>;
+    return CF1_PERSIST && TI == 1 && E::LDS_ELEMS * E::CSIZE > 80 * 1024;
+}
+
 // Four consecutive reals at p: one 16-byte store (FP32) or two (FP64) when
 // p is 16-byte aligned, else element by element.
 template <typename T>
@@ -171,13 +188,65 @@ __global__ __launch_bounds__(NT) void k_cosft1(const T* y, T* out, int64_t nsig,
     __shared__ cpx<T> uw[PPT1];
     PairTw<T, PPT1> ptw;
     if constexpr (TI == 1) ptw.init(uw, tw, tid, NT, LOG2N2);
+    constexpr int NP = L / 2 + 1;
+    const T half = (T)0.5;
+    if constexpr (cf1_persistent<T, L, TI, NT>()) {
+        // Persistent over signals blockIdx.x + k gridDim.x, the next
+        // signal's stage-0 loads in flight while this one finishes (signals
+        // are disjoint, so in place is safe).
+        const int64_t G = gridDim.x;
+        auto load = [&](cpx<T>* v, int64_t sg) {
+            const T* ys = y + sg * S;
+            E::load0(v, tid, [&](int, int m) -> cpx<T> {
+                return cpx<T>{cf1_ext(ys, 2 * m, L), cf1_ext(ys, 2 * m + 1, L)};
+            });
+        };
+        auto finish = [&](int64_t sg) {
+            T* os = out + sg * S;
+            int u = 0;
+            for (int p = tid; p < NP; p += NT, ++u) {
+                if (p == 0) {
+                    const cpx<T> z0 = lds[E::lidx(0, 0)];
+                    os[0] = half * (z0.x + z0.y);
+                    os[L] = half * (z0.x - z0.y);
+                } else {
+                    cpx<T> fa, fb;
+                    untangle_pair(lds[E::lidx(0, p)], lds[E::lidx(0, L - p)], ptw.at(uw, u), (T)-0.5, fa, fb);
+                    os[p] = half * fa.x;
+                    if (L - p != p) os[L - p] = half * fb.x;
+                }
+            }
+        };
+        auto step = [&](cpx<T>* vc, cpx<T>* vn, int64_t sg) {
+            if (sg + G < nsig) load(vn, sg + G);
+            E::template run_to_lds_loaded<LTM>(vc, tid, lds, tw.stage, true, ltw);
+            finish(sg);
+        };
+        if constexpr (NRFFT_CF1_PERSIST == 2) {
+            cpx<T> va[E::EPT], vb[E::EPT];
+            if (s0 < nsig) load(va, s0);
+            for (int64_t sg = s0; sg < nsig; sg += 2 * G) {
+                step(va, vb, sg);
+                if (sg + G < nsig) step(vb, va, sg + G);
+            }
+        } else {
+            // One register set: the next loads are issued after the FFT and
+            // overlap the finishing only.
+            cpx<T> v[E::EPT];
+            if (s0 < nsig) load(v, s0);
+            for (int64_t sg = s0; sg < nsig; sg += G) {
+                E::template run_to_lds_loaded<LTM>(v, tid, lds, tw.stage, true, ltw);
+                if (sg + G < nsig) load(v, sg + G);
+                finish(sg);
+            }
+        }
+        return;
+    }
     E::template run_to_lds<LTM>(tid, lds, tw.stage, [&](int t, int m) -> cpx<T> {
         if (!(TI == 1 || t < nvalid)) return cpx<T>{(T)0, (T)0};
         const T* ys = yb + t * S;
         return cpx<T>{cf1_ext(ys, 2 * m, L), cf1_ext(ys, 2 * m + 1, L)};
     }, false, ltw);
-    constexpr int NP = L / 2 + 1;
-    const T half = (T)0.5;
     int u = 0;
     for (int q = tid; q < TI * NP; q += NT, ++u) {
         const int t = q / NP, p = q % NP;

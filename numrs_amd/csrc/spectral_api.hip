@@ -1,5 +1,6 @@
 // spectral_api.hip -- twofft / cosft1 / cosft2 / rlft3: planning, execution
 // and the extern "C" entry points of include/nrfft.h.
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -397,10 +398,24 @@ static int exec_spec_t(const Plan& p, const Launch& l, void* const* bufs, int is
             break;
         }
         case SP_CF1_FUSED: {
-            const SpecRef& r = spec_lookup(p.dtype, ilog2((size_t)a[1]));
+            const int logl = ilog2((size_t)a[1]);
+            const SpecRef& r = spec_lookup(p.dtype, logl);
             int64_t ns = nsig;
             void* args[] = {&src, &dst, &ns, &tw};
-            HIPCHK(hipLaunchKernel(r.cosft1, dim3((unsigned)((nsig + r.ti - 1) / r.ti)), dim3(r.nt), args, 0, s));
+            int64_t grid = (nsig + r.ti - 1) / r.ti;
+            if (r.cf1_persist) {
+                // Persistent: as many groups as the GPU holds at once.
+                static std::atomic<int> per_cu[2][SPEC_MAX_LOGL + 1];
+                std::atomic<int>& pc = per_cu[p.dtype == NRFFT_F32 ? 0 : 1][logl];
+                int g = pc.load();
+                if (g == 0) {
+                    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&g, r.cosft1, r.nt, 0));
+                    g = std::max(g, 1);
+                    pc.store(g);
+                }
+                grid = std::min<int64_t>(grid, int64_t(g) * ctx->ncu);
+            }
+            HIPCHK(hipLaunchKernel(r.cosft1, dim3((unsigned)grid), dim3(r.nt), args, 0, s));
             return NRFFT_OK;
         }
         case SP_CF1_PACK:

@@ -23,6 +23,7 @@ struct SpecRef {
     int nt, ti;
     const void* r3rows;  // rlft3's fused last axis for rows of L points (16 <= L <= 2048), r3_ti rows a tile
     int r3_nt, r3_ti;
+    bool cf1_persist;  // cosft1 is persistent over signals (launch at most the resident groups)
 };
 
 // The fused rlft3 last-axis kernel for rows of L points: TI = 4096 / L FP32
@@ -52,7 +53,8 @@ SpecRef spec_ref_one() {
                    TI,
                    r3rows_ref<T, L>(),
                    r3_ti<T, L>() * L / 16,
-                   r3_ti<T, L>()};
+                   r3_ti<T, L>(),
+                   cf1_persistent<T, L, TI, NT>()};
 }
 
 template <typename T, int... LS>
@@ -64,7 +66,7 @@ template <typename T>
 struct SpecTable {
     SpecRef t[SPEC_MAX_LOGL + 1];
     SpecTable() {
-        for (auto& e : t) e = SpecRef{nullptr, nullptr, {nullptr, nullptr}, 0, 0, nullptr, 0, 0};
+        for (auto& e : t) e = SpecRef{nullptr, nullptr, {nullptr, nullptr}, 0, 0, nullptr, 0, 0, false};
         spec_fill<T>(t, std::make_integer_sequence<int, spec_max_logl<T>()>{});
     }
 };

@@ -90,12 +90,15 @@ def test_cosft1_vs_oracle(dt, n):
     assert rel_l2(g, 0.5 * n * y.astype(np.float64)) < 10 * TOL[dt], n
 
 
-@pytest.mark.parametrize("dt,n,b", [(np.float32, 16384, 5), (np.float32, 4096, 3), (np.float64, 8192, 4)])
+@pytest.mark.parametrize("dt,n,b", [(np.float32, 16384, 5), (np.float32, 4096, 3), (np.float64, 8192, 4),
+                                    (np.float32, 16384, 600), (np.float64, 8192, 520)])
 @pytest.mark.parametrize("inplace", [True, False])
 def test_cosft1_fused_batched(dt, n, b, inplace):
     # Batches of the one-group cosft1 whose signals of n + 1 values start at
     # any offset: every signal against the oracle and scipy's DCT-I, in place
-    # and out of place.
+    # and out of place.  FP32 16384 and FP64 8192 run persistent (one group
+    # per CU, each taking every gridDim-th signal): 600 / 520 signals give
+    # groups two or three each.
     import scipy.fft as sfft
     import torch
 
