@@ -44,7 +44,8 @@ constexpr bool CF2_LDS = NRFFT_CF2_LDS != 0;
 // finishing (2: during the FFT as well, a second register set).  MI355X,
 // 4096 x 2^14 FP32 0.270 -> 0.252 ms, FP64 4096 x 2^13 0.244 -> 0.228; with
 // two or more groups per CU it measured 7-10% slower, and the second
-// register set spilled (0.378 ms).
+// register set spilled (0.378 ms).  (k_twofft at one group per CU the same
+// way: 4096 x 2^14 FP32 0.386 against 0.356 ms -- its stores bound it.)
 #ifndef NRFFT_CF1_PERSIST
 #define NRFFT_CF1_PERSIST 1
 #endif
