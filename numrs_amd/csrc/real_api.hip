@@ -20,6 +20,11 @@ namespace nrfft {
     } while (0)
 
 constexpr int REAL_MAX_LOGL = 13;  // fused single-group path: n/2 <= 8192
+// NRFFT_REALFT_16K = 1: FP32 realft also runs fused at n/2 = 16384 (one
+// group per CU, 135 KB of LDS); 0: the four-step C2C and an untangle pass.
+#ifndef NRFFT_REALFT_16K
+#define NRFFT_REALFT_16K 1
+#endif
 // NRFFT_SPEC_RAW = 0: convlv / correl beyond one group untangle both forward
 // spectra in passes of their own before the spectrum operation.
 #ifndef NRFFT_SPEC_RAW
@@ -218,10 +223,17 @@ void real_fill(RealRef* tab, std::integer_sequence<int, LS...>) {
 
 template <typename T>
 struct RealTable {
-    RealRef t[REAL_MAX_LOGL + 1];
+    RealRef t[REAL_MAX_LOGL + 2];  // [REAL_MAX_LOGL + 1]: FP32 realft at n/2 = 16384 only
     RealTable() {
         t[0] = RealRef{nullptr, nullptr, nullptr, nullptr, 0, 0};
         real_fill<T>(t, std::make_integer_sequence<int, REAL_MAX_LOGL>{});
+        t[REAL_MAX_LOGL + 1] = RealRef{nullptr, nullptr, nullptr, nullptr, 0, 0};
+        if constexpr (sizeof(T) == 4 && NRFFT_REALFT_16K) {
+            constexpr int L = 2 << REAL_MAX_LOGL, NT = L / 16;
+            t[REAL_MAX_LOGL + 1] = RealRef{reinterpret_cast<const void*>(&k_realft<T, L, 1, NT, 1>),
+                                           reinterpret_cast<const void*>(&k_realft<T, L, 1, NT, -1>), nullptr,
+                                           nullptr, NT, 1};
+        }
     }
 };
 
@@ -297,7 +309,7 @@ int build_plan_real(Plan& p) {
     if (p.kind == PK_REALFT) {
         if (L == 1) {
             p.launches.push_back(real_launch(RS_N2, B_IN, B_OUT, {batch}));
-        } else if (logl <= REAL_MAX_LOGL) {
+        } else if (logl <= REAL_MAX_LOGL || (logl == REAL_MAX_LOGL + 1 && real_lookup(p.dtype, logl).fwd)) {
             p.launches.push_back(real_launch(RS_FUSED_REALFT, B_IN, B_OUT, {batch, L}));
         } else {
             // Forward: C2C(+1) then untangle; inverse: untangle^-1 then C2C(-1).

@@ -362,6 +362,31 @@ def test_realft_plan_multipass_chunked_f32():
         assert rel_l2(back[i * n:(i + 1) * n], (n / 2) * x[i * n:(i + 1) * n].astype(np.float64)) < TOL32, i
 
 
+@pytest.mark.parametrize("b", [1, 37])
+def test_realft_plan_fused_2p15_f32(b):
+    # FP32 signals of 2^15 reals run in one group each (16384-point complex
+    # FFT and the untangle in LDS): every signal against numpy's FP64 rfft
+    # in NR's packing, then the inverse returns (n/2) x, in place.
+    import torch
+
+    n = 1 << 15
+    x = synth.normal(33, b * n).astype(np.float32)
+    t = torch.from_numpy(x).cuda()
+    plan = numrs_amd.RealFTPlan("float32", n, b)
+    plan.execute(t, 1)
+    got = t.cpu().numpy()
+    for i in range(b):
+        f = np.conj(np.fft.rfft(x[i * n:(i + 1) * n].astype(np.float64)))
+        ref = np.empty(n)
+        ref[0], ref[1] = f[0].real, f[n // 2].real
+        ref[2::2], ref[3::2] = f[1:n // 2].real, f[1:n // 2].imag
+        assert rel_l2(got[i * n:(i + 1) * n], ref) < TOL32, i
+    plan.execute(t, -1)
+    back = t.cpu().numpy()
+    for i in range(b):
+        assert rel_l2(back[i * n:(i + 1) * n], (n / 2) * x[i * n:(i + 1) * n].astype(np.float64)) < TOL32, i
+
+
 def test_realft_plan_three_pass_f32():
     # 3 FP32 signals of 2^24 reals: the 2^23-point C2C runs as three passes
     # (chunks of 2 and 1) with one untangle pass after it; the inverse reads
