@@ -25,6 +25,11 @@ constexpr int REAL_MAX_LOGL = 13;  // fused single-group path: n/2 <= 8192
 #ifndef NRFFT_REALFT_16K
 #define NRFFT_REALFT_16K 1
 #endif
+// NRFFT_CONV_16K = 1: FP32 convlv / correl likewise at n/2 = 16384 (the
+// full-FFT form: both spectra and the inverse in one group).
+#ifndef NRFFT_CONV_16K
+#define NRFFT_CONV_16K 1
+#endif
 // NRFFT_SPEC_RAW = 0: convlv / correl beyond one group untangle both forward
 // spectra in passes of their own before the spectrum operation.
 #ifndef NRFFT_SPEC_RAW
@@ -230,9 +235,11 @@ struct RealTable {
         t[REAL_MAX_LOGL + 1] = RealRef{nullptr, nullptr, nullptr, nullptr, 0, 0};
         if constexpr (sizeof(T) == 4 && NRFFT_REALFT_16K) {
             constexpr int L = 2 << REAL_MAX_LOGL, NT = L / 16;
+            const void* conv = nullptr;
+            if constexpr (NRFFT_CONV_16K) conv = reinterpret_cast<const void*>(&k_convlv<T, L, 1, NT, 0>);
             t[REAL_MAX_LOGL + 1] = RealRef{reinterpret_cast<const void*>(&k_realft<T, L, 1, NT, 1>),
-                                           reinterpret_cast<const void*>(&k_realft<T, L, 1, NT, -1>), nullptr,
-                                           nullptr, NT, 1};
+                                           reinterpret_cast<const void*>(&k_realft<T, L, 1, NT, -1>), conv, nullptr,
+                                           NT, 1};
         }
     }
 };
@@ -354,7 +361,7 @@ int build_plan_real(Plan& p) {
         p.launches.push_back(real_launch(RS_CONV_N2, B_IN, B_OUT, {batch, m}));
         return NRFFT_OK;
     }
-    if (logl <= REAL_MAX_LOGL) {
+    if (logl <= REAL_MAX_LOGL || (logl == REAL_MAX_LOGL + 1 && real_lookup(p.dtype, logl).conv)) {
         p.launches.push_back(real_launch(RS_FUSED_CONV, B_IN, B_OUT, {batch, L, m}));
         return NRFFT_OK;
     }

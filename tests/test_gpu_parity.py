@@ -250,6 +250,26 @@ def test_convlv_batch_f32_multipass(n, b, isign):
         assert rel_l2(y[i], ref) < 1e-5, (n, isign, i)
 
 
+@pytest.mark.parametrize("m,isign", [(257, 1), (257, -1), (1001, 1)])
+def test_convlv_batch_f32_fused_2p15(m, isign):
+    # FP32 signals of 2^15 in one group each (both 16384-point spectra and
+    # the inverse in LDS, the full-FFT form): every signal against the oracle.
+    n, b = 1 << 15, 5
+    d = f32round(synth.normal(n + 7, n * b)).reshape(b, n)
+    if isign == 1:
+        x = np.arange(m) - (m - 1) / 2
+        r = np.stack([f32round(np.exp(-0.5 * (x / 40.0) ** 2) * (1 + 0.1 * synth.normal(70 + i, m)))
+                      for i in range(b)])
+    else:
+        r = np.stack([f32round(0.3 / m * synth.normal(70 + i, m)) for i in range(b)])
+        r[:, (m - 1) // 2] = 1.0
+    y = numrs_amd.convlv_batch_responses(d.astype(np.float32), r.astype(np.float32), isign)
+    for i in range(b):
+        assert rel_l2(y[i], oracle.convlv(d[i], r[i], isign)) < 1e-5, (m, isign, i)
+    a1, a2 = f32round(synth.normal(91, n)), f32round(synth.normal(92, n))
+    assert rel_l2(numrs_amd.correl(a1.astype(np.float32), a2.astype(np.float32)), oracle.correl(a1, a2)) < 1e-5
+
+
 @pytest.mark.parametrize("isign", [1, -1])
 def test_convlv_batch_f64_multipass(isign):
     # FP64 convlv beyond one group with 8 signals of 2^16: the response's
