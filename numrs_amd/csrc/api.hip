@@ -124,8 +124,16 @@ int find_ctx(DeviceCtx** out) {
 // ---------------------------------------------------------------------------
 // Kernel lookup.
 // ---------------------------------------------------------------------------
+// NRFFT_C2C_16K = 1: FP32 C2C of 16384 points runs as one row pass (one
+// transform per group, one group per CU) instead of a four-step.
+#ifndef NRFFT_C2C_16K
+#define NRFFT_C2C_16K 1
+#endif
+
 const KernelRef& c2c_lookup(int dtype, int sign, int mode, int logl, int tilog) {
     static const KernelRef none{nullptr, 0, 0, 0};
+    if (NRFFT_C2C_16K && dtype == NRFFT_F32 && logl == C2C_MAX_LOGL_ + 1 && mode == MODE_JJ && tilog == 0)
+        return sign > 0 ? c2c16k_lookup_f32p() : c2c16k_lookup_f32m();
     if (logl < 1 || logl > C2C_MAX_LOGL_ || tilog < 0 || tilog >= C2C_NTI_ || mode < 0 || mode > 3) return none;
     if (dtype == NRFFT_F32) return sign > 0 ? c2c_lookup_f32p(mode, logl, tilog) : c2c_lookup_f32m(mode, logl, tilog);
     return sign > 0 ? c2c_lookup_f64p(mode, logl, tilog) : c2c_lookup_f64m(mode, logl, tilog);
@@ -336,7 +344,9 @@ int plan_c2c_1d(Plan& p, int logn, int64_t batch, int src, int dst, size_t* tmp_
         p.launches.push_back(l);
         return NRFFT_OK;
     }
-    if (logn <= C2C_MAX_LOGL_) {
+    // (FP32 16384 points: one pass as well, except for split-precision plans.)
+    if (logn <= C2C_MAX_LOGL_ || (logn == C2C_MAX_LOGL_ + 1 && !(p.kind == PK_C2C && p.extra != 0) &&
+                                  c2c_lookup(p.dtype, +1, MODE_JJ, logn, 0).fn)) {
         PassDesc d{};
         d.nb0 = batch;
         d.nb1 = 1;

@@ -16,6 +16,8 @@ const KernelRef& c2c_lookup_f32m(int mode, int logl, int tilog);
 const KernelRef& c2c_lookup_f64p(int mode, int logl, int tilog);
 const KernelRef& c2c_lookup_f64m(int mode, int logl, int tilog);
 const KernelRef& c2c_lookup(int dtype, int sign, int mode, int logl, int tilog);
+const KernelRef& c2c16k_lookup_f32p();
+const KernelRef& c2c16k_lookup_f32m();
 const DmaRef& dma_lookup_f32p(int logl);
 const DmaRef& dma_lookup_f32m(int logl);
 int pick_tilog(int dtype, int mode, int logl, int64_t nb0, int want_log);
