@@ -129,6 +129,7 @@ int find_ctx(DeviceCtx** out) {
 #ifndef NRFFT_C2C_16K
 #define NRFFT_C2C_16K 1
 #endif
+constexpr int64_t C2C_16K_MIN_BATCH = 128;
 
 const KernelRef& c2c_lookup(int dtype, int sign, int mode, int logl, int tilog) {
     static const KernelRef none{nullptr, 0, 0, 0};
@@ -344,9 +345,12 @@ int plan_c2c_1d(Plan& p, int logn, int64_t batch, int src, int dst, size_t* tmp_
         p.launches.push_back(l);
         return NRFFT_OK;
     }
-    // (FP32 16384 points: one pass as well, except for split-precision plans.)
-    if (logn <= C2C_MAX_LOGL_ || (logn == C2C_MAX_LOGL_ + 1 && !(p.kind == PK_C2C && p.extra != 0) &&
-                                  c2c_lookup(p.dtype, +1, MODE_JJ, logn, 0).fn)) {
+    // (FP32 16384 points: one pass as well for batches of C2C_16K_MIN_BATCH
+    // and more -- one transform per CU, so a small batch keeps the
+    // four-step, which spreads each transform over many CUs -- except for
+    // split-precision plans.)
+    if (logn <= C2C_MAX_LOGL_ || (logn == C2C_MAX_LOGL_ + 1 && batch >= C2C_16K_MIN_BATCH &&
+                                  !(p.kind == PK_C2C && p.extra != 0) && c2c_lookup(p.dtype, +1, MODE_JJ, logn, 0).fn)) {
         PassDesc d{};
         d.nb0 = batch;
         d.nb1 = 1;
