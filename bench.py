@@ -620,6 +620,36 @@ def _extra_convlv() -> dict:
         "GFLOPs": round(2.5 * n * 21 * b / (ms * 1e-3) / 1e9, 1)}
     del plan, x, copies
     torch.cuda.empty_cache()
+
+    # FP32 realft and convlv at n = 2^15, one group per signal (the
+    # 16384-point FFTs in LDS): realft 4096 x 2^15 forward, convlv 2048 x
+    # 2^15 with m = 257 per-signal responses.
+    b, n = 4096, 1 << 15
+    x = torch.empty(b * n, dtype=torch.float32, device="cuda")
+    fill_normal(x, 3, b * n)
+    plan = numrs_amd.RealFTPlan("float32", n, b)
+    _settle(lambda: plan.execute(x, 1), 0.3)
+    ms = time_events(lambda: plan.execute(x, 1), 20)
+    byt = 2.0 * n * 4 * b
+    res["realft_f32_4096x2p15_fwd"] = {
+        "ms": round(ms, 4), "GBps": round(byt / (ms * 1e-3) / 1e9, 1),
+        "frac_hbm": round(byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "GFLOPs": round(2.5 * n * 15 * b / (ms * 1e-3) / 1e9, 1), "launches": plan.num_launches}
+    del plan, x
+    b, n, m = 2048, 1 << 15, 257
+    d = torch.empty(b * n, dtype=torch.float32, device="cuda")
+    fill_normal(d, 4, b * n)
+    r = torch.from_numpy(synth.gaussian_windowed_responses(4, b, m, 40.0, start=b * n).astype(np.float32)).cuda()
+    a = torch.empty_like(d)
+    plan = numrs_amd.ConvlvPlan("float32", n, m, b)
+    _settle(lambda: plan.execute(d, r, a, 1), 0.3)
+    ms = time_events(lambda: plan.execute(d, r, a, 1), 20)
+    byt = (2.0 * n + m) * 4 * b
+    res["convlv_f32_2048x2p15_m257"] = {
+        "ms": round(ms, 4), "GBps": round(byt / (ms * 1e-3) / 1e9, 1),
+        "frac_hbm": round(byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "launches": plan.num_launches}
+    del plan, d, r, a
+    torch.cuda.empty_cache()
     return res
 
 
